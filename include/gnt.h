@@ -1,0 +1,114 @@
+/*
+ * gnt — Goldilocks-NTT big-integer multiply on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of libgnt.so.  Plain pointers and sizes only; limb arrays are
+ * little-endian uint64 magnitudes (the reference's `&[u64]` / `Vec<u64>`).
+ * Every entry point returns 0 on success or a negative GNT_E_* code and never
+ * aborts; gnt_last_error() gives a thread-local message.  A context is bound
+ * to one device and is safe to share between host threads.
+ *
+ * Reference interface (joelodom/pi, crates/bignum):
+ *   ntt.rs:806        pub(crate) fn mul_mag_ntt(a: &[u64], b: &[u64]) -> Vec<u64>
+ *   integer.rs:460    fn mul_mag_dispatch(a, b, ntt_max_coeffs, ntt_threshold, karatsuba_threshold)
+ *   integer.rs:480    the only production call of mul_mag_ntt (guarded by ntt_fits && smaller >= ntt_threshold)
+ */
+#ifndef GNT_H
+#define GNT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GNT_OK 0
+#define GNT_E_ARG (-1)       /* bad argument (null pointer, capacity too small, unknown handle) */
+#define GNT_E_TOO_LARGE (-2) /* 4*(na+nb) exceeds the transform cap: caller splits (integer.rs:479-485) */
+#define GNT_E_NOMEM (-3)     /* device memory or the HBM admission cap exhausted */
+#define GNT_E_DEVICE (-4)    /* HIP runtime error */
+#define GNT_E_TIMEOUT (-5)   /* gnt_wait timed out; the request is still in flight */
+#define GNT_E_STATE (-6)     /* context shut down, or serving loop not running */
+
+typedef struct gnt_ctx gnt_ctx;
+typedef struct gnt_buf gnt_buf; /* device-resident limb array owned by the library */
+
+typedef struct gnt_config {
+    int device;                 /* HIP device ordinal */
+    size_t hbm_cap_bytes;       /* admission cap for serving/batches; 0 = 200 GB */
+    const char *plan_cache_path;/* autotuner plan cache (JSON); NULL = built-in plans */
+    size_t reserved[4];
+} gnt_config;
+
+/* ---- lifetime ---- */
+const char *gnt_version(void);
+const char *gnt_last_error(void);
+/* Replaces bignum::config::apply (config.rs:148) for this path: one context per device. */
+int gnt_init(gnt_ctx **ctx, const gnt_config *cfg);
+int gnt_shutdown(gnt_ctx *ctx);
+
+/* ---- single products (host buffers) ----
+ * gnt_mul replaces ntt::mul_mag_ntt (ntt.rs:806-865): out receives |a|*|b|
+ * trimmed of most-significant zero limbs (ntt.rs:712-715, 781-783);
+ * out_cap >= na+nb.  na == 0 or nb == 0 gives *out_len = 0 (ntt.rs:807-809).
+ * GNT_E_TOO_LARGE when 4*(na+nb) > 2^30 coefficients (the reference's cap is
+ * 2^32, integer.rs:476-478; the caller splits with Karatsuba either way). */
+int gnt_mul(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out,
+            size_t out_cap, size_t *out_len);
+/* gnt_sqr replaces the reference's squares, which all go through mul_mag(a, a)
+ * (float.rs:182, integer.rs:78,95, gauss_legendre.rs:130); one forward
+ * transform instead of two, same product.  out_cap >= 2*na. */
+int gnt_sqr(gnt_ctx *ctx, const uint64_t *a, size_t na, uint64_t *out, size_t out_cap, size_t *out_len);
+
+/* mul_mag / mul_mag_dispatch (integer.rs:445-486) with the GPU multiply as
+ * its NTT leaf: schoolbook below karatsuba_threshold, Karatsuba (recursing
+ * through this dispatch) in between and above the transform cap, the GPU
+ * product when it fits and min(na, nb) >= ntt_threshold.  Zero thresholds
+ * select the reference defaults (8192, 32).  out_cap >= na+nb+1. */
+int gnt_mul_mag(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out,
+                size_t out_cap, size_t *out_len, size_t ntt_threshold, size_t karatsuba_threshold);
+
+/* ---- device-resident operands (no reference counterpart: the reference has
+ * no device; these keep operands and results in HBM between dependent
+ * products, as LimbStorage (storage.rs:33-44) keeps them in host memory) ---- */
+int gnt_buf_upload(gnt_ctx *ctx, const uint64_t *limbs, size_t n, gnt_buf **out);
+int gnt_buf_download(gnt_ctx *ctx, gnt_buf *buf, uint64_t *out, size_t cap, size_t *out_len);
+int gnt_buf_len(gnt_ctx *ctx, gnt_buf *buf, size_t *len); /* trimmed length; waits for the producer */
+int gnt_buf_release(gnt_ctx *ctx, gnt_buf *buf);
+/* Product of two device-resident operands into a new buffer (a == b squares). */
+int gnt_mul_buf(gnt_ctx *ctx, gnt_buf *a, gnt_buf *b, gnt_buf **out);
+int gnt_sqr_buf(gnt_ctx *ctx, gnt_buf *a, gnt_buf **out);
+/* Wait until all work queued on the context's stream has finished. */
+int gnt_sync(gnt_ctx *ctx);
+
+/* ---- batched products (within a binary-splitting node the reference runs its
+ * four products one after another, chudnovsky.rs:411-413; independent
+ * subtrees run concurrently through rayon::join, chudnovsky.rs:397-408) ---- */
+typedef struct gnt_job {
+    const uint64_t *a;
+    size_t na;
+    const uint64_t *b; /* ignored when is_sqr */
+    size_t nb;
+    uint64_t *out;
+    size_t out_cap;
+    size_t out_len;  /* written */
+    int is_sqr;
+    int status;      /* written: GNT_OK or an error code */
+} gnt_job;
+/* All jobs in as few launches as the plans allow; returns when all finished. */
+int gnt_mul_batch(gnt_ctx *ctx, gnt_job *jobs, size_t n);
+/* Same jobs, one product at a time (the unbatched path, for comparison). */
+int gnt_mul_batch_serial(gnt_ctx *ctx, gnt_job *jobs, size_t n);
+
+/* ---- introspection ---- */
+/* Transform plan the library uses for an na x nb product. */
+int gnt_plan_info(size_t na, size_t nb, int *log_n, int *npass, int *log_r /* [4] */);
+/* Per-phase device time (ms) accumulated while timing is on:
+ * [0] forward passes, [1] pointwise, [2] inverse passes, [3] carry, [4] whole products. */
+int gnt_set_timing(gnt_ctx *ctx, int on);
+int gnt_get_timing(gnt_ctx *ctx, double *ms, int n, uint64_t *products);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

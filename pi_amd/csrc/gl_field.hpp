@@ -1,0 +1,169 @@
+// Goldilocks field arithmetic, p = 2^64 - 2^32 + 1, for gfx950 (and the host,
+// so the same code is unit-tested on the CPU).
+//
+// Elements are canonical u64 in [0, p).  Products are built from 32x32->64
+// pieces (v_mad_u64_u32 on CDNA) and reduced with 2^64 = 2^32 - 1 (mod p) and
+// 2^96 = -1 (mod p).  The reduction identity is the one the reference's
+// `reduce128` uses (crates/bignum/src/ntt.rs:218-252); the instruction
+// sequence here is written for a 32-bit VALU with carry flags.
+//
+// Roots of unity: 2 has multiplicative order 192 (2^96 = -1), so 2^(192/m) is
+// a primitive m-th root for m | 64.  This library picks its 2^32-th root r
+// with r^(2^26) = 8, so that every transform's small-order roots are powers
+// of two and their multiplications are shifts.  (The reference uses
+// 7^((p-1)/n); products are independent of that choice.)
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define GL_HD __host__ __device__ __forceinline__
+#else
+#define GL_HD static inline
+#endif
+
+namespace gl {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+constexpr u64 P = 0xFFFFFFFF00000001ull;
+constexpr u64 EPS = 0xFFFFFFFFull;  // 2^64 mod p
+
+// (a + b) mod p for a, b in [0, p).  s = a + b; t = s - p = s + EPS (mod 2^64).
+// The answer is t when a + b >= p, i.e. when either add carried.
+GL_HD u64 add(u64 a, u64 b) {
+    u64 s = a + b;
+    bool c1 = s < a;
+    u64 t = s + EPS;
+    bool c2 = t < s;
+    return (c1 | c2) ? t : s;
+}
+
+// (a - b) mod p for a, b in [0, p).  On borrow add p, i.e. subtract EPS.
+GL_HD u64 sub(u64 a, u64 b) {
+    u64 d = a - b;
+    return (a < b) ? d - EPS : d;
+}
+
+GL_HD u64 neg(u64 a) { return sub(0, a); }
+
+// Reduce hi*2^64 + lo (any 128-bit value) to [0, p):
+//   x = lo + (hh*2^32 + hl)*2^64 = lo - hh + hl*(2^32 - 1)  (mod p)
+GL_HD u64 reduce128(u64 lo, u64 hi) {
+    u64 hh = hi >> 32;
+    u64 hl = hi & 0xFFFFFFFFull;
+    u64 t0 = lo - hh;
+    if (lo < hh) t0 -= EPS;             // lo - hh + p; cannot underflow
+    u64 t1 = (hl << 32) - hl;           // hl * EPS < p
+    u64 s = t0 + t1;
+    bool c1 = s < t0;                   // wrapped: true value s + 2^64 = s + EPS (mod p)
+    u64 t = s + EPS;                    // s - p (mod 2^64); also s + 2^64 - p
+    bool c2 = t < s;                    // s >= p
+    return (c1 | c2) ? t : s;           // both branches land in [0, p)
+}
+
+// Full 64x64 -> 128 product from four 32x32 -> 64 pieces.
+GL_HD void mul_wide(u64 a, u64 b, u64 &lo, u64 &hi) {
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 p00 = (u64)a0 * b0;
+    u64 mid = (u64)a0 * b1 + (p00 >> 32);       // < 2^64
+    u64 mid2 = (u64)a1 * b0 + (u32)mid;          // < 2^64
+    lo = (mid2 << 32) | (u32)p00;
+    hi = (u64)a1 * b1 + (mid >> 32) + (mid2 >> 32);
+}
+
+GL_HD u64 mul(u64 a, u64 b) {
+    u64 lo, hi;
+    mul_wide(a, b, lo, hi);
+    return reduce128(lo, hi);
+}
+
+// x * 2^f mod p for 0 < f < 64: the shifted value is hi*2^64 + lo, hi < 2^f.
+GL_HD u64 shl_reduce(u64 x, int f) {
+    return reduce128(x << f, x >> (64 - f));
+}
+
+// 2^e mod p for 0 <= e < 192, as a plain constant.
+GL_HD u64 pow2_const(int e) {
+    // 2^e for e < 64 is exact; beyond, reduce by doubling.
+    u64 r = 1;
+    for (int i = 0; i < e; i++) r = add(r, r);
+    return r;
+}
+
+// x * 2^e mod p, 0 <= e < 192.  With e a compile-time constant (after
+// unrolling) every branch folds away.
+GL_HD u64 mul_pow2(u64 x, int e) {
+    bool negate = e >= 96;
+    int f = negate ? e - 96 : e;
+    u64 r;
+    if (f == 0) r = x;
+    else if (f < 64) r = shl_reduce(x, f);
+    else r = shl_reduce(shl_reduce(x, f - 48), 48);
+    return negate ? neg(r) : r;
+}
+
+GL_HD u64 pow(u64 base, u64 exp) {
+    u64 acc = 1;
+    while (exp) {
+        if (exp & 1) acc = mul(acc, base);
+        exp >>= 1;
+        if (exp) base = mul(base, base);
+    }
+    return acc;
+}
+
+GL_HD u64 inv(u64 a) { return pow(a, P - 2); }
+
+GL_HD u32 bitrev(u32 x, int bits) {
+    u32 r = 0;
+    for (int i = 0; i < bits; i++) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident small DFTs.  R in {2, 4, 8, 16, 32, 64}; omega_R = 2^(192/R).
+// ---------------------------------------------------------------------------
+
+// Forward DIF: natural-order input, output register q holds frequency
+// bitrev_R(q).  Sum_i v[i] * omega_R^(i*k).
+template <int R>
+GL_HD void dft_dif(u64 *v) {
+#pragma unroll
+    for (int m = R; m >= 2; m >>= 1) {
+        const int h = m >> 1;
+#pragma unroll
+        for (int b = 0; b < R; b += m) {
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                u64 u = v[b + j], w = v[b + j + h];
+                v[b + j] = add(u, w);
+                v[b + j + h] = mul_pow2(sub(u, w), (192 / m) * j);
+            }
+        }
+    }
+}
+
+// Inverse DIT (unscaled): register q holds frequency bitrev_R(q); output is
+// natural order, v[i] = Sum_k V[k] * omega_R^(-i*k)  (R times the inverse DFT).
+template <int R>
+GL_HD void dft_dit_inv(u64 *v) {
+#pragma unroll
+    for (int m = 2; m <= R; m <<= 1) {
+        const int h = m >> 1;
+#pragma unroll
+        for (int b = 0; b < R; b += m) {
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                u64 u = v[b + j];
+                u64 w = mul_pow2(v[b + j + h], (192 - (192 / m) * j) % 192);
+                v[b + j] = add(u, w);
+                v[b + j + h] = sub(u, w);
+            }
+        }
+    }
+}
+
+}  // namespace gl

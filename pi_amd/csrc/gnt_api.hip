@@ -1,0 +1,486 @@
+// C ABI of libgnt.so: contexts, device memory pool, single / device-resident /
+// batched products and the host-side multiply dispatch.  See include/gnt.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gnt.h"
+#include "gl_field.hpp"
+#include "gnt_kernels.hpp"
+#include "gnt_plan.hpp"
+#include "gnt_roots.hpp"
+#include "gnt_runtime.hpp"
+
+using gl::u32;
+using gl::u64;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(GNT_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int ceil_log2(size_t x) {
+    int l = 0;
+    while (((size_t)1 << l) < x) l++;
+    return l;
+}
+
+}  // namespace
+
+namespace gnt {
+
+int set_error(int code, const char *msg) {
+    g_last_error = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------
+// Device memory pool: exact-size free lists, at most 4 buffers per size and 1
+// per size >= 1 GiB (the reference's scratch pool policy, ntt.rs:51-155).
+// Buffers are handed back in stream order, so reuse by later work on the same
+// stream is safe without a sync.
+// ---------------------------------------------------------------------------
+void *Pool::acquire(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = free_.find(bytes);
+        if (it != free_.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            pooled_bytes -= bytes;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        // give cached buffers back to the device and try once more
+        trim();
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    }
+    std::lock_guard<std::mutex> g(mu);
+    live_bytes += bytes;
+    return p;
+}
+
+void Pool::release(void *p, size_t bytes) {
+    if (!p) return;
+    bytes = (bytes + 255) & ~(size_t)255;
+    std::lock_guard<std::mutex> g(mu);
+    auto &v = free_[bytes];
+    size_t cap = bytes >= ((size_t)1 << 30) ? 1 : 4;
+    if (v.size() < cap) {
+        v.push_back(p);
+        pooled_bytes += bytes;
+    } else {
+        (void)hipFree(p);
+        live_bytes -= bytes;
+    }
+}
+
+void Pool::trim() {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto &kv : free_)
+        for (void *p : kv.second) {
+            (void)hipFree(p);
+            live_bytes -= kv.first;
+            pooled_bytes -= kv.first;
+        }
+    free_.clear();
+}
+
+int Ctx::init_tables() {
+    std::vector<u64> lo, hi;
+    if (!build_root_tables(lo, hi)) return set_error(GNT_E_DEVICE, "root table construction failed");
+    const size_t T = lo.size();
+    HIP_TRY(hipMalloc(&wlo, T * sizeof(u64)));
+    HIP_TRY(hipMalloc(&whi, T * sizeof(u64)));
+    HIP_TRY(hipMemcpy(wlo, lo.data(), T * sizeof(u64), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(whi, hi.data(), T * sizeof(u64), hipMemcpyHostToDevice));
+    return GNT_OK;
+}
+
+bool plan_for(size_t na, size_t nb, Plan &pl) {
+    size_t n_out = 4 * (na + nb) - 1;
+    int log_n = ceil_log2(n_out);
+    if (log_n > MAX_LOG_N) return false;
+    pl = make_plan(log_n);
+    return true;
+}
+
+// Enqueue one product of device limb arrays on ctx->stream.  b == nullptr
+// squares a.  out holds na+nb limbs; *len receives the trimmed length.
+int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64 *dout, u32 *dlen) {
+    const bool sqr = db == nullptr;
+    if (sqr) nb = na;
+    Plan pl;
+    if (!plan_for(na, nb, pl)) return set_error(GNT_E_TOO_LARGE, "product exceeds the transform cap");
+    const size_t N = (size_t)1 << pl.log_n;
+    const size_t n_out = 4 * (na + nb) - 1;
+    const size_t n_limbs = na + nb;
+    const size_t nt = carry_tiles(n_limbs);
+    u64 *xa = (u64 *)pool.acquire(N * sizeof(u64));
+    u64 *xb = sqr ? nullptr : (u64 *)pool.acquire(N * sizeof(u64));
+    unsigned char *gp = (unsigned char *)pool.acquire(2 * nt);
+    if (!xa || (!sqr && !xb) || !gp) {
+        pool.release(xa, N * sizeof(u64));
+        pool.release(xb, N * sizeof(u64));
+        pool.release(gp, 2 * nt);
+        return set_error(GNT_E_NOMEM, "device memory exhausted");
+    }
+    hipEvent_t ev[5];
+    if (timing) {
+        for (auto &e : ev) (void)hipEventCreate(&e);
+        (void)hipEventRecord(ev[0], stream);
+    }
+    int rc = launch_forward(pl, xa, da, na, wlo, whi, stream);
+    if (!rc && !sqr) rc = launch_forward(pl, xb, db, nb, wlo, whi, stream);
+    if (timing) (void)hipEventRecord(ev[1], stream);
+    if (!rc) rc = launch_pointwise(xa, sqr ? xa : xb, pl.log_n, stream);
+    if (timing) (void)hipEventRecord(ev[2], stream);
+    if (!rc) rc = launch_inverse(pl, xa, wlo, whi, stream);
+    if (timing) (void)hipEventRecord(ev[3], stream);
+    if (!rc) rc = launch_carry(xa, n_out, dout, n_limbs, gp, gp + nt, dlen, stream);
+    if (timing) (void)hipEventRecord(ev[4], stream);
+    pool.release(xa, N * sizeof(u64));
+    pool.release(xb, N * sizeof(u64));
+    pool.release(gp, 2 * nt);
+    if (timing) {
+        (void)hipEventSynchronize(ev[4]);
+        float t;
+        (void)hipEventElapsedTime(&t, ev[0], ev[1]); phase_ms[0] += t;
+        (void)hipEventElapsedTime(&t, ev[1], ev[2]); phase_ms[1] += t;
+        (void)hipEventElapsedTime(&t, ev[2], ev[3]); phase_ms[2] += t;
+        (void)hipEventElapsedTime(&t, ev[3], ev[4]); phase_ms[3] += t;
+        (void)hipEventElapsedTime(&t, ev[0], ev[4]); phase_ms[4] += t;
+        products++;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    if (rc) return set_error(GNT_E_DEVICE, "kernel launch failed");
+    return GNT_OK;
+}
+
+}  // namespace gnt
+
+using gnt::Ctx;
+
+struct gnt_buf {
+    u64 *d = nullptr;
+    size_t cap = 0;      // limbs allocated (untrimmed length of the value)
+    u32 *dlen = nullptr; // device-side trimmed length
+    size_t len = 0;
+    bool len_known = false;
+};
+
+extern "C" {
+
+const char *gnt_version(void) { return "gnt 0.1 (gfx950)"; }
+const char *gnt_last_error(void) { return g_last_error.c_str(); }
+
+int gnt_init(gnt_ctx **out, const gnt_config *cfg) {
+    if (!out) return fail(GNT_E_ARG, "null ctx pointer");
+    int dev = cfg ? cfg->device : 0;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(GNT_E_DEVICE, "no HIP device");
+    if (dev < 0 || dev >= count) return fail(GNT_E_ARG, "device %d out of range", dev);
+    HIP_TRY(hipSetDevice(dev));
+    Ctx *c = new Ctx();
+    c->device = dev;
+    c->hbm_cap = (cfg && cfg->hbm_cap_bytes) ? cfg->hbm_cap_bytes : (size_t)200 * 1000 * 1000 * 1000;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(GNT_E_DEVICE, "stream creation failed");
+    }
+    int rc = c->init_tables();
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = reinterpret_cast<gnt_ctx *>(c);
+    return GNT_OK;
+}
+
+int gnt_shutdown(gnt_ctx *ctx) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return GNT_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->pool.trim();
+    (void)hipFree(c->wlo);
+    (void)hipFree(c->whi);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GNT_OK;
+}
+
+static int check_ctx(Ctx *c) {
+    if (!c) return fail(GNT_E_ARG, "null context");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(GNT_E_DEVICE, "hipSetDevice failed");
+    return GNT_OK;
+}
+
+// One product of host operands (b == nullptr squares a).
+static int host_product(Ctx *c, const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, size_t out_cap,
+                        size_t *out_len) {
+    const bool sqr = b == nullptr;
+    if (sqr) nb = na;
+    if (!out_len) return fail(GNT_E_ARG, "null out_len");
+    if (na == 0 || nb == 0) {
+        *out_len = 0;
+        return GNT_OK;
+    }
+    if (!a || (!sqr && !b) || !out) return fail(GNT_E_ARG, "null operand");
+    if (out_cap < na + nb) return fail(GNT_E_ARG, "out_cap %zu < %zu", out_cap, na + nb);
+    gnt::Plan pl;
+    if (!gnt::plan_for(na, nb, pl)) return fail(GNT_E_TOO_LARGE, "4*(na+nb) exceeds 2^%d", gnt::MAX_LOG_N);
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t nl = na + nb;
+    u64 *da = (u64 *)c->pool.acquire(na * 8);
+    u64 *db = sqr ? nullptr : (u64 *)c->pool.acquire(nb * 8);
+    u64 *dout = (u64 *)c->pool.acquire(nl * 8);
+    u32 *dlen = (u32 *)c->pool.acquire(sizeof(u32));
+    int rc = GNT_OK;
+    if (!da || (!sqr && !db) || !dout || !dlen) rc = fail(GNT_E_NOMEM, "device memory exhausted");
+    u32 len = 0;
+    if (!rc && hipMemcpyAsync(da, a, na * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(GNT_E_DEVICE, "upload failed");
+    if (!rc && !sqr && hipMemcpyAsync(db, b, nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(GNT_E_DEVICE, "upload failed");
+    if (!rc) rc = c->enqueue_product(da, na, db, nb, dout, dlen);
+    if (!rc && hipMemcpyAsync(&len, dlen, sizeof(u32), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(GNT_E_DEVICE, "download failed");
+    if (!rc && hipMemcpyAsync(out, dout, nl * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(GNT_E_DEVICE, "download failed");
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (!rc && se != hipSuccess) rc = fail(GNT_E_DEVICE, "stream: %s", hipGetErrorString(se));
+    c->pool.release(da, na * 8);
+    c->pool.release(db, nb * 8);
+    c->pool.release(dout, nl * 8);
+    c->pool.release(dlen, sizeof(u32));
+    if (rc) return rc;
+    *out_len = len;
+    return GNT_OK;
+}
+
+int gnt_mul(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out,
+            size_t out_cap, size_t *out_len) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    if (na && nb && !b) return fail(GNT_E_ARG, "null operand");
+    return host_product(c, a, na, b, nb, out, out_cap, out_len);
+}
+
+int gnt_sqr(gnt_ctx *ctx, const uint64_t *a, size_t na, uint64_t *out, size_t out_cap, size_t *out_len) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    return host_product(c, a, na, nullptr, na, out, out_cap, out_len);
+}
+
+// ---- device-resident operands ----
+
+static gnt_buf *new_buf(Ctx *c, size_t n) {
+    gnt_buf *b = new gnt_buf();
+    b->cap = n;
+    b->d = n ? (u64 *)c->pool.acquire(n * 8) : nullptr;
+    b->dlen = (u32 *)c->pool.acquire(sizeof(u32));
+    if ((n && !b->d) || !b->dlen) {
+        c->pool.release(b->d, n * 8);
+        c->pool.release(b->dlen, sizeof(u32));
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+int gnt_buf_upload(gnt_ctx *ctx, const uint64_t *limbs, size_t n, gnt_buf **out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    if (!out || (n && !limbs)) return fail(GNT_E_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    size_t len = n;
+    while (len > 0 && limbs[len - 1] == 0) len--;
+    gnt_buf *b = new_buf(c, len);
+    if (!b) return fail(GNT_E_NOMEM, "device memory exhausted");
+    if (len) HIP_TRY(hipMemcpyAsync(b->d, limbs, len * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    b->len = len;
+    b->len_known = true;
+    *out = b;
+    return GNT_OK;
+}
+
+static int buf_len_locked(Ctx *c, gnt_buf *b, size_t *len) {
+    if (!b->len_known) {
+        u32 l = 0;
+        HIP_TRY(hipMemcpyAsync(&l, b->dlen, sizeof(u32), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        b->len = l;
+        b->len_known = true;
+    }
+    *len = b->len;
+    return GNT_OK;
+}
+
+int gnt_buf_len(gnt_ctx *ctx, gnt_buf *b, size_t *len) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    if (!b || !len) return fail(GNT_E_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return buf_len_locked(c, b, len);
+}
+
+int gnt_buf_download(gnt_ctx *ctx, gnt_buf *b, uint64_t *out, size_t cap, size_t *out_len) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    if (!b || !out_len) return fail(GNT_E_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    size_t len;
+    if (int rc = buf_len_locked(c, b, &len)) return rc;
+    if (cap < len || (len && !out)) return fail(GNT_E_ARG, "capacity %zu < %zu", cap, len);
+    if (len) {
+        HIP_TRY(hipMemcpyAsync(out, b->d, len * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    *out_len = len;
+    return GNT_OK;
+}
+
+int gnt_buf_release(gnt_ctx *ctx, gnt_buf *b) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    if (!b) return GNT_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    c->pool.release(b->d, b->cap * 8);
+    c->pool.release(b->dlen, sizeof(u32));
+    delete b;
+    return GNT_OK;
+}
+
+static int buf_product(Ctx *c, gnt_buf *a, gnt_buf *b, gnt_buf **out) {
+    if (!a || !out) return fail(GNT_E_ARG, "null argument");
+    const bool sqr = (b == nullptr || b == a);
+    // Operand lengths: the trimmed length when known, else the capacity (the
+    // extra top limbs are zeros and leave the product unchanged).
+    size_t na = a->len_known ? a->len : a->cap;
+    size_t nb = sqr ? na : (b->len_known ? b->len : b->cap);
+    if (na == 0 || nb == 0) {
+        gnt_buf *r = new_buf(c, 0);
+        if (!r) return fail(GNT_E_NOMEM, "device memory exhausted");
+        r->len = 0;
+        r->len_known = true;
+        *out = r;
+        return GNT_OK;
+    }
+    gnt::Plan pl;
+    if (!gnt::plan_for(na, nb, pl)) return fail(GNT_E_TOO_LARGE, "4*(na+nb) exceeds 2^%d", gnt::MAX_LOG_N);
+    gnt_buf *r = new_buf(c, na + nb);
+    if (!r) return fail(GNT_E_NOMEM, "device memory exhausted");
+    int rc = c->enqueue_product(a->d, na, sqr ? nullptr : b->d, nb, r->d, r->dlen);
+    if (rc) {
+        c->pool.release(r->d, r->cap * 8);
+        c->pool.release(r->dlen, sizeof(u32));
+        delete r;
+        return rc;
+    }
+    *out = r;
+    return GNT_OK;
+}
+
+int gnt_mul_buf(gnt_ctx *ctx, gnt_buf *a, gnt_buf *b, gnt_buf **out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    if (!b) return fail(GNT_E_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return buf_product(c, a, b, out);
+}
+
+int gnt_sqr_buf(gnt_ctx *ctx, gnt_buf *a, gnt_buf **out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
+    return buf_product(c, a, nullptr, out);
+}
+
+int gnt_sync(gnt_ctx *ctx) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GNT_OK;
+}
+
+// ---- batches ----
+
+int gnt_mul_batch_serial(gnt_ctx *ctx, gnt_job *jobs, size_t n) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = check_ctx(c)) return rc;
+    if (n && !jobs) return fail(GNT_E_ARG, "null jobs");
+    int first = GNT_OK;
+    for (size_t i = 0; i < n; i++) {
+        gnt_job &j = jobs[i];
+        j.status = host_product(c, j.a, j.na, j.is_sqr ? nullptr : j.b, j.is_sqr ? j.na : j.nb, j.out, j.out_cap,
+                                &j.out_len);
+        if (j.status && !first) first = j.status;
+    }
+    return first;
+}
+
+int gnt_mul_batch(gnt_ctx *ctx, gnt_job *jobs, size_t n) {
+    return gnt::run_batch(reinterpret_cast<Ctx *>(ctx), jobs, n);
+}
+
+// ---- introspection ----
+
+int gnt_plan_info(size_t na, size_t nb, int *log_n, int *npass, int *log_r) {
+    if (na == 0 || nb == 0) return fail(GNT_E_ARG, "empty operand");
+    gnt::Plan pl;
+    if (!gnt::plan_for(na, nb, pl)) return fail(GNT_E_TOO_LARGE, "too large");
+    if (log_n) *log_n = pl.log_n;
+    if (npass) *npass = pl.npass;
+    if (log_r)
+        for (int p = 0; p < 4; p++) log_r[p] = p < pl.npass ? pl.pass[p].log_r : 0;
+    return GNT_OK;
+}
+
+int gnt_set_timing(gnt_ctx *ctx, int on) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return fail(GNT_E_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->timing = on != 0;
+    for (double &t : c->phase_ms) t = 0;
+    c->products = 0;
+    return GNT_OK;
+}
+
+int gnt_get_timing(gnt_ctx *ctx, double *ms, int n, uint64_t *products) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return fail(GNT_E_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    for (int i = 0; i < n && i < 5; i++) ms[i] = c->phase_ms[i];
+    if (products) *products = c->products;
+    return GNT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+// Host-side multiply dispatch with the GPU product as its transform leaf, and
+// the batched entry point.
+//
+// gnt_mul_mag restates bignum's mul_mag / mul_mag_dispatch
+// (crates/bignum/src/integer.rs:445-486): schoolbook below the Karatsuba
+// threshold, Karatsuba in between, the transform product when it fits and the
+// smaller operand reaches the NTT threshold, and Karatsuba above the transform
+// cap so that oversized products become several transform leaves
+// (integer.rs:433-444, 552-608).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/gnt.h"
+#include "gnt_kernels.hpp"
+#include "gnt_runtime.hpp"
+
+typedef unsigned __int128 u128;
+using u64 = uint64_t;
+
+namespace gnt {
+namespace {
+
+struct Limbs {
+    std::vector<u64> v;
+    size_t len() const { return v.size(); }
+    void trim() {
+        while (!v.empty() && v.back() == 0) v.pop_back();
+    }
+};
+
+// integer.rs:492-518
+std::vector<u64> schoolbook(const u64 *a, size_t na, const u64 *b, size_t nb) {
+    std::vector<u64> out(na + nb, 0);
+    for (size_t i = 0; i < na; i++) {
+        u128 carry = 0;
+        u128 av = a[i];
+        for (size_t j = 0; j < nb; j++) {
+            u128 cur = (u128)out[i + j] + av * b[j] + carry;
+            out[i + j] = (u64)cur;
+            carry = cur >> 64;
+        }
+        size_t k = i + nb;
+        while (carry) {
+            u128 cur = (u128)out[k] + carry;
+            out[k] = (u64)cur;
+            carry = cur >> 64;
+            k++;
+        }
+    }
+    while (!out.empty() && out.back() == 0) out.pop_back();
+    return out;
+}
+
+// integer.rs:351-366 (length not normalised, as in the reference)
+std::vector<u64> add_mag(const u64 *a, size_t na, const u64 *b, size_t nb) {
+    const u64 *lg = na >= nb ? a : b, *sh = na >= nb ? b : a;
+    size_t nl = na >= nb ? na : nb, ns = na >= nb ? nb : na;
+    std::vector<u64> out;
+    out.reserve(nl + 1);
+    u128 carry = 0;
+    for (size_t i = 0; i < nl; i++) {
+        u128 s = (u128)lg[i] + (i < ns ? sh[i] : 0) + carry;
+        out.push_back((u64)s);
+        carry = s >> 64;
+    }
+    if (carry) out.push_back((u64)carry);
+    return out;
+}
+
+// integer.rs:369-389, a >= b
+std::vector<u64> sub_mag(const std::vector<u64> &a, const std::vector<u64> &b) {
+    std::vector<u64> out(a.size());
+    u64 borrow = 0;
+    for (size_t i = 0; i < a.size(); i++) {
+        u64 bi = i < b.size() ? b[i] : 0;
+        u64 d1 = a[i] - bi;
+        u64 b1 = a[i] < bi;
+        u64 d2 = d1 - borrow;
+        u64 b2 = d1 < borrow;
+        out[i] = d2;
+        borrow = b1 | b2;
+    }
+    while (!out.empty() && out.back() == 0) out.pop_back();
+    return out;
+}
+
+// integer.rs:612-635
+void add_at(std::vector<u64> &dest, const std::vector<u64> &src, size_t off) {
+    if (src.empty()) return;
+    if (dest.size() < off + src.size()) dest.resize(off + src.size(), 0);
+    u128 carry = 0;
+    for (size_t i = 0; i < src.size(); i++) {
+        u128 s = (u128)dest[off + i] + src[i] + carry;
+        dest[off + i] = (u64)s;
+        carry = s >> 64;
+    }
+    size_t k = off + src.size();
+    while (carry) {
+        if (dest.size() <= k) dest.push_back(0);
+        u128 s = (u128)dest[k] + carry;
+        dest[k] = (u64)s;
+        carry = s >> 64;
+        k++;
+    }
+}
+
+struct Dispatch {
+    gnt_ctx *ctx;
+    size_t max_coeffs;
+    size_t ntt_thr;
+    size_t kara_thr;
+    int err = GNT_OK;
+
+    std::vector<u64> gpu(const u64 *a, size_t na, const u64 *b, size_t nb) {
+        std::vector<u64> out(na + nb);
+        size_t len = 0;
+        int rc = (a == b && na == nb) ? gnt_sqr(ctx, a, na, out.data(), out.size(), &len)
+                                      : gnt_mul(ctx, a, na, b, nb, out.data(), out.size(), &len);
+        if (rc) {
+            err = rc;
+            return {};
+        }
+        out.resize(len);
+        return out;
+    }
+
+    std::vector<u64> karatsuba(const u64 *a, size_t na, const u64 *b, size_t nb) {
+        size_t n = na > nb ? na : nb;
+        size_t m = n / 2;
+        size_t la = m < na ? m : na, lb = m < nb ? m : nb;
+        std::vector<u64> z0 = mul(a, la, b, lb);
+        std::vector<u64> z2 = mul(a + la, na - la, b + lb, nb - lb);
+        std::vector<u64> as = add_mag(a, la, a + la, na - la);
+        std::vector<u64> bs = add_mag(b, lb, b + lb, nb - lb);
+        std::vector<u64> z1p = mul(as.data(), as.size(), bs.data(), bs.size());
+        if (err) return {};
+        std::vector<u64> z02 = add_mag(z0.data(), z0.size(), z2.data(), z2.size());
+        std::vector<u64> z1 = sub_mag(z1p, z02);
+        std::vector<u64> out(na + nb + 1, 0);
+        add_at(out, z0, 0);
+        add_at(out, z1, m);
+        add_at(out, z2, 2 * m);
+        while (!out.empty() && out.back() == 0) out.pop_back();
+        return out;
+    }
+
+    // integer.rs:460-486
+    std::vector<u64> mul(const u64 *a, size_t na, const u64 *b, size_t nb) {
+        if (na == 0 || nb == 0 || err) return {};
+        size_t smaller = na < nb ? na : nb;
+        bool fits = (na + nb) * 4 <= max_coeffs;
+        if (fits && smaller >= ntt_thr) return gpu(a, na, b, nb);
+        if (smaller < kara_thr) return schoolbook(a, na, b, nb);
+        return karatsuba(a, na, b, nb);
+    }
+};
+
+}  // namespace
+
+int run_batch(Ctx *c, gnt_job *jobs, size_t n) {
+    if (!c) return set_error(GNT_E_ARG, "null context");
+    if (n && !jobs) return set_error(GNT_E_ARG, "null jobs");
+    if (hipSetDevice(c->device) != hipSuccess) return set_error(GNT_E_DEVICE, "hipSetDevice failed");
+    std::lock_guard<std::mutex> g(c->mu);
+    struct Slot {
+        u64 *da = nullptr, *db = nullptr, *dout = nullptr;
+        uint32_t *dlen = nullptr;
+        uint32_t len = 0;
+        size_t na = 0, nb = 0;
+    };
+    std::vector<Slot> slots(n);
+    int first = GNT_OK;
+    for (size_t i = 0; i < n; i++) {
+        gnt_job &j = jobs[i];
+        Slot &s = slots[i];
+        j.status = GNT_OK;
+        j.out_len = 0;
+        const bool sqr = j.is_sqr != 0;
+        s.na = j.na;
+        s.nb = sqr ? j.na : j.nb;
+        if (s.na == 0 || s.nb == 0) continue;
+        Plan pl;
+        if (!j.a || (!sqr && !j.b) || !j.out || j.out_cap < s.na + s.nb) { j.status = GNT_E_ARG; continue; }
+        if (!plan_for(s.na, s.nb, pl)) { j.status = GNT_E_TOO_LARGE; continue; }
+        s.da = (u64 *)c->pool.acquire(s.na * 8);
+        s.db = sqr ? nullptr : (u64 *)c->pool.acquire(s.nb * 8);
+        s.dout = (u64 *)c->pool.acquire((s.na + s.nb) * 8);
+        s.dlen = (uint32_t *)c->pool.acquire(sizeof(uint32_t));
+        if (!s.da || (!sqr && !s.db) || !s.dout || !s.dlen) { j.status = GNT_E_NOMEM; continue; }
+        if (hipMemcpyAsync(s.da, j.a, s.na * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            (!sqr && hipMemcpyAsync(s.db, j.b, s.nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+            j.status = GNT_E_DEVICE;
+            continue;
+        }
+        j.status = c->enqueue_product(s.da, s.na, s.db, s.nb, s.dout, s.dlen);
+        if (j.status) continue;
+        if (hipMemcpyAsync(&s.len, s.dlen, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(j.out, s.dout, (s.na + s.nb) * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            j.status = GNT_E_DEVICE;
+    }
+    hipError_t se = hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < n; i++) {
+        Slot &s = slots[i];
+        if (se != hipSuccess && jobs[i].status == GNT_OK && s.da) jobs[i].status = GNT_E_DEVICE;
+        if (jobs[i].status == GNT_OK) jobs[i].out_len = s.dout ? s.len : 0;
+        c->pool.release(s.da, s.na * 8);
+        c->pool.release(s.db, s.nb * 8);
+        c->pool.release(s.dout, (s.na + s.nb) * 8);
+        c->pool.release(s.dlen, sizeof(uint32_t));
+        if (jobs[i].status && !first) first = jobs[i].status;
+    }
+    if (first) set_error(first, "one or more batch jobs failed");
+    return first;
+}
+
+}  // namespace gnt
+
+extern "C" int gnt_mul_mag(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                           uint64_t *out, size_t out_cap, size_t *out_len, size_t ntt_threshold,
+                           size_t karatsuba_threshold) {
+    if (!ctx || !out_len) return gnt::set_error(GNT_E_ARG, "null argument");
+    if ((na && !a) || (nb && !b)) return gnt::set_error(GNT_E_ARG, "null operand");
+    gnt::Dispatch d{ctx, (size_t)1 << gnt::MAX_LOG_N, ntt_threshold ? ntt_threshold : 8192,
+                    karatsuba_threshold ? karatsuba_threshold : 32};
+    std::vector<u64> r = d.mul(a, na, b, nb);
+    if (d.err) return d.err;
+    if (r.size() > out_cap) return gnt::set_error(GNT_E_ARG, "out_cap too small");
+    if (!r.empty()) std::memcpy(out, r.data(), r.size() * 8);
+    *out_len = r.size();
+    return GNT_OK;
+}

@@ -1,0 +1,74 @@
+// Transform plans shared by the HIP kernels, the host runtime and the CPU
+// emulation used in tests.
+//
+// A length-N transform (N = 2^log_n) is a chain of passes.  Pass p applies a
+// radix-R_p decimation-in-frequency step in the Bailey form over blocks of
+// M_p elements: for every column j < s_p = M_p / R_p of every block, an
+// R_p-point DFT over the elements j + u*s_p (u < R_p), then the twiddle
+// omega_{M_p}^(j*k).  The next pass works on blocks of s_p elements.  All
+// radices are powers of two, so the forward output is in bit-reversed order;
+// the inverse passes run in reverse order and return natural order.
+//
+// A pass keeps a tile of R_p x C_p elements (C_p adjacent columns) in LDS;
+// the R_p-point DFT is done as register sub-steps of radix 2..16.
+#pragma once
+
+#include <stdint.h>
+
+namespace gnt {
+
+constexpr int TILE_LOG = 13;         // 8192 u64 = 64 KiB of LDS per workgroup
+constexpr int MAX_PASSES = 4;
+constexpr int MAX_LOG_N = 30;        // transform length cap of this library
+constexpr int ROOT_LOG = 32;         // twiddle tables hold powers of a 2^32-th root
+constexpr int TW_SPLIT = 16;         // two-level table: 2^16 low x 2^16 high
+
+struct PassDesc {
+    int log_r;   // radix of the pass
+    int log_m;   // block size M of the pass
+    int log_s;   // column stride s = M / R
+    int log_c;   // columns per workgroup tile
+};
+
+struct Plan {
+    int log_n;
+    int npass;
+    PassDesc pass[MAX_PASSES];
+};
+
+// Sub-step radices of an R-point column DFT: split log_r into parts of at
+// most 4 bits, as evenly as possible, larger parts first.
+static inline int substep_count(int log_r) { return (log_r + 3) / 4; }
+static inline int substep_log(int log_r, int i) {
+    int k = substep_count(log_r);
+    int base = log_r / k, extra = log_r % k;
+    return base + (i < extra ? 1 : 0);
+}
+
+static inline Plan make_plan(int log_n) {
+    Plan pl;
+    pl.log_n = log_n;
+    int np = log_n <= TILE_LOG ? 1 : (log_n <= 20 ? 2 : 3);
+    pl.npass = np;
+    int rem = log_n;
+    for (int p = 0; p < np; p++) {
+        int left = np - p;
+        int lr = rem / left;             // earlier passes get the smaller share
+        pl.pass[p].log_r = lr;
+        pl.pass[p].log_m = rem;
+        pl.pass[p].log_s = rem - lr;
+        rem -= lr;
+    }
+    for (int p = 0; p < np; p++) {
+        PassDesc &d = pl.pass[p];
+        int cols = TILE_LOG - d.log_r;
+        if (cols < 0) cols = 0;
+        if (d.log_s > 0 && cols > d.log_s) cols = d.log_s;
+        // a tile never spans more than the whole array
+        if (d.log_r + cols > log_n) cols = log_n - d.log_r;
+        d.log_c = cols;
+    }
+    return pl;
+}
+
+}  // namespace gnt

@@ -1,0 +1,49 @@
+// Internal runtime types of libgnt.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/gnt.h"
+#include "gnt_plan.hpp"
+
+namespace gnt {
+
+int set_error(int code, const char *msg);
+
+class Pool {
+public:
+    void *acquire(size_t bytes);
+    void release(void *p, size_t bytes);
+    void trim();
+    size_t live_bytes = 0;    // allocated from the device (pooled or in use)
+    size_t pooled_bytes = 0;  // of which idle in the free lists
+private:
+    std::mutex mu;
+    std::map<size_t, std::vector<void *>> free_;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;  // serialises host entry points on this context
+    uint64_t *wlo = nullptr, *whi = nullptr;
+    Pool pool;
+    size_t hbm_cap = 0;
+    bool timing = false;
+    double phase_ms[5] = {0, 0, 0, 0, 0};
+    uint64_t products = 0;
+
+    int init_tables();
+    int enqueue_product(const uint64_t *da, size_t na, const uint64_t *db, size_t nb, uint64_t *dout,
+                        uint32_t *dlen);
+};
+
+bool plan_for(size_t na, size_t nb, Plan &pl);
+int run_batch(Ctx *c, gnt_job *jobs, size_t n);
+
+}  // namespace gnt
