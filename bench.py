@@ -1,0 +1,187 @@
+"""Benchmark: Goldilocks-NTT multiply throughput on one MI355X.
+
+Workload (default): one 2^24 x 2^24-limb product per step (uniform random
+64-bit limbs, SplitMix64 seeds 3/4, top limb nonzero), transform length
+N = 2^27, operands resident in HBM when the timed region starts.  A step is
+one full product: pack + two forward transforms + pointwise + inverse + carry.
+
+Prints ONE JSON line.  `value` is Mlimbs/s = (na + nb) / t / 1e6.
+`roofline.achieved` is the algorithmic byte count of a product,
+B_mul = 16 (na + nb) + 80 N, divided by the measured product time (HIP events
+on the library's stream), against the 8 TB/s HBM peak.  `cpu_baseline` times
+the CPU restatement of the reference multiply on a bounded sample of the same
+workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """SplitMix64 stream (vectorised numpy; same sequence as the oracle's)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def operand(seed: int, n: int) -> np.ndarray:
+    x = splitmix64(seed, n)
+    if x[-1] == 0:
+        x[-1] = 1
+    return x
+
+
+def b_mul(na: int, nb: int, log_n: int) -> float:
+    return 16.0 * (na + nb) + 80.0 * (1 << log_n)
+
+
+def b_sqr(n: int, log_n: int) -> float:
+    return 24.0 * n + 48.0 * (1 << log_n)
+
+
+def cpu_baseline(log_size: int, op: str, threads: int) -> dict:
+    """The oracle (CPU restatement of ntt.rs::mul_mag_ntt) on a bounded sample."""
+    from oracle import oracle as orc
+
+    orc.set_threads(threads)
+    orc.apply_host_knobs(threads)
+    n = 1 << log_size
+    a, b = operand(3, n), operand(4, n)
+    if op == "sqr":
+        b = a
+    orc.mul_mag_ntt(a[:4096], b[:4096])  # warm the pool/twiddle caches for small sizes
+    times = []
+    t_budget = 25.0
+    t_start = time.perf_counter()
+    reps = 0
+    while reps < 3 and (time.perf_counter() - t_start) < t_budget:
+        t0 = time.perf_counter()
+        orc.mul_mag_ntt(a, b)
+        times.append(time.perf_counter() - t0)
+        reps += 1
+    t = float(np.median(times[1:] if len(times) > 1 else times))
+    return {
+        "value": round(2 * n / t / 1e6, 3),
+        "unit": "Mlimbs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{reps} x {op} 2^{log_size} x 2^{log_size} limbs (median of runs after the first), "
+                  f"four-step threshold {orc.get_four_step_threshold()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-size", type=int, default=24, help="log2 limbs per operand")
+    ap.add_argument("--op", choices=["mul", "sqr"], default="mul")
+    ap.add_argument("--cpu-log-size", type=int, default=22, help="log2 limbs of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+              "independent products shard across GPUs with no exchange, see the design notes.", file=sys.stderr)
+        sys.exit(2)
+
+    from pi_amd import gnt
+
+    n = 1 << args.log_size
+    a, b = operand(3, n), operand(4, n)
+    ctx = gnt.Context(0)
+    log_n, radices = gnt.plan_info(n, n)
+    da = ctx.upload(a)
+    db = ctx.upload(b)
+
+    def step():
+        r = ctx.sqr_buf(da) if args.op == "sqr" else ctx.mul_buf(da, db)
+        r.release()  # stream-ordered: the buffer is reused by the next step
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+
+    # device time per product with HIP events on the library's stream
+    ctx.set_timing(True)
+    for _ in range(max(3, min(args.steps, 10))):
+        step()
+    ctx.sync()
+    phases, prods = ctx.get_timing()
+    ctx.set_timing(False)
+    t_prod_ms = phases[4] / prods
+    limbs = 2 * n
+    value = limbs / (ms_per_step * 1e-3) / 1e6
+    bytes_alg = b_sqr(n, log_n) if args.op == "sqr" else b_mul(n, n, log_n)
+    achieved = bytes_alg / (t_prod_ms * 1e-3) / 1e9
+
+    # a check that the timed path computes the right thing (low limb + length)
+    r = ctx.sqr_buf(da) if args.op == "sqr" else ctx.mul_buf(da, db)
+    out = r.download()
+    r.release()
+    bb = a if args.op == "sqr" else b
+    ok = out.size in (2 * n, 2 * n - 1) and int(out[0]) == (int(a[0]) * int(bb[0])) % (1 << 64)
+
+    line = {
+        "metric": "Goldilocks-NTT multiply Mlimbs/s at 2^24 limbs",
+        "value": round(value, 3),
+        "unit": "Mlimbs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (SplitMix64 uniform limbs, seeds 3/4)",
+        "config": {
+            "workload": f"single GPU {args.op} of 2^{args.log_size} x 2^{args.log_size} limbs",
+            "limbs_a": n, "limbs_b": n, "log_n": log_n, "radices_log2": radices,
+            "inputs": "resident in HBM", "check_low_limb_and_length": ok,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "unit_of_work": "one whole product (all launches); bytes = B_mul = 16(na+nb)+80N"
+                            if args.op == "mul" else "one whole square; bytes = B_sqr = 24n+48N",
+            "device_ms_per_product": round(t_prod_ms, 4),
+            "phase_ms_per_product": {k: round(v / prods, 4) for k, v in
+                                     zip(["forward", "pointwise", "inverse", "carry"], phases[:4])},
+        },
+    }
+    ctx.close()
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        line["cpu_baseline"] = cpu_baseline(min(args.cpu_log_size, args.log_size), args.op, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

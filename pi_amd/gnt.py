@@ -1,0 +1,259 @@
+"""Python binding of libgnt.so through its C ABI (include/gnt.h).
+
+Mirrors the reference's multiply surface (joelodom/pi crates/bignum):
+``mul_mag_ntt(a, b)`` (ntt.rs:806) and ``mul_mag(a, b)`` (integer.rs:445)
+take little-endian u64 limb arrays and return the trimmed product.  There is
+no CPU fallback: without the built library or a GPU these raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgnt.so")
+
+U64 = ctypes.c_uint64
+SZ = ctypes.c_size_t
+PU64 = ctypes.POINTER(ctypes.c_uint64)
+PSZ = ctypes.POINTER(ctypes.c_size_t)
+
+GNT_OK = 0
+GNT_E_ARG = -1
+GNT_E_TOO_LARGE = -2
+GNT_E_NOMEM = -3
+GNT_E_DEVICE = -4
+GNT_E_TIMEOUT = -5
+GNT_E_STATE = -6
+
+# Every entry point include/gnt.h declares (checked by the ABI test).
+EXPORTS = (
+    "gnt_version", "gnt_last_error", "gnt_init", "gnt_shutdown", "gnt_mul", "gnt_sqr", "gnt_mul_mag",
+    "gnt_buf_upload", "gnt_buf_download", "gnt_buf_len", "gnt_buf_release", "gnt_mul_buf", "gnt_sqr_buf",
+    "gnt_sync", "gnt_mul_batch", "gnt_mul_batch_serial", "gnt_plan_info", "gnt_set_timing", "gnt_get_timing",
+)
+
+
+class GntError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gnt error {code}: {msg}")
+        self.code = code
+
+
+class GntConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_cap_bytes", SZ), ("plan_cache_path", ctypes.c_char_p),
+                ("reserved", SZ * 4)]
+
+
+class GntJob(ctypes.Structure):
+    _fields_ = [("a", PU64), ("na", SZ), ("b", PU64), ("nb", SZ), ("out", PU64), ("out_cap", SZ),
+                ("out_len", SZ), ("is_sqr", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libgnt.so (raises if it was never built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GntError(GNT_E_STATE, f"{LIB_PATH} is not built; run python -m pi_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    L.gnt_version.restype = ctypes.c_char_p
+    L.gnt_last_error.restype = ctypes.c_char_p
+    L.gnt_init.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GntConfig)]
+    L.gnt_shutdown.argtypes = [ctypes.c_void_p]
+    L.gnt_mul.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PU64, SZ, PSZ]
+    L.gnt_sqr.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PSZ]
+    L.gnt_mul_mag.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PU64, SZ, PSZ, SZ, SZ]
+    L.gnt_buf_upload.argtypes = [ctypes.c_void_p, PU64, SZ, ctypes.POINTER(ctypes.c_void_p)]
+    L.gnt_buf_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, PU64, SZ, PSZ]
+    L.gnt_buf_len.argtypes = [ctypes.c_void_p, ctypes.c_void_p, PSZ]
+    L.gnt_buf_release.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.gnt_mul_buf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    L.gnt_sqr_buf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    L.gnt_sync.argtypes = [ctypes.c_void_p]
+    L.gnt_mul_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(GntJob), SZ]
+    L.gnt_mul_batch_serial.argtypes = [ctypes.c_void_p, ctypes.POINTER(GntJob), SZ]
+    L.gnt_plan_info.argtypes = [SZ, SZ, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int)]
+    L.gnt_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.gnt_get_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_uint64)]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != GNT_OK:
+        raise GntError(rc, load().gnt_last_error().decode(errors="replace"))
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(PU64)
+
+
+def plan_info(na: int, nb: int):
+    L = load()
+    log_n, npass = ctypes.c_int(), ctypes.c_int()
+    lr = (ctypes.c_int * 4)()
+    _check(L.gnt_plan_info(na, nb, ctypes.byref(log_n), ctypes.byref(npass), lr))
+    return log_n.value, [lr[i] for i in range(npass.value)]
+
+
+class DeviceBuf:
+    """A limb array resident in HBM (gnt_buf)."""
+
+    def __init__(self, ctx: "Context", handle: ctypes.c_void_p):
+        self.ctx = ctx
+        self.handle = handle
+
+    def __len__(self) -> int:
+        n = SZ()
+        _check(load().gnt_buf_len(self.ctx.h, self.handle, ctypes.byref(n)))
+        return n.value
+
+    def download(self) -> np.ndarray:
+        n = len(self)
+        out = np.zeros(max(n, 1), dtype=np.uint64)
+        got = SZ()
+        _check(load().gnt_buf_download(self.ctx.h, self.handle, _p(out), out.size, ctypes.byref(got)))
+        return out[:got.value].copy()
+
+    def release(self):
+        if self.handle:
+            _check(load().gnt_buf_release(self.ctx.h, self.handle))
+            self.handle = None
+
+
+class Context:
+    """One device context (gnt_ctx)."""
+
+    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0):
+        L = load()
+        cfg = GntConfig(device=device, hbm_cap_bytes=hbm_cap_bytes, plan_cache_path=None)
+        h = ctypes.c_void_p()
+        _check(L.gnt_init(ctypes.byref(h), ctypes.byref(cfg)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().gnt_shutdown(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- the reference's surface ----
+    def mul_mag_ntt(self, a, b) -> np.ndarray:
+        """ntt::mul_mag_ntt (ntt.rs:806): trimmed |a|*|b|."""
+        x, y = _u64(a), _u64(b)
+        out = np.zeros(max(x.size + y.size, 1), dtype=np.uint64)
+        n = SZ()
+        _check(load().gnt_mul(self.h, _p(x), x.size, _p(y), y.size, _p(out), out.size, ctypes.byref(n)))
+        return out[:n.value].copy()
+
+    def sqr(self, a) -> np.ndarray:
+        x = _u64(a)
+        out = np.zeros(max(2 * x.size, 1), dtype=np.uint64)
+        n = SZ()
+        _check(load().gnt_sqr(self.h, _p(x), x.size, _p(out), out.size, ctypes.byref(n)))
+        return out[:n.value].copy()
+
+    def mul_mag(self, a, b, ntt_threshold: int = 0, karatsuba_threshold: int = 0) -> np.ndarray:
+        """mul_mag / mul_mag_dispatch (integer.rs:445-486) with the GPU leaf."""
+        x, y = _u64(a), _u64(b)
+        out = np.zeros(x.size + y.size + 1, dtype=np.uint64)
+        n = SZ()
+        _check(load().gnt_mul_mag(self.h, _p(x), x.size, _p(y), y.size, _p(out), out.size, ctypes.byref(n),
+                                  ntt_threshold, karatsuba_threshold))
+        return out[:n.value].copy()
+
+    # ---- device-resident ----
+    def upload(self, a) -> DeviceBuf:
+        x = _u64(a)
+        h = ctypes.c_void_p()
+        _check(load().gnt_buf_upload(self.h, _p(x), x.size, ctypes.byref(h)))
+        return DeviceBuf(self, h)
+
+    def mul_buf(self, a: DeviceBuf, b: DeviceBuf) -> DeviceBuf:
+        h = ctypes.c_void_p()
+        _check(load().gnt_mul_buf(self.h, a.handle, b.handle, ctypes.byref(h)))
+        return DeviceBuf(self, h)
+
+    def sqr_buf(self, a: DeviceBuf) -> DeviceBuf:
+        h = ctypes.c_void_p()
+        _check(load().gnt_sqr_buf(self.h, a.handle, ctypes.byref(h)))
+        return DeviceBuf(self, h)
+
+    def sync(self):
+        _check(load().gnt_sync(self.h))
+
+    # ---- batches ----
+    def _jobs(self, pairs: Sequence):
+        keep, outs = [], []
+        jobs = (GntJob * len(pairs))()
+        for i, pr in enumerate(pairs):
+            a = _u64(pr[0])
+            sq = pr[1] is None
+            b = a if sq else _u64(pr[1])
+            out = np.zeros(max(a.size + (a.size if sq else b.size), 1), dtype=np.uint64)
+            keep += [a, b]
+            outs.append(out)
+            jobs[i] = GntJob(_p(a), a.size, _p(b), b.size, _p(out), out.size, 0, int(sq), 0)
+        return jobs, keep, outs
+
+    def mul_batch(self, pairs: Sequence, serial: bool = False):
+        """Products of (a, b) pairs; b None squares.  Returns trimmed results."""
+        jobs, keep, outs = self._jobs(pairs)
+        fn = load().gnt_mul_batch_serial if serial else load().gnt_mul_batch
+        rc = fn(self.h, jobs, len(pairs))
+        for j in jobs:
+            if j.status != GNT_OK:
+                raise GntError(j.status, load().gnt_last_error().decode(errors="replace"))
+        _check(rc)
+        return [o[:j.out_len].copy() for o, j in zip(outs, jobs)]
+
+    # ---- timing ----
+    def set_timing(self, on: bool):
+        _check(load().gnt_set_timing(self.h, int(on)))
+
+    def get_timing(self):
+        ms = (ctypes.c_double * 5)()
+        n = ctypes.c_uint64()
+        _check(load().gnt_get_timing(self.h, ms, 5, ctypes.byref(n)))
+        return list(ms), n.value
+
+
+_default: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def mul_mag_ntt(a, b) -> np.ndarray:
+    return default_context().mul_mag_ntt(a, b)
+
+
+def mul_mag(a, b) -> np.ndarray:
+    return default_context().mul_mag(a, b)
+
+
+def sqr(a) -> np.ndarray:
+    return default_context().sqr(a)

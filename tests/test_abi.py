@@ -1,0 +1,60 @@
+"""The C ABI library builds, loads and exports every entry point include/gnt.h
+declares.  No compute calls: these run without a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "gnt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gnt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for must in ("gnt_init", "gnt_mul", "gnt_sqr", "gnt_mul_mag", "gnt_mul_batch", "gnt_buf_upload",
+                 "gnt_mul_buf", "gnt_shutdown", "gnt_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from pi_amd import gnt
+
+    lib = gnt.load()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", gnt.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (gnt_[a-z0-9_]+)", out))
+    assert set(declared()) <= exported
+    assert set(gnt.EXPORTS) == set(declared())
+
+
+def test_library_is_gfx950_code():
+    from pi_amd import gnt
+
+    data = open(gnt.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_plan_info_and_errors_without_gpu():
+    from pi_amd import gnt
+
+    log_n, radices = gnt.plan_info(1 << 24, 1 << 24)
+    assert log_n == 27 and sum(radices) == 27
+    with pytest.raises(gnt.GntError) as e:
+        gnt.plan_info(1 << 29, 1 << 29)
+    assert e.value.code == gnt.GNT_E_TOO_LARGE
+    lib = gnt.load()
+    assert lib.gnt_version().startswith(b"gnt")
+    # no device in this container: init fails with an error code, never aborts
+    h = ctypes.c_void_p()
+    rc = lib.gnt_init(ctypes.byref(h), None)
+    assert rc in (gnt.GNT_OK, gnt.GNT_E_DEVICE)
+    if rc == gnt.GNT_OK:
+        lib.gnt_shutdown(h)

@@ -1,0 +1,143 @@
+"""CPU checks of the GPU kernels' arithmetic, plans and index math.
+
+tests/emu/libgnt_emu.so runs the same per-task code as the HIP kernels
+(pi_amd/csrc/gnt_pass.hpp, gl_field.hpp) one task at a time on the CPU.  Its
+field operations are checked against exact integer arithmetic, its forward
+transform against the closed-form DFT, and its whole products against the
+oracle and GMP, so index or twiddle mistakes show up before any GPU run.
+"""
+import ctypes
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+P = 0xFFFFFFFF00000001
+HERE = os.path.dirname(os.path.abspath(__file__))
+U = ctypes.c_uint64
+PU = ctypes.POINTER(U)
+
+
+@pytest.fixture(scope="module")
+def emu():
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "emu")], check=True)
+    E = ctypes.CDLL(os.path.join(HERE, "emu", "libgnt_emu.so"))
+    for f in ("emu_add", "emu_sub", "emu_mul_mod", "emu_reduce128"):
+        getattr(E, f).restype = U
+        getattr(E, f).argtypes = [U, U]
+    E.emu_mul_pow2.restype = U
+    E.emu_mul_pow2.argtypes = [U, ctypes.c_int]
+    E.emu_root.restype = U
+    E.emu_root.argtypes = [ctypes.c_int]
+    E.emu_forward.argtypes = [PU, ctypes.c_int]
+    E.emu_inverse.argtypes = [PU, ctypes.c_int]
+    E.emu_mul.restype = ctypes.c_size_t
+    E.emu_mul.argtypes = [PU, ctypes.c_size_t, PU, ctypes.c_size_t, PU]
+    E.emu_plan.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+    return E
+
+
+def emu_mul(E, a, b=None):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    nb = a.size if b is None else len(b)
+    out = np.zeros(a.size + nb, dtype=np.uint64)
+    pb = None if b is None else np.ascontiguousarray(b, dtype=np.uint64)
+    n = E.emu_mul(a.ctypes.data_as(PU), a.size, None if pb is None else pb.ctypes.data_as(PU), nb,
+                  out.ctypes.data_as(PU))
+    return out[:n]
+
+
+def test_field_ops_exact(emu):
+    rnd = random.Random(5)
+    vals = [0, 1, 2, P - 1, P - 2, 2**32 - 1, 2**32, P // 2, (P + 1) // 2] + [rnd.randrange(P) for _ in range(200)]
+    for a in vals:
+        for b in vals[:30]:
+            assert emu.emu_add(a, b) == (a + b) % P
+            assert emu.emu_sub(a, b) == (a - b) % P
+            assert emu.emu_mul_mod(a, b) == a * b % P
+    for x in [0, 1, 2**128 - 1, P * P, P * P - 1] + [rnd.randrange(2**128) for _ in range(500)]:
+        assert emu.emu_reduce128(x & (2**64 - 1), x >> 64) == x % P
+
+
+def test_shift_multiplies(emu):
+    """x * 2^e for every exponent used by the register DFTs (2^192 = 1, 2^96 = -1)."""
+    assert pow(2, 96, P) == P - 1 and pow(2, 192, P) == 1
+    rnd = random.Random(6)
+    for x in [0, 1, P - 1, 2**63, 2**32 + 5] + [rnd.randrange(P) for _ in range(40)]:
+        for e in range(192):
+            assert emu.emu_mul_pow2(x, e) == x * pow(2, e, P) % P
+
+
+def test_library_root(emu):
+    for log_n in range(1, 31):
+        w = emu.emu_root(log_n)
+        n = 1 << log_n
+        assert pow(w, n, P) == 1 and pow(w, n // 2, P) == P - 1
+        if log_n <= 6:
+            assert w == pow(2, 192 // n, P)  # small-order roots are powers of two
+
+
+def test_plans(emu):
+    for log_n in range(1, 31):
+        npass = ctypes.c_int()
+        lr, lc = (ctypes.c_int * 4)(), (ctypes.c_int * 4)()
+        emu.emu_plan(log_n, ctypes.byref(npass), lr, lc)
+        assert sum(lr[i] for i in range(npass.value)) == log_n
+        m = log_n
+        for p in range(npass.value):
+            s = m - lr[p]
+            assert lr[p] + lc[p] <= 13                     # tile fits 64 KiB of LDS
+            if s > 0:
+                assert lc[p] <= s and lc[p] >= 3           # >= 64-byte contiguous runs
+                assert lr[p] <= 10
+            m = s
+
+
+@pytest.mark.parametrize("log_n", list(range(1, 10)))
+def test_forward_is_dft_in_bit_reversed_order(emu, log_n):
+    n = 1 << log_n
+    rnd = random.Random(log_n)
+    x = np.array([rnd.randrange(P) for _ in range(n)], dtype=np.uint64)
+    y = x.copy()
+    emu.emu_forward(y.ctypes.data_as(PU), log_n)
+    w = emu.emu_root(log_n)
+    xs = [int(v) for v in x]
+    for pos in rnd.sample(range(n), min(n, 12)):
+        k = int(format(pos, f"0{log_n}b")[::-1], 2)
+        assert int(y[pos]) == sum(xs[i] * pow(w, i * k, P) for i in range(n)) % P
+    z = y.copy()
+    emu.emu_inverse(z.ctypes.data_as(PU), log_n)
+    ninv = pow(n, P - 2, P)
+    assert [int(v) * ninv % P for v in z] == xs
+
+
+@pytest.mark.parametrize("log_n", [14, 17, 21])
+def test_round_trip_multi_pass(emu, log_n):
+    n = 1 << log_n
+    x = np.random.default_rng(log_n).integers(0, 2**63, size=n, dtype=np.uint64)
+    y = x.copy()
+    emu.emu_forward(y.ctypes.data_as(PU), log_n)
+    emu.emu_inverse(y.ctypes.data_as(PU), log_n)
+    # inverse is unscaled: y = n * x mod P
+    assert all(int(y[i]) == int(x[i]) * n % P for i in range(0, n, 997))
+
+
+@pytest.mark.parametrize("la,lb", [(1, 1), (1, 4), (8, 5), (200, 200), (1000, 3), (2048, 2048), (5000, 3000),
+                                   (1 << 15, 1 << 15), (1 << 18, 1 << 17)])
+def test_products_match_oracle(emu, orc, la, lb):
+    a, b = orc.splitmix64(la, la), orc.splitmix64(lb + 7, lb)
+    assert np.array_equal(emu_mul(emu, a, b), orc.mul_mag_ntt(a, b))
+    assert np.array_equal(emu_mul(emu, a), orc.mul_mag_ntt(a, a))
+
+
+@pytest.mark.parametrize("n", [1, 5, 1000, 20000])
+def test_all_ones(emu, orc, n):
+    a = np.full(n, 2**64 - 1, dtype=np.uint64)
+    want = np.zeros(2 * n, dtype=np.uint64)
+    want[0] = 1
+    want[n] = np.uint64(2**64 - 2)
+    want[n + 1:] = np.uint64(2**64 - 1)
+    assert np.array_equal(emu_mul(emu, a, a), want)
+    assert np.array_equal(emu_mul(emu, a), want)

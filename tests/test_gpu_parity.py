@@ -1,0 +1,240 @@
+"""GPU parity: the HIP multiply (through the C ABI) against the oracle.
+
+Bit-exact everywhere: the product of two integers is unique.  Cases follow the
+reference's own multiply tests (crates/bignum/src/ntt.rs:1384-1487,
+1527-1567, 1712-1722, 2055-2101) with its `random_vec` inputs, plus GMP
+mpn_mul/mpn_sqr and CPython int as independent references, closed forms for
+all-ones operands, and residue fingerprints at the 2^24-limb size.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+M61 = (1 << 61) - 1
+FP_PRIMES = (M61, 0x1FFFFFFFFFFFFFC7, 0x0FFFFFFFFFFFFFC5)  # 2^61-1 and two primes below 2^61, 2^60
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def check(orc, got, a, b=None):
+    want = orc.gmp_mul(a, b) if b is not None else orc.gmp_sqr(a)
+    assert got.dtype == np.uint64
+    assert np.array_equal(got, want), f"mismatch: got {got.size} limbs, want {want.size}"
+
+
+# ---- the reference's mul_mag_ntt tests (ntt.rs:1384-1487) ----
+
+def test_tiny_3x7(ctx, orc):
+    log("3*7")
+    assert list(ctx.mul_mag_ntt([3], [7])) == [21]
+
+
+@pytest.mark.parametrize("la,lb,seed", [(1, 1, 1), (1, 4, 2), (4, 4, 3), (8, 5, 4), (32, 16, 5), (60, 60, 6),
+                                        (200, 200, 7)])
+def test_small_random_vs_schoolbook(ctx, orc, la, lb, seed):
+    log(f"random_vec {la}x{lb} seed {seed}")
+    a = orc.random_vec(la, seed)
+    b = orc.random_vec(lb, seed ^ 0xFF)
+    assert np.array_equal(ctx.mul_mag_ntt(a, b), orc.schoolbook(a, b))
+
+
+def test_u64_max_squared(ctx, orc):
+    m = np.array([2**64 - 1], dtype=np.uint64)
+    got = ctx.mul_mag_ntt(m, m)
+    assert orc.to_int(got) == (2**64 - 1) ** 2
+    assert np.array_equal(got, orc.schoolbook(m, m))
+
+
+def test_medium_500(ctx, orc):
+    a = orc.random_vec(500, 0xDEADBEEFCAFEBABE)
+    b = orc.random_vec(500, 0xDEADBEEFCAFEBABE ^ 0xFF)
+    assert np.array_equal(ctx.mul_mag_ntt(a, b), orc.schoolbook(a, b))
+
+
+@pytest.mark.parametrize("n,sa,sb", [(200, 0x123456789ABCDEF0, 0x0FEDCBA987654321), (300, 0xCAFE0001, 0xBEEF0002),
+                                     (300, 0x1357, 0x2468), (400, 0x1111222233334444, 0x5555666677778888),
+                                     (600, 0xAB, 0xCD)])
+def test_reference_seeds(ctx, orc, n, sa, sb):
+    a, b = orc.random_vec(n, sa), orc.random_vec(n, sb)
+    want = orc.schoolbook(a, b)
+    assert np.array_equal(ctx.mul_mag_ntt(a, b), want)
+    assert np.array_equal(ctx.mul_mag_ntt(a, b), want)  # pooled buffers reused
+
+
+def test_32768_limbs(ctx, orc):
+    a, b = orc.random_vec(32768, 0xAA), orc.random_vec(32768, 0xBB)
+    assert np.array_equal(ctx.mul_mag_ntt(a, b), orc.mul_mag_ntt(a, b))
+
+
+# ---- edge cases ----
+
+def test_empty_and_zero(ctx, orc):
+    e = np.zeros(0, dtype=np.uint64)
+    assert ctx.mul_mag_ntt(e, [5]).size == 0
+    assert ctx.mul_mag_ntt([5], e).size == 0
+    assert ctx.sqr(e).size == 0
+    assert ctx.mul_mag_ntt([0], [7]).size == 0          # zero, not normalised
+    assert ctx.mul_mag_ntt([0, 0, 0], [1, 2]).size == 0
+
+
+def test_unnormalised_inputs_trimmed(ctx, orc):
+    a = np.array([5, 0, 0, 0], dtype=np.uint64)
+    b = np.array([7, 9, 0], dtype=np.uint64)
+    got = ctx.mul_mag_ntt(a, b)
+    assert list(got) == [35, 45]
+    assert np.array_equal(got, orc.mul_mag_ntt(a, b))
+
+
+@pytest.mark.parametrize("la,lb", [(1000, 3), (1, 100000), (3, 70000), (12345, 54321), (8191, 8193)])
+def test_unbalanced(ctx, orc, la, lb):
+    log(f"unbalanced {la}x{lb}")
+    a, b = orc.splitmix64(la, la), orc.splitmix64(lb + 1, lb)
+    check(orc, ctx.mul_mag_ntt(a, b), a, b)
+
+
+def test_every_transform_length(ctx, orc):
+    """One product per transform length 2^3 .. 2^24 (every plan shape)."""
+    for log_n in range(3, 25):
+        # na + nb = 2^(log_n-2) gives n_out = 2^log_n - 1
+        tot = 1 << (log_n - 2) if log_n >= 3 else 2
+        na = max(1, tot // 2 - 3)
+        nb = max(1, tot - na)
+        a, b = orc.splitmix64(100 + log_n, na), orc.splitmix64(200 + log_n, nb)
+        log(f"log_n={log_n} na={na} nb={nb}")
+        check(orc, ctx.mul_mag_ntt(a, b), a, b)
+        check(orc, ctx.sqr(a), a)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 1000, 4096, 65537, 1 << 18])
+def test_all_ones_closed_form(ctx, orc, n):
+    """(B^n - 1)^2 = [1, 0 x (n-1), 2^64-2, (2^64-1) x (n-1)]: maximal carries."""
+    log(f"all-ones n={n}")
+    a = np.full(n, 2**64 - 1, dtype=np.uint64)
+    want = np.zeros(2 * n, dtype=np.uint64)
+    want[0] = 1
+    want[n] = np.uint64(2**64 - 2)
+    want[n + 1:] = np.uint64(2**64 - 1)
+    assert np.array_equal(ctx.mul_mag_ntt(a, a), want)
+    assert np.array_equal(ctx.sqr(a), want)
+
+
+def test_all_ones_rectangular(ctx, orc):
+    """(B^m - 1)(B^n - 1) = B^(m+n) - B^m - B^n + 1."""
+    m, n = 3000, 777
+    a = np.full(m, 2**64 - 1, dtype=np.uint64)
+    b = np.full(n, 2**64 - 1, dtype=np.uint64)
+    B = 1 << 64
+    assert orc.to_int(ctx.mul_mag_ntt(a, b)) == B ** (m + n) - B**m - B**n + 1
+
+
+def test_powers_of_two(ctx, orc):
+    a = np.zeros(5000, dtype=np.uint64); a[-1] = 1 << 63
+    b = np.zeros(3000, dtype=np.uint64); b[-1] = 1 << 7
+    assert orc.to_int(ctx.mul_mag_ntt(a, b)) == (1 << (64 * 4999 + 63)) * (1 << (64 * 2999 + 7))
+
+
+def test_carry_chain_across_tiles(ctx, orc):
+    """A long propagate run fed by one generate: x = B^n - 1 times (B^k + 1)."""
+    n, k = 10000, 1
+    a = np.full(n, 2**64 - 1, dtype=np.uint64)
+    b = np.zeros(k + 1, dtype=np.uint64); b[0] = 1; b[k] = 1
+    B = 1 << 64
+    assert orc.to_int(ctx.mul_mag_ntt(a, b)) == (B**n - 1) * (B**k + 1)
+
+
+def test_cpython_int_cross_check(ctx, orc):
+    a, b = orc.splitmix64(11, 3333), orc.splitmix64(12, 4444)
+    assert orc.to_int(ctx.mul_mag_ntt(a, b)) == orc.to_int(a) * orc.to_int(b)
+
+
+# ---- workload distributions at 2^20 limbs (exact vs GMP) ----
+
+def _dist(orc, kind, n, seed):
+    if kind == "uniform":
+        x = orc.splitmix64(seed, n)
+    elif kind == "allones":
+        x = np.full(n, 2**64 - 1, dtype=np.uint64)
+    else:  # sparse: 1% nonzero limbs
+        r = orc.splitmix64(seed, 2 * n)
+        keep = (r[:n] % np.uint64(100)) == 0
+        x = np.where(keep, r[n:], np.uint64(0)).astype(np.uint64)
+    if x[-1] == 0:
+        x[-1] = 1
+    return x
+
+
+@pytest.mark.parametrize("kind", ["uniform", "allones", "sparse"])
+def test_2p20_distributions(ctx, orc, kind):
+    n = 1 << 20
+    log(f"2^20 {kind}")
+    a, b = _dist(orc, kind, n, 3), _dist(orc, kind, n, 4)
+    check(orc, ctx.mul_mag_ntt(a, b), a, b)
+    check(orc, ctx.sqr(a), a)
+
+
+def _fingerprints_ok(orc, c, a, b):
+    for q in FP_PRIMES:
+        if orc.mod_small(c, q) != (orc.mod_small(a, q) * orc.mod_small(b, q)) % q:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("kind", ["uniform", "allones", "sparse"])
+def test_2p24_distributions(ctx, orc, kind):
+    """2^24 x 2^24 (N = 2^27): GMP when present, residue fingerprints and the
+    low limb always; the closed form for all-ones."""
+    n = 1 << 24
+    log(f"2^24 {kind}")
+    a, b = _dist(orc, kind, n, 5), _dist(orc, kind, n, 6)
+    c = ctx.mul_mag_ntt(a, b)
+    assert c.size in (2 * n, 2 * n - 1)
+    assert int(c[0]) == (int(a[0]) * int(b[0])) % (1 << 64)
+    assert _fingerprints_ok(orc, c, a, b)
+    if kind == "allones":
+        assert c[0] == 1 and c[n] == 2**64 - 2
+        assert not c[1:n].any() and (c[n + 1:] == np.uint64(2**64 - 1)).all()
+    elif orc.gmp() is not None:
+        assert np.array_equal(c, orc.gmp_mul(a, b))
+    s = ctx.sqr(a)
+    assert _fingerprints_ok(orc, s, a, a)
+
+
+# ---- device-resident operands, batches, dispatch ----
+
+def test_device_chain_reuse(ctx, orc):
+    """A result kept in HBM feeds the next product: ((a*b)^2) * a."""
+    a, b = orc.splitmix64(21, 7000), orc.splitmix64(22, 5000)
+    da, db = ctx.upload(a), ctx.upload(b)
+    p = ctx.mul_buf(da, db)
+    s = ctx.sqr_buf(p)
+    t = ctx.mul_buf(s, da)
+    A, Bv = orc.to_int(a), orc.to_int(b)
+    assert orc.to_int(t.download()) == (A * Bv) ** 2 * A
+    assert len(p) == orc.gmp_mul(a, b).size
+    for h in (da, db, p, s, t):
+        h.release()
+
+
+def test_batch_matches_single(ctx, orc):
+    rng = np.random.default_rng(7)
+    pairs = []
+    for i in range(40):
+        n = int(round(2 ** rng.uniform(8, 16)))
+        a = orc.splitmix64(1000 + i, n)
+        pairs.append((a, None if i % 3 == 0 else orc.splitmix64(2000 + i, n)))
+    got = ctx.mul_batch(pairs)
+    ser = ctx.mul_batch(pairs, serial=True)
+    for (a, b), g, s in zip(pairs, got, ser):
+        want = orc.gmp_sqr(a) if b is None else orc.gmp_mul(a, b)
+        assert np.array_equal(g, want) and np.array_equal(s, want)
+
+
+@pytest.mark.parametrize("la,lb", [(5, 7), (40, 100), (3000, 2500), (9000, 9000), (20000, 9000), (100, 30000)])
+def test_mul_mag_dispatch(ctx, orc, la, lb):
+    a, b = orc.splitmix64(la, la), orc.splitmix64(lb + 3, lb)
+    check(orc, ctx.mul_mag(a, b), a, b)
+    # a low NTT threshold routes the Karatsuba leaves to the GPU too
+    check(orc, ctx.mul_mag(a, b, ntt_threshold=64), a, b)
