@@ -172,7 +172,7 @@ def main():
                             if args.op == "mul" else "one whole square; bytes = B_sqr = 24n+48N",
             "device_ms_per_product": round(t_prod_ms, 4),
             "phase_ms_per_product": {k: round(v / prods, 4) for k, v in
-                                     zip(["forward", "pointwise", "inverse", "carry"], phases[:4])},
+                                     zip(["forward", "last_fused", "inverse", "carry"], phases[:4])},
         },
     }
     ctx.close()

@@ -104,7 +104,8 @@ int gnt_mul_batch_serial(gnt_ctx *ctx, gnt_job *jobs, size_t n);
 /* Transform plan the library uses for an na x nb product. */
 int gnt_plan_info(size_t na, size_t nb, int *log_n, int *npass, int *log_r /* [4] */);
 /* Per-phase device time (ms) accumulated while timing is on:
- * [0] forward passes, [1] pointwise, [2] inverse passes, [3] carry, [4] whole products. */
+ * [0] forward passes before the last, [1] last forward pass + pointwise + first inverse pass,
+ * [2] remaining inverse passes, [3] carry, [4] whole products. */
 int gnt_set_timing(gnt_ctx *ctx, int on);
 int gnt_get_timing(gnt_ctx *ctx, double *ms, int n, uint64_t *products);
 

@@ -144,30 +144,44 @@ int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64
     const size_t nt = carry_tiles(n_limbs);
     u64 *xa = (u64 *)pool.acquire(N * sizeof(u64));
     u64 *xb = sqr ? nullptr : (u64 *)pool.acquire(N * sizeof(u64));
+    u64 *lh = (u64 *)pool.acquire(2 * n_limbs * sizeof(u64));
     unsigned char *gp = (unsigned char *)pool.acquire(2 * nt);
-    if (!xa || (!sqr && !xb) || !gp) {
+    auto give_back = [&]() {
         pool.release(xa, N * sizeof(u64));
         pool.release(xb, N * sizeof(u64));
+        pool.release(lh, 2 * n_limbs * sizeof(u64));
         pool.release(gp, 2 * nt);
+    };
+    if (!xa || (!sqr && !xb) || !lh || !gp) {
+        give_back();
         return set_error(GNT_E_NOMEM, "device memory exhausted");
     }
+    u64 *Ls = lh, *Hs = lh + n_limbs;
     hipEvent_t ev[5];
     if (timing) {
         for (auto &e : ev) (void)hipEventCreate(&e);
         (void)hipEventRecord(ev[0], stream);
     }
-    int rc = launch_forward(pl, xa, da, na, wlo, whi, stream);
-    if (!rc && !sqr) rc = launch_forward(pl, xb, db, nb, wlo, whi, stream);
+    const int P = pl.npass;
+    int rc = 0;
+    if (sqr) {
+        if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, wlo, whi, stream);
+    } else {
+        rc = launch_forward(pl, P, xa, da, na, wlo, whi, stream);
+        if (!rc && P > 1) rc = launch_forward(pl, P - 1, xb, db, nb, wlo, whi, stream);
+    }
     if (timing) (void)hipEventRecord(ev[1], stream);
-    if (!rc) rc = launch_pointwise(xa, sqr ? xa : xb, pl.log_n, stream);
+    if (!rc) {
+        const u64 *bsrc = sqr ? xa : xb;
+        const u64 *blimbs = sqr ? da : db;
+        rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, wlo, whi, Ls, Hs, n_out, n_limbs, stream);
+    }
     if (timing) (void)hipEventRecord(ev[2], stream);
-    if (!rc) rc = launch_inverse(pl, xa, wlo, whi, stream);
+    if (!rc) rc = launch_inverse_rest(pl, xa, wlo, whi, Ls, Hs, n_out, n_limbs, stream);
     if (timing) (void)hipEventRecord(ev[3], stream);
-    if (!rc) rc = launch_carry(xa, n_out, dout, n_limbs, gp, gp + nt, dlen, stream);
+    if (!rc) rc = launch_carry(Ls, Hs, dout, n_limbs, gp, gp + nt, dlen, stream);
     if (timing) (void)hipEventRecord(ev[4], stream);
-    pool.release(xa, N * sizeof(u64));
-    pool.release(xb, N * sizeof(u64));
-    pool.release(gp, 2 * nt);
+    give_back();
     if (timing) {
         (void)hipEventSynchronize(ev[4]);
         float t;

@@ -1,11 +1,15 @@
 // HIP kernels of the Goldilocks-NTT multiply, written for gfx950 (wave64,
 // 64 KiB LDS tiles).  Integer-only: no MFMA, no floating point.
 //
-// Pipeline for one product (reference: crates/bignum/src/ntt.rs:806-865):
-//   fwd passes (pack fused into the first)  x2 operands (x1 for a square)
-//   pointwise  c = a * b * N^-1               (N^-1 = 2^(192 - log N))
-//   inv passes
-//   carry: tile scan -> tile-carry scan -> sparse ripple  (exact limbs)
+// One product (reference: crates/bignum/src/ntt.rs:806-865):
+//   k_fwd_pass       forward passes of A (pack fused into the first) and of
+//                    B except B's last pass
+//   k_last_fused     B's last forward pass, the pointwise product with A and
+//                    1/N, and the first inverse pass, in one tile round trip
+//   k_inv_pass       the remaining inverse passes; the last one emits limb
+//                    sums L_j + 2^64 H_j instead of coefficients
+//   k_carry_*        exact carry: tile scan -> tile-carry scan -> sparse ripple
+// A square runs A through k_last_fused with itself.
 #include <hip/hip_runtime.h>
 
 #include "gnt_kernels.hpp"
@@ -14,119 +18,74 @@
 namespace gnt {
 
 constexpr int BLOCK = 256;
+using X = DevExec<BLOCK>;
 
-template <int LOGR, int I>
-struct FwdSteps {
-    __device__ __forceinline__ static void run(u64 *tile, int log_c, int tile_n, size_t cg0, int log_m, int log_s,
-                                               const u64 *wlo, const u64 *whi) {
-        const int ntask = tile_n >> Sub<LOGR>::log(I);
-        for (int L = threadIdx.x; L < ntask; L += BLOCK)
-            fwd_task<LOGR, I>(tile, L, log_c, cg0, log_m, log_s, wlo, whi);
-        __syncthreads();
-        FwdSteps<LOGR, I + 1>::run(tile, log_c, tile_n, cg0, log_m, log_s, wlo, whi);
-    }
-};
-template <int LOGR>
-struct FwdSteps<LOGR, (LOGR + 3) / 4> {
-    __device__ __forceinline__ static void run(u64 *, int, int, size_t, int, int, const u64 *, const u64 *) {}
-};
-
-template <int LOGR, int I>
-struct InvSteps {  // sub-steps I, I-1, ..., 0
-    __device__ __forceinline__ static void run(u64 *tile, int log_c, int tile_n, size_t cg0, int log_m, int log_s,
-                                               const u64 *wlo, const u64 *whi) {
-        const int ntask = tile_n >> Sub<LOGR>::log(I);
-        for (int L = threadIdx.x; L < ntask; L += BLOCK)
-            inv_task<LOGR, I>(tile, L, log_c, cg0, log_m, log_s, wlo, whi);
-        __syncthreads();
-        InvSteps<LOGR, I - 1>::run(tile, log_c, tile_n, cg0, log_m, log_s, wlo, whi);
-    }
-};
-template <int LOGR>
-struct InvSteps<LOGR, -1> {
-    __device__ __forceinline__ static void run(u64 *, int, int, size_t, int, int, const u64 *, const u64 *) {}
-};
-
-// Forward pass.  With `limbs` set this is the first pass: the 16-bit
-// coefficients are unpacked from the operand's limbs as the tile is loaded.
 template <int LOGR>
 __global__ __launch_bounds__(BLOCK) void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs,
-                                                    size_t n_limbs, int log_m, int log_c,
-                                                    const u64 *__restrict__ wlo, const u64 *__restrict__ whi) {
+                                                    size_t n_limbs, PassDesc d, const u64 *__restrict__ wlo,
+                                                    const u64 *__restrict__ whi) {
     extern __shared__ u64 tile[];
-    const int log_s = log_m - LOGR;
-    const int tile_n = 1 << (LOGR + log_c);
-    const size_t cg0 = (size_t)blockIdx.x << log_c;
-    for (int idx = threadIdx.x; idx < tile_n; idx += BLOCK) {
-        int n, c;
-        tile_rc(idx, LOGR, log_c, log_s, n, c);
-        size_t a = tile_addr(cg0, n, c, log_m, log_s);
-        tile[(n << log_c) + c] = limbs ? coeff_from_limbs(limbs, n_limbs, a) : x[a];
-    }
-    __syncthreads();
-    FwdSteps<LOGR, 0>::run(tile, log_c, tile_n, cg0, log_m, log_s, wlo, whi);
-    for (int idx = threadIdx.x; idx < tile_n; idx += BLOCK) {
-        int n, c;
-        tile_rc(idx, LOGR, log_c, log_s, n, c);
-        x[tile_addr(cg0, n, c, log_m, log_s)] = tile[(n << log_c) + c];
-    }
+    const TileGeom g = tile_geom<LOGR>(d, blockIdx.x);
+    tile_load<X, LOGR>(tile, g, x, limbs, n_limbs);
+    tile_fwd<X, LOGR, 0>(tile, g, wlo, whi);
+    tile_store<X, LOGR>(tile, g, x);
 }
 
+// Inverse pass; with Ls set it is the last one and writes limb sums.
 template <int LOGR>
-__global__ __launch_bounds__(BLOCK) void k_inv_pass(u64 *__restrict__ x, int log_m, int log_c,
-                                                    const u64 *__restrict__ wlo, const u64 *__restrict__ whi) {
+__global__ __launch_bounds__(BLOCK) void k_inv_pass(u64 *__restrict__ x, PassDesc d, const u64 *__restrict__ wlo,
+                                                    const u64 *__restrict__ whi, u64 *__restrict__ Ls,
+                                                    u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
     extern __shared__ u64 tile[];
-    const int log_s = log_m - LOGR;
-    const int tile_n = 1 << (LOGR + log_c);
-    const size_t cg0 = (size_t)blockIdx.x << log_c;
-    for (int idx = threadIdx.x; idx < tile_n; idx += BLOCK) {
-        int n, c;
-        tile_rc(idx, LOGR, log_c, log_s, n, c);
-        tile[(n << log_c) + c] = x[tile_addr(cg0, n, c, log_m, log_s)];
-    }
-    __syncthreads();
-    InvSteps<LOGR, (LOGR + 3) / 4 - 1>::run(tile, log_c, tile_n, cg0, log_m, log_s, wlo, whi);
-    for (int idx = threadIdx.x; idx < tile_n; idx += BLOCK) {
-        int n, c;
-        tile_rc(idx, LOGR, log_c, log_s, n, c);
-        x[tile_addr(cg0, n, c, log_m, log_s)] = tile[(n << log_c) + c];
-    }
+    const TileGeom g = tile_geom<LOGR>(d, blockIdx.x);
+    tile_load<X, LOGR>(tile, g, x, nullptr, 0);
+    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, wlo, whi);
+    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    else tile_store<X, LOGR>(tile, g, x);
 }
 
-// c[i] = a[i] * b[i] * 2^(192 - log_n): the 1/N of the inverse (ntt.rs:376-391).
-__global__ __launch_bounds__(BLOCK) void k_pointwise(u64 *__restrict__ a, const u64 *__restrict__ b, size_t n,
-                                                     int log_n) {
-    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * BLOCK;
-    const int e = 192 - log_n;
-    for (; i < n; i += stride) a[i] = gl::mul_pow2(gl::mul(a[i], b[i]), e);
+// Last forward pass of B (read from xb, or packed from limbs), pointwise with
+// A (xa at the same positions; xa == nullptr squares), 1/N, first inverse
+// pass; result to xa, or as limb sums when this is the only pass.
+template <int LOGR>
+__global__ __launch_bounds__(BLOCK) void k_last_fused(const u64 *__restrict__ xb, const u64 *__restrict__ limbs,
+                                                      size_t n_limbs_in, u64 *xa, const u64 *a_other, PassDesc d,
+                                                      int log_n, const u64 *__restrict__ wlo,
+                                                      const u64 *__restrict__ whi, u64 *__restrict__ Ls,
+                                                      u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
+    extern __shared__ u64 tile[];
+    const TileGeom g = tile_geom<LOGR>(d, blockIdx.x);
+    tile_load<X, LOGR>(tile, g, xb, limbs, n_limbs_in);
+    tile_fwd<X, LOGR, 0>(tile, g, wlo, whi);
+    tile_pointwise<X, LOGR>(tile, g, a_other, log_n);
+    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, wlo, whi);
+    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    else tile_store<X, LOGR>(tile, g, xa);
 }
 
 // ---------------------------------------------------------------------------
-// Carry: coefficients (exact integers < 2^58) -> limbs (ntt.rs:669-791).
-// Limb j collects S_j = L_j + 2^64 H_j; its value is y_j = L_j + H_{j-1} with a
-// 1-bit overflow g_j; what is left is a 1-bit carry-lookahead scan over
-// (generate g_j, propagate y_j == 2^64-1), done per tile, then across tiles,
-// then applied to the (rare) tiles that receive a carry.
+// Carry: limb sums -> limbs (ntt.rs:669-791).  Limb j's value is
+// y_j = L_j + H_{j-1} with a 1-bit overflow g_j; what is left is a 1-bit
+// carry-lookahead scan over (generate g_j, propagate y_j == 2^64-1), done per
+// tile, then across tiles, then applied to the (rare) tiles that receive one.
 // ---------------------------------------------------------------------------
 
 constexpr int CARRY_LIMBS_PER_THREAD = 4;
 constexpr int CARRY_TILE = BLOCK * CARRY_LIMBS_PER_THREAD;  // limbs per workgroup
 
-__global__ __launch_bounds__(BLOCK) void k_carry_tiles(const u64 *__restrict__ coeffs, size_t n_coeffs,
+__global__ __launch_bounds__(BLOCK) void k_carry_tiles(const u64 *__restrict__ Ls, const u64 *__restrict__ Hs,
                                                        u64 *__restrict__ out, size_t n_limbs,
                                                        unsigned char *__restrict__ tile_gp, u32 *__restrict__ out_len) {
     __shared__ u32 scan[BLOCK];
     const size_t j0 = (size_t)blockIdx.x * CARRY_TILE + (size_t)threadIdx.x * CARRY_LIMBS_PER_THREAD;
     u64 y[CARRY_LIMBS_PER_THREAD];
     u32 gbits = 0, pbits = 0;
-    u64 Hprev = 0, L, H;
-    if (j0 > 0 && j0 - 1 < n_limbs) limb_sum(coeffs, n_coeffs, j0 - 1, L, Hprev);
+    u64 Hprev = (j0 > 0 && j0 - 1 < n_limbs) ? Hs[j0 - 1] : 0;
 #pragma unroll
     for (int k = 0; k < CARRY_LIMBS_PER_THREAD; k++) {
         size_t j = j0 + k;
-        if (j < n_limbs) limb_sum(coeffs, n_coeffs, j, L, H);
-        else { L = 0; H = 0; }
+        u64 L = 0, H = 0;
+        if (j < n_limbs) { L = Ls[j]; H = Hs[j]; }
         u64 s = L + Hprev;
         gbits |= (u32)(s < L) << k;
         pbits |= (u32)(s == ~0ull) << k;
@@ -237,22 +196,10 @@ __global__ void k_zero_u32(u32 *p, int n) {
 // Host-side launchers
 // ---------------------------------------------------------------------------
 
-template <int LOGR>
-static void launch_fwd(const PassDesc &d, u64 *x, const u64 *limbs, size_t n_limbs, int log_n, const u64 *wlo,
-                       const u64 *whi, hipStream_t st) {
-    size_t tile = (size_t)1 << (LOGR + d.log_c);
-    size_t grid = ((size_t)1 << log_n) / tile;
-    hipLaunchKernelGGL(k_fwd_pass<LOGR>, dim3((unsigned)grid), dim3(BLOCK), tile * sizeof(u64), st, x, limbs, n_limbs,
-                       d.log_m, d.log_c, wlo, whi);
+static inline dim3 pass_grid(const PassDesc &d, int log_n) {
+    return dim3((unsigned)(((size_t)1 << log_n) >> (d.log_r + d.log_c)));
 }
-
-template <int LOGR>
-static void launch_inv(const PassDesc &d, u64 *x, int log_n, const u64 *wlo, const u64 *whi, hipStream_t st) {
-    size_t tile = (size_t)1 << (LOGR + d.log_c);
-    size_t grid = ((size_t)1 << log_n) / tile;
-    hipLaunchKernelGGL(k_inv_pass<LOGR>, dim3((unsigned)grid), dim3(BLOCK), tile * sizeof(u64), st, x, d.log_m,
-                       d.log_c, wlo, whi);
-}
+static inline size_t pass_lds(const PassDesc &d) { return ((size_t)1 << (d.log_r + d.log_c)) * sizeof(u64); }
 
 #define GNT_DISPATCH_R(LR, CALL)                                                      \
     switch (LR) {                                                                     \
@@ -264,43 +211,58 @@ static void launch_inv(const PassDesc &d, u64 *x, int log_n, const u64 *wlo, con
         default: return -1;                                                           \
     }
 
-int launch_forward(const Plan &pl, u64 *x, const u64 *limbs, size_t n_limbs, const u64 *wlo, const u64 *whi,
-                   hipStream_t st) {
-    for (int p = 0; p < pl.npass; p++) {
-        const PassDesc &d = pl.pass[p];
+int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs, const u64 *wlo,
+                   const u64 *whi, hipStream_t st) {
+    for (int p = 0; p < npasses; p++) {
+        const PassDesc d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
-#define FWD_CALL(R) launch_fwd<R>(d, x, src, n_limbs, pl.log_n, wlo, whi, st)
+#define FWD_CALL(R) \
+    hipLaunchKernelGGL(k_fwd_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, src, n_limbs, d, wlo, whi)
         GNT_DISPATCH_R(d.log_r, FWD_CALL)
 #undef FWD_CALL
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_inverse(const Plan &pl, u64 *x, const u64 *wlo, const u64 *whi, hipStream_t st) {
-    for (int p = pl.npass - 1; p >= 0; p--) {
-        const PassDesc &d = pl.pass[p];
-#define INV_CALL(R) launch_inv<R>(d, x, pl.log_n, wlo, whi, st)
+int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t nb, u64 *xa, bool square,
+                      const u64 *wlo, const u64 *whi, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
+                      hipStream_t st) {
+    const PassDesc d = pl.pass[pl.npass - 1];
+    const bool only = pl.npass == 1;
+    const u64 *src = only ? limbs_b : nullptr;
+    const u64 *other = square ? nullptr : xa;
+    u64 *ls = only ? Ls : nullptr;
+    u64 *hs = only ? Hs : nullptr;
+#define FUSED_CALL(R)                                                                                            \
+    hipLaunchKernelGGL(k_last_fused<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, xb, src, nb, xa, \
+                       other, d, pl.log_n, wlo, whi, ls, hs, n_coeffs, n_limbs)
+    GNT_DISPATCH_R(d.log_r, FUSED_CALL)
+#undef FUSED_CALL
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_inverse_rest(const Plan &pl, u64 *x, const u64 *wlo, const u64 *whi, u64 *Ls, u64 *Hs, size_t n_coeffs,
+                        size_t n_limbs, hipStream_t st) {
+    for (int p = pl.npass - 2; p >= 0; p--) {
+        const PassDesc d = pl.pass[p];
+        u64 *ls = p == 0 ? Ls : nullptr;
+        u64 *hs = p == 0 ? Hs : nullptr;
+#define INV_CALL(R)                                                                                            \
+    hipLaunchKernelGGL(k_inv_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, d, wlo, whi, ls, \
+                       hs, n_coeffs, n_limbs)
         GNT_DISPATCH_R(d.log_r, INV_CALL)
 #undef INV_CALL
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_pointwise(u64 *a, const u64 *b, int log_n, hipStream_t st) {
-    size_t n = (size_t)1 << log_n;
-    size_t grid = (n + BLOCK - 1) / BLOCK;
-    if (grid > 65536) grid = 65536;
-    hipLaunchKernelGGL(k_pointwise, dim3((unsigned)grid), dim3(BLOCK), 0, st, a, b, n, log_n);
-    return hipGetLastError() == hipSuccess ? 0 : -2;
-}
-
 size_t carry_tiles(size_t n_limbs) { return (n_limbs + CARRY_TILE - 1) / CARRY_TILE; }
 
-int launch_carry(const u64 *coeffs, size_t n_coeffs, u64 *out, size_t n_limbs, unsigned char *tile_gp,
+int launch_carry(const u64 *Ls, const u64 *Hs, u64 *out, size_t n_limbs, unsigned char *tile_gp,
                  unsigned char *tile_cin, u32 *out_len, hipStream_t st) {
     size_t nt = carry_tiles(n_limbs);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, out_len, 1);
-    hipLaunchKernelGGL(k_carry_tiles, dim3((unsigned)nt), dim3(BLOCK), 0, st, coeffs, n_coeffs, out, n_limbs, tile_gp,
+    hipLaunchKernelGGL(k_carry_tiles, dim3((unsigned)nt), dim3(BLOCK), 0, st, Ls, Hs, out, n_limbs, tile_gp,
                        out_len);
     hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, tile_gp, tile_cin, nt);
     hipLaunchKernelGGL(k_carry_ripple, dim3((unsigned)nt), dim3(BLOCK), 0, st, out, n_limbs, tile_cin, out_len);

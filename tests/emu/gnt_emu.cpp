@@ -21,40 +21,47 @@ void tables() {
     if (g_lo.empty()) build_root_tables(g_lo, g_hi);
 }
 
-template <int LOGR, int I, bool FWD>
-void steps(u64 *tile, int log_c, int tile_n, size_t cg0, int log_m, int log_s) {
-    constexpr int count = (LOGR + 3) / 4;
-    if constexpr (I < count) {
-        constexpr int STEP = FWD ? I : count - 1 - I;
-        const int ntask = tile_n >> Sub<LOGR>::log(STEP);
-        for (int L = 0; L < ntask; L++) {
-            if constexpr (FWD) fwd_task<LOGR, STEP>(tile, L, log_c, cg0, log_m, log_s, g_lo.data(), g_hi.data());
-            else inv_task<LOGR, STEP>(tile, L, log_c, cg0, log_m, log_s, g_lo.data(), g_hi.data());
-        }
-        steps<LOGR, I + 1, FWD>(tile, log_c, tile_n, cg0, log_m, log_s);
+using HX = HostExec;
+
+// The three kernel bodies of gnt_kernels.hip, run tile after tile.
+template <int LOGR>
+void k_fwd_pass(const PassDesc &d, int log_n, u64 *x, const u64 *limbs, size_t n_limbs) {
+    std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
+    const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
+    for (size_t t = 0; t < grid; t++) {
+        const TileGeom g = tile_geom<LOGR>(d, t);
+        tile_load<HX, LOGR>(tile.data(), g, x, limbs, n_limbs);
+        tile_fwd<HX, LOGR, 0>(tile.data(), g, g_lo.data(), g_hi.data());
+        tile_store<HX, LOGR>(tile.data(), g, x);
     }
 }
 
-template <int LOGR, bool FWD>
-void pass(const PassDesc &d, int log_n, u64 *x, const u64 *limbs, size_t n_limbs) {
-    const int log_s = d.log_m - LOGR;
-    const int tile_n = 1 << (LOGR + d.log_c);
-    std::vector<u64> tile(tile_n);
-    const size_t grid = ((size_t)1 << log_n) / tile_n;
-    for (size_t blk = 0; blk < grid; blk++) {
-        const size_t cg0 = blk << d.log_c;
-        for (int idx = 0; idx < tile_n; idx++) {
-            int n, c;
-            tile_rc(idx, LOGR, d.log_c, log_s, n, c);
-            size_t a = tile_addr(cg0, n, c, d.log_m, log_s);
-            tile[(n << d.log_c) + c] = limbs ? coeff_from_limbs(limbs, n_limbs, a) : x[a];
-        }
-        steps<LOGR, 0, FWD>(tile.data(), d.log_c, tile_n, cg0, d.log_m, log_s);
-        for (int idx = 0; idx < tile_n; idx++) {
-            int n, c;
-            tile_rc(idx, LOGR, d.log_c, log_s, n, c);
-            x[tile_addr(cg0, n, c, d.log_m, log_s)] = tile[(n << d.log_c) + c];
-        }
+template <int LOGR>
+void k_inv_pass(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
+    std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
+    const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
+    for (size_t t = 0; t < grid; t++) {
+        const TileGeom g = tile_geom<LOGR>(d, t);
+        tile_load<HX, LOGR>(tile.data(), g, x, nullptr, 0);
+        tile_inv<HX, LOGR, Sub<LOGR>::count - 1>(tile.data(), g, g_lo.data(), g_hi.data());
+        if (Ls) tile_limb_sums<HX, LOGR>(tile.data(), g, n_coeffs, n_limbs, Ls, Hs);
+        else tile_store<HX, LOGR>(tile.data(), g, x);
+    }
+}
+
+template <int LOGR>
+void k_last_fused(const PassDesc &d, int log_n, const u64 *xb, const u64 *limbs, size_t n_in, u64 *xa,
+                  const u64 *other, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
+    std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
+    const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
+    for (size_t t = 0; t < grid; t++) {
+        const TileGeom g = tile_geom<LOGR>(d, t);
+        tile_load<HX, LOGR>(tile.data(), g, xb, limbs, n_in);
+        tile_fwd<HX, LOGR, 0>(tile.data(), g, g_lo.data(), g_hi.data());
+        tile_pointwise<HX, LOGR>(tile.data(), g, other, log_n);
+        tile_inv<HX, LOGR, Sub<LOGR>::count - 1>(tile.data(), g, g_lo.data(), g_hi.data());
+        if (Ls) tile_limb_sums<HX, LOGR>(tile.data(), g, n_coeffs, n_limbs, Ls, Hs);
+        else tile_store<HX, LOGR>(tile.data(), g, xa);
     }
 }
 
@@ -68,20 +75,22 @@ void pass(const PassDesc &d, int log_n, u64 *x, const u64 *limbs, size_t n_limbs
         default: break;                                                               \
     }
 
-void forward(const Plan &pl, u64 *x, const u64 *limbs, size_t n_limbs) {
-    for (int p = 0; p < pl.npass; p++) {
+void forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs) {
+    for (int p = 0; p < npasses; p++) {
         const PassDesc &d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
-#define F(R) pass<R, true>(d, pl.log_n, x, src, n_limbs)
+#define F(R) k_fwd_pass<R>(d, pl.log_n, x, src, n_limbs)
         DISPATCH(d.log_r, F)
 #undef F
     }
 }
 
-void inverse(const Plan &pl, u64 *x) {
-    for (int p = pl.npass - 1; p >= 0; p--) {
+void inverse(const Plan &pl, int from, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
+    for (int p = from; p >= 0; p--) {
         const PassDesc &d = pl.pass[p];
-#define F(R) pass<R, false>(d, pl.log_n, x, nullptr, 0)
+        u64 *ls = p == 0 ? Ls : nullptr;
+        u64 *hs = p == 0 ? Hs : nullptr;
+#define F(R) k_inv_pass<R>(d, pl.log_n, x, ls, hs, n_coeffs, n_limbs)
         DISPATCH(d.log_r, F)
 #undef F
     }
@@ -100,12 +109,14 @@ extern "C" {
 // Forward transform of x (length 2^log_n) in place, as the GPU computes it.
 void emu_forward(u64 *x, int log_n) {
     tables();
-    forward(make_plan(log_n), x, nullptr, 0);
+    Plan pl = make_plan(log_n);
+    forward(pl, pl.npass, x, nullptr, 0);
 }
 
 void emu_inverse(u64 *x, int log_n) {
     tables();
-    inverse(make_plan(log_n), x);
+    Plan pl = make_plan(log_n);
+    inverse(pl, pl.npass - 1, x, nullptr, nullptr, 0, 0);
 }
 
 // omega_N of the library (primitive N-th root used by the transforms).
@@ -136,25 +147,39 @@ size_t emu_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
     const Plan pl = make_plan(log_n);
     const size_t N = (size_t)1 << log_n;
     std::vector<u64> xa(N), xb(sqr ? 0 : N);
-    forward(pl, xa.data(), a, na);
-    if (!sqr) forward(pl, xb.data(), b, nb);
-    const u64 *pb = sqr ? xa.data() : xb.data();
-    for (size_t i = 0; i < N; i++) xa[i] = gl::mul_pow2(gl::mul(xa[i], pb[i]), 192 - log_n);
-    inverse(pl, xa.data());
-    // carry, the same limb sums and carry rules as the kernels, run serially
     const size_t nl = na + nb;
+    std::vector<u64> Ls(nl), Hs(nl);
+    const int P = pl.npass;
+    // the same launch sequence as Ctx::enqueue_product
+    if (sqr) {
+        if (P > 1) forward(pl, P - 1, xa.data(), a, na);
+    } else {
+        forward(pl, P, xa.data(), a, na);
+        if (P > 1) forward(pl, P - 1, xb.data(), b, nb);
+    }
+    {
+        const PassDesc &d = pl.pass[P - 1];
+        const bool only = P == 1;
+        const u64 *bsrc = sqr ? xa.data() : xb.data();
+        const u64 *blimbs = only ? (sqr ? a : b) : nullptr;
+        const u64 *other = sqr ? nullptr : xa.data();
+#define F(R) k_last_fused<R>(d, log_n, bsrc, blimbs, sqr ? na : nb, xa.data(), other, only ? Ls.data() : nullptr, \
+                             only ? Hs.data() : nullptr, n_out, nl)
+        DISPATCH(d.log_r, F)
+#undef F
+    }
+    inverse(pl, P - 2, xa.data(), Ls.data(), Hs.data(), n_out, nl);
+    // carry: the same limb rule as the kernels, run serially
     u64 Hprev = 0;
     u32 carry = 0;
     size_t len = 0;
     for (size_t j = 0; j < nl; j++) {
-        u64 L, H;
-        limb_sum(xa.data(), n_out, j, L, H);
-        u64 s = L + Hprev;
-        u32 g = s < L;
+        u64 s = Ls[j] + Hprev;
+        u32 g = s < Ls[j];
         u32 p = s == ~0ull;
         out[j] = s + carry;
         carry = g | (p & carry);
-        Hprev = H;
+        Hprev = Hs[j];
         if (out[j]) len = j + 1;
     }
     return len;
