@@ -53,6 +53,10 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # calibration microbenchmarks (HBM copy, modular-arithmetic throughput)
+    calib = os.path.join(HERE, "gnt_calib")
+    subprocess.run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "gnt_calib.hip"), "-o", calib + ".tmp"], check=True)
+    os.replace(calib + ".tmp", calib)
     return LIB
 
 

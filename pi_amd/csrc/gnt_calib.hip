@@ -1,0 +1,146 @@
+// Calibration microbenchmarks for the roofline numbers (standalone binary).
+//
+//   gnt_calib copy  <bytes>   plain device copy: achievable HBM GB/s, and the
+//                             reference kernel for scaling rocprofv3
+//                             FETCH_SIZE / WRITE_SIZE counters
+//   gnt_calib valu            Goldilocks mul / add+sub / shift-multiply
+//                             throughput (G ops/s over the whole device)
+// Prints one JSON line per measurement.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "gl_field.hpp"
+
+using gl::u64;
+
+#define CK(x)                                                                   \
+    do {                                                                        \
+        hipError_t e = (x);                                                     \
+        if (e != hipSuccess) {                                                  \
+            fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e));              \
+            exit(1);                                                            \
+        }                                                                       \
+    } while (0)
+
+__global__ __launch_bounds__(256) void k_copy(const ulonglong2 *__restrict__ src, ulonglong2 *__restrict__ dst,
+                                              size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+constexpr int CHAINS = 8;
+
+__global__ __launch_bounds__(256) void k_mul(u64 *out, int iters, u64 seed) {
+    u64 v[CHAINS];
+    u64 w = seed + threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < CHAINS; c++) v[c] = seed * (c + 1) + blockIdx.x;
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int c = 0; c < CHAINS; c++) v[c] = gl::mul(v[c], w);
+    }
+    u64 s = 0;
+#pragma unroll
+    for (int c = 0; c < CHAINS; c++) s ^= v[c];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_addsub(u64 *out, int iters, u64 seed) {
+    u64 v[CHAINS];
+    u64 w = seed + threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < CHAINS; c++) v[c] = (seed * (c + 1) + blockIdx.x) % gl::P;
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int c = 0; c < CHAINS; c += 2) {
+            u64 a = gl::add(v[c], v[c + 1]);
+            u64 b = gl::sub(v[c], v[c + 1]);
+            v[c] = a;
+            v[c + 1] = gl::add(b, w);
+        }
+    }
+    u64 s = 0;
+#pragma unroll
+    for (int c = 0; c < CHAINS; c++) s ^= v[c];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_shift(u64 *out, int iters, u64 seed) {
+    u64 v[CHAINS];
+#pragma unroll
+    for (int c = 0; c < CHAINS; c++) v[c] = (seed * (c + 1) + blockIdx.x + threadIdx.x) % gl::P;
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int c = 0; c < CHAINS; c++) v[c] = gl::mul_pow2(v[c], 24);
+    }
+    u64 s = 0;
+#pragma unroll
+    for (int c = 0; c < CHAINS; c++) s ^= v[c];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+static float time_ms(void (*launch)(void *), void *arg, int reps) {
+    hipEvent_t a, b;
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
+    launch(arg);
+    CK(hipDeviceSynchronize());
+    CK(hipEventRecord(a));
+    for (int i = 0; i < reps; i++) launch(arg);
+    CK(hipEventRecord(b));
+    CK(hipEventSynchronize(b));
+    float ms;
+    CK(hipEventElapsedTime(&ms, a, b));
+    return ms / reps;
+}
+
+struct CopyArg { ulonglong2 *src, *dst; size_t n; };
+static void launch_copy(void *p) {
+    CopyArg *c = (CopyArg *)p;
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, 0, c->src, c->dst, c->n);
+}
+
+struct ValuArg { u64 *out; int iters; int kind; };
+static const int VALU_BLOCKS = 256 * 32;
+static void launch_valu(void *p) {
+    ValuArg *v = (ValuArg *)p;
+    if (v->kind == 0) hipLaunchKernelGGL(k_mul, dim3(VALU_BLOCKS), dim3(256), 0, 0, v->out, v->iters, 12345ull);
+    else if (v->kind == 1) hipLaunchKernelGGL(k_addsub, dim3(VALU_BLOCKS), dim3(256), 0, 0, v->out, v->iters, 12345ull);
+    else hipLaunchKernelGGL(k_shift, dim3(VALU_BLOCKS), dim3(256), 0, 0, v->out, v->iters, 12345ull);
+}
+
+int main(int argc, char **argv) {
+    const char *mode = argc > 1 ? argv[1] : "all";
+    if (!strcmp(mode, "copy") || !strcmp(mode, "all")) {
+        size_t bytes = argc > 2 ? strtoull(argv[2], 0, 10) : ((size_t)1 << 30);
+        CopyArg c;
+        c.n = bytes / sizeof(ulonglong2);
+        CK(hipMalloc(&c.src, bytes));
+        CK(hipMalloc(&c.dst, bytes));
+        CK(hipMemset(c.src, 1, bytes));
+        float ms = time_ms(launch_copy, &c, 20);
+        printf("{\"calib\": \"copy\", \"bytes_each_way\": %zu, \"ms\": %.4f, \"GBps_read_plus_write\": %.1f}\n", bytes,
+               ms, 2.0 * bytes / (ms * 1e-3) / 1e9);
+        CK(hipFree(c.src));
+        CK(hipFree(c.dst));
+    }
+    if (!strcmp(mode, "valu") || !strcmp(mode, "all")) {
+        ValuArg v;
+        v.iters = 2048;
+        CK(hipMalloc(&v.out, (size_t)VALU_BLOCKS * 256 * sizeof(u64)));
+        const char *names[3] = {"mul", "add_sub_add", "mul_pow2_24"};
+        const double ops_per_iter[3] = {CHAINS, 1.5 * CHAINS, CHAINS};
+        for (int k = 0; k < 3; k++) {
+            v.kind = k;
+            float ms = time_ms(launch_valu, &v, 5);
+            double ops = (double)VALU_BLOCKS * 256 * v.iters * ops_per_iter[k];
+            printf("{\"calib\": \"%s\", \"Gops_per_s\": %.1f, \"ms\": %.3f}\n", names[k], ops / (ms * 1e-3) / 1e9, ms);
+        }
+        CK(hipFree(v.out));
+    }
+    return 0;
+}

@@ -117,7 +117,13 @@ __global__ __launch_bounds__(BLOCK) void k_carry_tiles(const u64 *__restrict__ L
             if (z != 0) hi_nz = (u32)(j + 1);
         }
     }
-    if (hi_nz) atomicMax(out_len, hi_nz);
+    // one atomic per workgroup for the trimmed length
+    __shared__ u32 tile_hi;
+    if (threadIdx.x == 0) tile_hi = 0;
+    __syncthreads();
+    if (hi_nz) atomicMax(&tile_hi, hi_nz);
+    __syncthreads();
+    if (threadIdx.x == 0 && tile_hi) atomicMax(out_len, tile_hi);
     if (threadIdx.x == BLOCK - 1) tile_gp[blockIdx.x] = (unsigned char)scan[BLOCK - 1];
 }
 
