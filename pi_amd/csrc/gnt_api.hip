@@ -113,13 +113,15 @@ void Pool::trim() {
 }
 
 int Ctx::init_tables() {
-    std::vector<u64> lo, hi;
-    if (!build_root_tables(lo, hi)) return set_error(GNT_E_DEVICE, "root table construction failed");
+    std::vector<u64> lo, hi, sm;
+    if (!build_root_tables(lo, hi, sm)) return set_error(GNT_E_DEVICE, "root table construction failed");
     const size_t T = lo.size();
     HIP_TRY(hipMalloc(&wlo, T * sizeof(u64)));
     HIP_TRY(hipMalloc(&whi, T * sizeof(u64)));
+    HIP_TRY(hipMalloc(&wsm, sm.size() * sizeof(u64)));
     HIP_TRY(hipMemcpy(wlo, lo.data(), T * sizeof(u64), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(whi, hi.data(), T * sizeof(u64), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(wsm, sm.data(), sm.size() * sizeof(u64), hipMemcpyHostToDevice));
     return GNT_OK;
 }
 
@@ -165,19 +167,19 @@ int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64
     const int P = pl.npass;
     int rc = 0;
     if (sqr) {
-        if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, wlo, whi, stream);
+        if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, tw(), stream);
     } else {
-        rc = launch_forward(pl, P, xa, da, na, wlo, whi, stream);
-        if (!rc && P > 1) rc = launch_forward(pl, P - 1, xb, db, nb, wlo, whi, stream);
+        rc = launch_forward(pl, P, xa, da, na, tw(), stream);
+        if (!rc && P > 1) rc = launch_forward(pl, P - 1, xb, db, nb, tw(), stream);
     }
     if (timing) (void)hipEventRecord(ev[1], stream);
     if (!rc) {
         const u64 *bsrc = sqr ? xa : xb;
         const u64 *blimbs = sqr ? da : db;
-        rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, wlo, whi, Ls, Hs, n_out, n_limbs, stream);
+        rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, tw(), Ls, Hs, n_out, n_limbs, stream);
     }
     if (timing) (void)hipEventRecord(ev[2], stream);
-    if (!rc) rc = launch_inverse_rest(pl, xa, wlo, whi, Ls, Hs, n_out, n_limbs, stream);
+    if (!rc) rc = launch_inverse_rest(pl, xa, tw(), Ls, Hs, n_out, n_limbs, stream);
     if (timing) (void)hipEventRecord(ev[3], stream);
     if (!rc) rc = launch_carry(Ls, Hs, dout, n_limbs, gp, gp + nt, dlen, stream);
     if (timing) (void)hipEventRecord(ev[4], stream);
@@ -245,6 +247,7 @@ int gnt_shutdown(gnt_ctx *ctx) {
     c->pool.trim();
     (void)hipFree(c->wlo);
     (void)hipFree(c->whi);
+    (void)hipFree(c->wsm);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return GNT_OK;

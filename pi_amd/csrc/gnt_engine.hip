@@ -1,0 +1,349 @@
+// Stage engine kernels and host logic (see gnt_engine.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "gnt_engine.hpp"
+#include "gnt_pass.hpp"
+
+namespace gnt {
+
+namespace {
+
+constexpr int BLOCK = 256;
+using X = DevExec<BLOCK>;
+constexpr int CARRY_LPT = 4;                  // limbs per thread
+constexpr int CARRY_TILE = BLOCK * CARRY_LPT;  // limbs per carry tile
+constexpr int SCAN_BLOCK = 1024;
+// resident tiles per launch: every CU holds two 64 KiB tiles
+constexpr uint64_t MAX_GRID = 256 * 2 * 4;
+
+size_t ceil_log2(size_t x) {
+    size_t l = 0;
+    while (((size_t)1 << l) < x) l++;
+    return l;
+}
+
+// Largest i with items[i].tile0 <= t (items sorted by tile0, items[0].tile0 == 0).
+template <class T>
+__device__ __forceinline__ int find_item(const T *__restrict__ items, int n, uint64_t t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (items[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int LOGR, int KIND>
+__global__ __launch_bounds__(BLOCK) void k_tpass(const TItem *__restrict__ items, int n_items, uint64_t total, Tw tw) {
+    extern __shared__ uint64_t tile[];
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const TItem it = items[find_item(items, n_items, t)];
+        const TileGeom g = tile_geom<LOGR>(it.d, t - it.tile0);
+        if (KIND == K_FWD) body_fwd<X, LOGR>(tile, g, it.x, it.limbs, it.n_limbs_in, tw);
+        else if (KIND == K_INV) body_inv<X, LOGR>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs);
+        else body_fused<X, LOGR>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls,
+                                 it.Hs, it.n_coeffs, it.n_limbs);
+        __syncthreads();  // the tile buffer is reused by the next iteration
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ctiles(const CItem *__restrict__ items, int n_items, uint64_t total) {
+    __shared__ uint32_t scan[BLOCK];
+    __shared__ uint32_t tile_hi;
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const CItem it = items[find_item(items, n_items, t)];
+        const uint64_t k = t - it.tile0;
+        const size_t j0 = (size_t)k * CARRY_TILE + (size_t)threadIdx.x * CARRY_LPT;
+        uint64_t y[CARRY_LPT];
+        uint32_t gbits = 0, pbits = 0;
+        uint64_t Hprev = (j0 > 0 && j0 - 1 < it.n_limbs) ? it.Hs[j0 - 1] : 0;
+#pragma unroll
+        for (int q = 0; q < CARRY_LPT; q++) {
+            size_t j = j0 + q;
+            uint64_t L = 0, H = 0;
+            if (j < it.n_limbs) { L = it.Ls[j]; H = it.Hs[j]; }
+            uint64_t s = L + Hprev;
+            gbits |= (uint32_t)(s < L) << q;
+            pbits |= (uint32_t)(s == ~0ull) << q;
+            y[q] = s;
+            Hprev = H;
+        }
+        uint32_t agg = 2;
+#pragma unroll
+        for (int q = 0; q < CARRY_LPT; q++) agg = gp_combine(agg, ((gbits >> q) & 1) | (((pbits >> q) & 1) << 1));
+        scan[threadIdx.x] = agg;
+        if (threadIdx.x == 0) tile_hi = 0;
+        __syncthreads();
+        for (int off = 1; off < BLOCK; off <<= 1) {
+            uint32_t v = scan[threadIdx.x];
+            uint32_t w = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 2u;
+            __syncthreads();
+            scan[threadIdx.x] = gp_combine(w, v);
+            __syncthreads();
+        }
+        uint32_t carry = threadIdx.x > 0 ? (scan[threadIdx.x - 1] & 1) : 0;
+        uint32_t hi_nz = 0;
+#pragma unroll
+        for (int q = 0; q < CARRY_LPT; q++) {
+            size_t j = j0 + q;
+            uint64_t z = y[q] + carry;
+            carry = ((gbits >> q) & 1) | (((pbits >> q) & 1) & carry);
+            if (j < it.n_limbs) {
+                it.out[j] = z;
+                if (z != 0) hi_nz = (uint32_t)(j + 1);
+            }
+        }
+        if (hi_nz) atomicMax(&tile_hi, hi_nz);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            it.hi[k] = tile_hi;
+            it.gp[k] = (unsigned char)scan[BLOCK - 1];
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per product: tile carry-ins and the trimmed length.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_cscan(const CItem *__restrict__ items) {
+    __shared__ uint32_t scan[SCAN_BLOCK];
+    __shared__ uint32_t maxhi;
+    const CItem it = items[blockIdx.x];
+    const size_t nt = it.ntiles;
+    const size_t per = (nt + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const size_t t0 = (size_t)threadIdx.x * per;
+    uint32_t agg = 2, mh = 0;
+    for (size_t t = t0; t < t0 + per && t < nt; t++) {
+        agg = gp_combine(agg, it.gp[t]);
+        mh = max(mh, it.hi[t]);
+    }
+    scan[threadIdx.x] = agg;
+    if (threadIdx.x == 0) maxhi = 0;
+    __syncthreads();
+    if (mh) atomicMax(&maxhi, mh);
+    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+        uint32_t v = scan[threadIdx.x];
+        uint32_t w = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 2u;
+        __syncthreads();
+        scan[threadIdx.x] = gp_combine(w, v);
+        __syncthreads();
+    }
+    uint32_t carry = threadIdx.x > 0 ? (scan[threadIdx.x - 1] & 1) : 0;
+    for (size_t t = t0; t < t0 + per && t < nt; t++) {
+        it.cin[t] = (unsigned char)carry;
+        uint32_t gp = it.gp[t];
+        carry = (gp & 1) | ((gp >> 1) & carry);
+    }
+    if (threadIdx.x == 0) *it.out_len = maxhi;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cripple(const CItem *__restrict__ items, int n_items, uint64_t total) {
+    __shared__ uint32_t allmax[BLOCK];
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const CItem it = items[find_item(items, n_items, t)];
+        const uint64_t k = t - it.tile0;
+        if (it.cin[k] == 0) continue;  // uniform per workgroup
+        const size_t j0 = (size_t)k * CARRY_TILE + (size_t)threadIdx.x * CARRY_LPT;
+        uint64_t z[CARRY_LPT];
+        uint32_t am = 1;
+#pragma unroll
+        for (int q = 0; q < CARRY_LPT; q++) {
+            size_t j = j0 + q;
+            z[q] = j < it.n_limbs ? it.out[j] : 0;
+            am &= (j < it.n_limbs && z[q] == ~0ull) ? 1u : 0u;
+        }
+        allmax[threadIdx.x] = am;
+        __syncthreads();
+        for (int off = 1; off < BLOCK; off <<= 1) {
+            uint32_t v = allmax[threadIdx.x];
+            uint32_t w = threadIdx.x >= (unsigned)off ? allmax[threadIdx.x - off] : 1u;
+            __syncthreads();
+            allmax[threadIdx.x] = v & w;
+            __syncthreads();
+        }
+        uint32_t carry = threadIdx.x > 0 ? allmax[threadIdx.x - 1] : 1u;
+        uint32_t hi_nz = 0;
+#pragma unroll
+        for (int q = 0; q < CARRY_LPT; q++) {
+            size_t j = j0 + q;
+            if (carry && j < it.n_limbs) {
+                z[q] += 1;
+                carry = z[q] == 0;
+                it.out[j] = z[q];
+                if (z[q] != 0) hi_nz = (uint32_t)(j + 1);
+            }
+        }
+        if (hi_nz) atomicMax(it.out_len, hi_nz);
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t Prod::cs_bytes() const { return (2 * nt + 15) / 16 * 16 + 4 * nt; }
+
+size_t Prod::scratch_bytes() const {
+    return N * 8 * (sqr() ? 1 : 2) + 2 * nl * 8 + cs_bytes();
+}
+
+bool prod_init(Prod &p) {
+    if (p.sqr()) p.nb = p.na;
+    p.nl = p.na + p.nb;
+    p.n_out = 4 * p.nl - 1;
+    int log_n = (int)ceil_log2(p.n_out);
+    if (log_n > MAX_LOG_N) return false;
+    p.pl = make_plan(log_n);
+    p.N = (size_t)1 << log_n;
+    p.nt = (p.nl + CARRY_TILE - 1) / CARRY_TILE;
+    const int P = p.pl.npass;
+    p.nstages = p.sqr() ? 2 * P + 2 : 2 * P + 3;
+    p.stage = 0;
+    return true;
+}
+
+static TItem titem(const Prod &p, int pass, uint64_t *x, const uint64_t *limbs, size_t n_in) {
+    TItem t{};
+    t.x = x;
+    t.limbs = limbs;
+    t.n_limbs_in = n_in;
+    t.n_coeffs = p.n_out;
+    t.n_limbs = p.nl;
+    t.d = p.pl.pass[pass];
+    t.log_n = p.pl.log_n;
+    return t;
+}
+
+void add_stage(Iteration &it, Prod &p) {
+    const int P = p.pl.npass;
+    const int s = p.stage++;
+    const bool sq = p.sqr();
+    uint64_t *Ls = p.lh, *Hs = p.lh + p.nl;
+    // forward passes before the fused one
+    const int nfwd = sq ? P - 1 : P;  // stages that hold forward passes
+    if (s < nfwd) {
+        TItem a = titem(p, s, p.xa, s == 0 ? p.da : nullptr, p.na);
+        a.pad = K_FWD;
+        it.titems.push_back(a);
+        if (!sq && s < P - 1) {
+            TItem b = titem(p, s, p.xb, s == 0 ? p.db : nullptr, p.nb);
+            b.pad = K_FWD;
+            it.titems.push_back(b);
+        }
+        return;
+    }
+    if (s == nfwd) {
+        TItem f = titem(p, P - 1, sq ? p.xa : p.xb, P == 1 ? (sq ? p.da : p.db) : nullptr, sq ? p.na : p.nb);
+        f.xa = p.xa;
+        f.other = sq ? nullptr : p.xa;
+        if (P == 1) { f.Ls = Ls; f.Hs = Hs; }
+        f.pad = K_FUSED;
+        it.titems.push_back(f);
+        return;
+    }
+    const int inv_stage = s - nfwd - 1;  // 0 .. P-2
+    if (inv_stage < P - 1) {
+        const int pass = P - 2 - inv_stage;
+        TItem v = titem(p, pass, p.xa, nullptr, 0);
+        if (pass == 0) { v.Ls = Ls; v.Hs = Hs; }
+        v.pad = K_INV;
+        it.titems.push_back(v);
+        return;
+    }
+    const int cstep = inv_stage - (P - 1);  // 0, 1, 2
+    CItem c{};
+    c.Ls = Ls;
+    c.Hs = Hs;
+    c.out = p.dout;
+    c.gp = p.cs;
+    c.cin = p.cs + p.nt;
+    c.hi = (uint32_t *)(p.cs + (2 * p.nt + 15) / 16 * 16);
+    c.out_len = p.dlen;
+    c.n_limbs = p.nl;
+    c.ntiles = p.nt;
+    c.tile0 = (uint64_t)cstep;  // the step, resolved in finalize_iteration
+    it.citems.push_back(c);
+}
+
+void finalize_iteration(Iteration &it) {
+    // transform items: group by (kind, radix)
+    std::stable_sort(it.titems.begin(), it.titems.end(), [](const TItem &a, const TItem &b) {
+        if (a.pad != b.pad) return a.pad < b.pad;
+        return a.d.log_r < b.d.log_r;
+    });
+    it.groups.clear();
+    for (size_t i = 0; i < it.titems.size();) {
+        size_t j = i;
+        uint64_t tiles = 0;
+        int lds_log = 0;
+        while (j < it.titems.size() && it.titems[j].pad == it.titems[i].pad &&
+               it.titems[j].d.log_r == it.titems[i].d.log_r) {
+            const TItem &t = it.titems[j];
+            it.titems[j].tile0 = tiles;
+            tiles += ((uint64_t)1 << t.log_n) >> (t.d.log_r + t.d.log_c);
+            lds_log = std::max(lds_log, t.d.log_r + t.d.log_c);
+            j++;
+        }
+        it.groups.push_back(Group{it.titems[i].pad, it.titems[i].d.log_r, i, j - i, tiles, lds_log});
+        i = j;
+    }
+    // carry items: group by step
+    std::stable_sort(it.citems.begin(), it.citems.end(),
+                     [](const CItem &a, const CItem &b) { return a.tile0 < b.tile0; });
+    for (size_t i = 0; i < it.citems.size();) {
+        size_t j = i;
+        const uint64_t step = it.citems[i].tile0;
+        uint64_t tiles = 0;
+        while (j < it.citems.size() && it.citems[j].tile0 == step) j++;
+        for (size_t q = i; q < j; q++) {
+            it.citems[q].tile0 = tiles;
+            tiles += step == 1 ? 1 : it.citems[q].ntiles;
+        }
+        it.groups.push_back(Group{10 + (int)step, 0, i, j - i, tiles, 0});
+        i = j;
+    }
+}
+
+#define GNT_DISPATCH_R(LR, CALL)                                                      \
+    switch (LR) {                                                                     \
+        case 1: CALL(1); break;  case 2: CALL(2); break;  case 3: CALL(3); break;     \
+        case 4: CALL(4); break;  case 5: CALL(5); break;  case 6: CALL(6); break;     \
+        case 7: CALL(7); break;  case 8: CALL(8); break;  case 9: CALL(9); break;     \
+        case 10: CALL(10); break; case 11: CALL(11); break; case 12: CALL(12); break; \
+        case 13: CALL(13); break;                                                     \
+        default: return -1;                                                           \
+    }
+
+template <int KIND>
+static int launch_tgroup(const Group &g, const TItem *dev, const Tw &tw, hipStream_t st) {
+    const size_t lds = ((size_t)1 << g.lds_log) * 8;
+    dim3 grid((unsigned)std::min<uint64_t>(g.tiles, MAX_GRID));
+    const TItem *items = dev + g.first;
+    const int n = (int)g.count;
+#define TP_CALL(R) hipLaunchKernelGGL((k_tpass<R, KIND>), grid, dim3(BLOCK), lds, st, items, n, g.tiles, tw)
+    GNT_DISPATCH_R(g.log_r, TP_CALL)
+#undef TP_CALL
+    return 0;
+}
+
+int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c, const Tw &tw, hipStream_t st) {
+    for (const Group &g : it.groups) {
+        int rc = 0;
+        if (g.kind == K_FWD) rc = launch_tgroup<K_FWD>(g, dev_t, tw, st);
+        else if (g.kind == K_INV) rc = launch_tgroup<K_INV>(g, dev_t, tw, st);
+        else if (g.kind == K_FUSED) rc = launch_tgroup<K_FUSED>(g, dev_t, tw, st);
+        else {
+            const CItem *items = dev_c + g.first;
+            const int n = (int)g.count;
+            dim3 grid((unsigned)std::min<uint64_t>(g.tiles, 65536));
+            if (g.kind == 10) hipLaunchKernelGGL(k_ctiles, grid, dim3(BLOCK), 0, st, items, n, g.tiles);
+            else if (g.kind == 11) hipLaunchKernelGGL(k_cscan, dim3((unsigned)n), dim3(SCAN_BLOCK), 0, st, items);
+            else hipLaunchKernelGGL(k_cripple, grid, dim3(BLOCK), 0, st, items, n, g.tiles);
+        }
+        if (rc) return rc;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace gnt

@@ -1,0 +1,97 @@
+// Stage engine: many products of uneven size advanced together.
+//
+// Every product is a fixed sequence of stages (forward passes, the fused
+// last pass, inverse passes, three carry steps).  One engine iteration
+// advances every product in flight by one stage; all tile-passes of that
+// iteration that share a kernel (kind, radix) go out as ONE launch over a
+// device table of items, so a level of N independent products costs a handful
+// of launches instead of ~10 per product.  The same iterations serve the
+// batched API (all products admitted at once, optionally captured as one
+// hipGraph and replayed) and the serving loop (products admitted between
+// iterations as memory frees up).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "gnt_pass.hpp"
+#include "gnt_plan.hpp"
+
+namespace gnt {
+
+enum ItemKind : int { K_FWD = 0, K_INV = 1, K_FUSED = 2 };
+
+// One tile-pass over one array (device descriptor).
+struct TItem {
+    uint64_t *x;            // FWD/INV: array in place; FUSED: B's array (source)
+    const uint64_t *limbs;  // pack source for a first pass, else null
+    uint64_t *xa;           // FUSED: A's transform and output
+    const uint64_t *other;  // FUSED: A (null squares)
+    uint64_t *Ls, *Hs;      // limb sums (last inverse pass), else null
+    uint64_t n_limbs_in;
+    uint64_t n_coeffs;
+    uint64_t n_limbs;
+    uint64_t tile0;         // first launch-wide tile index of this item
+    PassDesc d;
+    int log_n;
+    int pad;
+};
+
+// One product's carry (device descriptor).
+struct CItem {
+    const uint64_t *Ls, *Hs;
+    uint64_t *out;
+    unsigned char *gp, *cin;  // per carry tile
+    uint32_t *hi;             // per carry tile: highest nonzero limb + 1
+    uint32_t *out_len;
+    uint64_t n_limbs;
+    uint64_t ntiles;
+    uint64_t tile0;
+};
+
+// A product in flight (host side).
+struct Prod {
+    const uint64_t *da = nullptr, *db = nullptr;  // device limbs (db null: square)
+    size_t na = 0, nb = 0;
+    uint64_t *dout = nullptr;
+    uint32_t *dlen = nullptr;
+    Plan pl;
+    size_t N = 0, n_out = 0, nl = 0, nt = 0;
+    uint64_t *xa = nullptr, *xb = nullptr, *lh = nullptr;
+    unsigned char *cs = nullptr;  // carry scratch: gp[nt] cin[nt] pad, hi[nt] u32
+    int stage = 0;
+    int nstages = 0;
+    bool sqr() const { return db == nullptr; }
+    size_t scratch_bytes() const;            // xa + xb + lh + cs
+    size_t cs_bytes() const;
+};
+
+// Fill the derived fields of p (plan, sizes, stage count).  False if too large.
+bool prod_init(Prod &p);
+
+// Launch groups of one iteration.
+struct Group {
+    int kind;   // K_FWD/K_INV/K_FUSED, or 10+k for carry step k
+    int log_r;
+    size_t first, count;  // range in the iteration's item table
+    uint64_t tiles;       // total tiles of the launch
+    int lds_log;          // log2 of the largest tile (u64 elements) in the group
+};
+
+struct Iteration {
+    std::vector<TItem> titems;
+    std::vector<CItem> citems;
+    std::vector<Group> groups;
+};
+
+// Append the stage p.stage of product p to `it` and advance p.stage.
+void add_stage(Iteration &it, Prod &p);
+// Sort the items into launch groups (kind, radix) and fill tile offsets.
+void finalize_iteration(Iteration &it);
+// Launch one iteration whose items were copied to dev_t / dev_c.
+int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c, const Tw &tw, hipStream_t st);
+
+}  // namespace gnt

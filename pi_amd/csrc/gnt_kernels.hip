@@ -22,45 +22,30 @@ using X = DevExec<BLOCK>;
 
 template <int LOGR>
 __global__ __launch_bounds__(BLOCK) void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs,
-                                                    size_t n_limbs, PassDesc d, const u64 *__restrict__ wlo,
-                                                    const u64 *__restrict__ whi) {
+                                                    size_t n_limbs, PassDesc d, Tw tw) {
     extern __shared__ u64 tile[];
-    const TileGeom g = tile_geom<LOGR>(d, blockIdx.x);
-    tile_load<X, LOGR>(tile, g, x, limbs, n_limbs);
-    tile_fwd<X, LOGR, 0>(tile, g, wlo, whi);
-    tile_store<X, LOGR>(tile, g, x);
+    body_fwd<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), x, limbs, n_limbs, tw);
 }
 
 // Inverse pass; with Ls set it is the last one and writes limb sums.
 template <int LOGR>
-__global__ __launch_bounds__(BLOCK) void k_inv_pass(u64 *__restrict__ x, PassDesc d, const u64 *__restrict__ wlo,
-                                                    const u64 *__restrict__ whi, u64 *__restrict__ Ls,
+__global__ __launch_bounds__(BLOCK) void k_inv_pass(u64 *__restrict__ x, PassDesc d, Tw tw, u64 *__restrict__ Ls,
                                                     u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
     extern __shared__ u64 tile[];
-    const TileGeom g = tile_geom<LOGR>(d, blockIdx.x);
-    tile_load<X, LOGR>(tile, g, x, nullptr, 0);
-    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, wlo, whi);
-    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
-    else tile_store<X, LOGR>(tile, g, x);
+    body_inv<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), x, tw, Ls, Hs, n_coeffs, n_limbs);
 }
 
 // Last forward pass of B (read from xb, or packed from limbs), pointwise with
-// A (xa at the same positions; xa == nullptr squares), 1/N, first inverse
-// pass; result to xa, or as limb sums when this is the only pass.
+// A (a_other; null squares), 1/N, first inverse pass; result to xa, or as
+// limb sums when this is the only pass.
 template <int LOGR>
 __global__ __launch_bounds__(BLOCK) void k_last_fused(const u64 *__restrict__ xb, const u64 *__restrict__ limbs,
                                                       size_t n_limbs_in, u64 *xa, const u64 *a_other, PassDesc d,
-                                                      int log_n, const u64 *__restrict__ wlo,
-                                                      const u64 *__restrict__ whi, u64 *__restrict__ Ls,
+                                                      int log_n, Tw tw, u64 *__restrict__ Ls,
                                                       u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
     extern __shared__ u64 tile[];
-    const TileGeom g = tile_geom<LOGR>(d, blockIdx.x);
-    tile_load<X, LOGR>(tile, g, xb, limbs, n_limbs_in);
-    tile_fwd<X, LOGR, 0>(tile, g, wlo, whi);
-    tile_pointwise<X, LOGR>(tile, g, a_other, log_n);
-    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, wlo, whi);
-    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
-    else tile_store<X, LOGR>(tile, g, xa);
+    body_fused<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), xb, limbs, n_limbs_in, xa, a_other, log_n, tw, Ls,
+                        Hs, n_coeffs, n_limbs);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,13 +202,13 @@ static inline size_t pass_lds(const PassDesc &d) { return ((size_t)1 << (d.log_r
         default: return -1;                                                           \
     }
 
-int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs, const u64 *wlo,
-                   const u64 *whi, hipStream_t st) {
+int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw,
+                   hipStream_t st) {
     for (int p = 0; p < npasses; p++) {
         const PassDesc d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
 #define FWD_CALL(R) \
-    hipLaunchKernelGGL(k_fwd_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, src, n_limbs, d, wlo, whi)
+    hipLaunchKernelGGL(k_fwd_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, src, n_limbs, d, tw)
         GNT_DISPATCH_R(d.log_r, FWD_CALL)
 #undef FWD_CALL
     }
@@ -231,8 +216,7 @@ int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t
 }
 
 int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t nb, u64 *xa, bool square,
-                      const u64 *wlo, const u64 *whi, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
-                      hipStream_t st) {
+                      const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs, hipStream_t st) {
     const PassDesc d = pl.pass[pl.npass - 1];
     const bool only = pl.npass == 1;
     const u64 *src = only ? limbs_b : nullptr;
@@ -241,21 +225,21 @@ int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t 
     u64 *hs = only ? Hs : nullptr;
 #define FUSED_CALL(R)                                                                                            \
     hipLaunchKernelGGL(k_last_fused<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, xb, src, nb, xa, \
-                       other, d, pl.log_n, wlo, whi, ls, hs, n_coeffs, n_limbs)
+                       other, d, pl.log_n, tw, ls, hs, n_coeffs, n_limbs)
     GNT_DISPATCH_R(d.log_r, FUSED_CALL)
 #undef FUSED_CALL
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_inverse_rest(const Plan &pl, u64 *x, const u64 *wlo, const u64 *whi, u64 *Ls, u64 *Hs, size_t n_coeffs,
-                        size_t n_limbs, hipStream_t st) {
+int launch_inverse_rest(const Plan &pl, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
+                        hipStream_t st) {
     for (int p = pl.npass - 2; p >= 0; p--) {
         const PassDesc d = pl.pass[p];
         u64 *ls = p == 0 ? Ls : nullptr;
         u64 *hs = p == 0 ? Hs : nullptr;
 #define INV_CALL(R)                                                                                            \
-    hipLaunchKernelGGL(k_inv_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, d, wlo, whi, ls, \
-                       hs, n_coeffs, n_limbs)
+    hipLaunchKernelGGL(k_inv_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, d, tw, ls, hs, \
+                       n_coeffs, n_limbs)
         GNT_DISPATCH_R(d.log_r, INV_CALL)
 #undef INV_CALL
     }

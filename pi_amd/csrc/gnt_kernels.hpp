@@ -5,6 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "gnt_pass.hpp"
 #include "gnt_plan.hpp"
 
 namespace gnt {
@@ -12,17 +13,17 @@ namespace gnt {
 // Forward passes 0 .. npasses-1 in place on x (N = 2^pl.log_n u64).  With
 // `limbs` set, pass 0 packs the operand's limbs (16 bits per coefficient,
 // zero tail) instead of reading x.
-int launch_forward(const Plan &pl, int npasses, uint64_t *x, const uint64_t *limbs, size_t n_limbs,
-                   const uint64_t *wlo, const uint64_t *whi, hipStream_t st);
+int launch_forward(const Plan &pl, int npasses, uint64_t *x, const uint64_t *limbs, size_t n_limbs, const Tw &tw,
+                   hipStream_t st);
 // Last forward pass of B + pointwise with A + 1/N + first inverse pass, result
 // in xa.  square: B is A (xb == xa) and the pointwise product is A*A.  For a
 // single-pass plan B is packed from limbs_b and the output is limb sums.
 int launch_last_fused(const Plan &pl, const uint64_t *xb, const uint64_t *limbs_b, size_t nb, uint64_t *xa,
-                      bool square, const uint64_t *wlo, const uint64_t *whi, uint64_t *Ls, uint64_t *Hs,
-                      size_t n_coeffs, size_t n_limbs, hipStream_t st);
+                      bool square, const Tw &tw, uint64_t *Ls, uint64_t *Hs, size_t n_coeffs, size_t n_limbs,
+                      hipStream_t st);
 // Inverse passes npass-2 .. 0; pass 0 writes limb sums Ls/Hs (n_limbs each).
-int launch_inverse_rest(const Plan &pl, uint64_t *x, const uint64_t *wlo, const uint64_t *whi, uint64_t *Ls,
-                        uint64_t *Hs, size_t n_coeffs, size_t n_limbs, hipStream_t st);
+int launch_inverse_rest(const Plan &pl, uint64_t *x, const Tw &tw, uint64_t *Ls, uint64_t *Hs, size_t n_coeffs,
+                        size_t n_limbs, hipStream_t st);
 // Exact carry of the limb sums into n_limbs limbs; *out_len receives the
 // trimmed length.  tile_gp / tile_cin hold carry_tiles(n_limbs) bytes.
 size_t carry_tiles(size_t n_limbs);

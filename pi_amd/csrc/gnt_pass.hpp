@@ -36,10 +36,16 @@ struct HostExec {
 
 #define GNT_FOR(X, L, n) for (int L = X::first(); L < (n); L += X::step)
 
+// Twiddle tables (device or host copies):
+//   wlo[i] = r^i, whi[i] = r^(i 2^16) for i < 2^16   (r the library's 2^32-th root)
+//   wsm[2^LM + e] = omega_{2^LM}^e for e < 2^LM, LM <= TILE_LOG   (compact, per size)
+struct Tw {
+    const u64 *wlo, *whi, *wsm;
+};
+constexpr int WSM_SIZE = 2 << TILE_LOG;
+
 // omega_{2^32}^e from the two-level table.
-GL_HD u64 root_pow(const u64 *__restrict__ wlo, const u64 *__restrict__ whi, u32 e) {
-    return gl::mul(wlo[e & ((1u << TW_SPLIT) - 1)], whi[e >> TW_SPLIT]);
-}
+GL_HD u64 root_pow(const Tw &tw, u32 e) { return gl::mul(tw.wlo[e & ((1u << TW_SPLIT) - 1)], tw.whi[e >> TW_SPLIT]); }
 
 template <int LOGR>
 struct Sub {
@@ -48,6 +54,29 @@ struct Sub {
     // block size (log) seen by sub-step i inside the R-point column DFT
     static constexpr int log_m(int i) { return i == 0 ? LOGR : log_m(i - 1) - log(i - 1); }
 };
+
+struct TileGeom {
+    int log_m, log_c, log_s, tile_n;
+    size_t cg0;
+};
+
+template <int LOGR>
+GL_HD TileGeom tile_geom(const PassDesc &d, size_t tile_index) {
+    TileGeom g;
+    g.log_m = d.log_m;
+    g.log_c = d.log_c;
+    g.log_s = d.log_m - LOGR;
+    g.tile_n = 1 << (LOGR + d.log_c);
+    g.cg0 = tile_index << d.log_c;
+    return g;
+}
+
+// LDS position of tile element (row n, column c).  Columns are XOR-swizzled
+// by the row so that walking down a column (the contiguous last pass) and
+// walking along a row both hit distinct banks.
+GL_HD int tix(const TileGeom &g, int n, int c) {
+    return (n << g.log_c) + (c ^ (n & ((1 << g.log_c) - 1)));
+}
 
 // Global address of tile element (row n, tile column c).
 GL_HD size_t tile_addr(size_t cg0, int n, int c, int log_m, int log_s) {
@@ -71,105 +100,65 @@ GL_HD u64 coeff_from_limbs(const u64 *__restrict__ limbs, size_t n_limbs, size_t
     return (limbs[l] >> (16 * (i & 3))) & 0xFFFFull;
 }
 
-// Sub-step I of the forward R-point column DFT for task L (DIF, Bailey form):
-// an r-point DFT over rows base + u*t, then the twiddle omega_{2^LM}^(jj*k1),
-// or, after the last sub-step, the pass twiddle omega_M^(j*bitrev_R(p)).
+// Sub-step I of the forward R-point column DFT for task L (DIF): an r-point
+// DFT over rows base + u*t of column c, then the twiddle omega_{2^LM}^(jj*k1)
+// (none after the last sub-step; the pass twiddle is applied on the way out).
 template <int LOGR, int I>
-GL_HD void fwd_task(u64 *tile, int L, int log_c, size_t cg0, int log_m, int log_s, const u64 *__restrict__ wlo,
-                    const u64 *__restrict__ whi) {
+GL_HD void fwd_task(u64 *tile, const TileGeom &g, int L, const Tw &tw) {
     constexpr int A = Sub<LOGR>::log(I);
     constexpr int R = 1 << A;
     constexpr int LM = Sub<LOGR>::log_m(I);
     constexpr int LT = LM - A;
     constexpr bool LAST = (I == Sub<LOGR>::count - 1);
-    const int c = L & ((1 << log_c) - 1);
-    const int rest = L >> log_c;
+    const int c = L & ((1 << g.log_c) - 1);
+    const int rest = L >> g.log_c;
     const int jj = rest & ((1 << LT) - 1);
     const int blk = rest >> LT;
     const int base = (blk << LM) + jj;
     u64 v[R];
 #pragma unroll
-    for (int u = 0; u < R; u++) v[u] = tile[((base + (u << LT)) << log_c) + c];
+    for (int u = 0; u < R; u++) v[u] = tile[tix(g, base + (u << LT), c)];
     gl::dft_dif<R>(v);
     if (!LAST) {
-        // omega_{2^LM}^e = whi[e << (16 - LM)] (LM <= 13): one lookup
+        const u64 *w = tw.wsm + (1 << LM);
 #pragma unroll
-        for (int q = 1; q < R; q++) {
-            u32 k1 = gl::bitrev((u32)q, A);
-            v[q] = gl::mul(v[q], whi[((u32)jj * k1) << (TW_SPLIT - LM)]);
-        }
-    } else if (log_s > 0) {
-        size_t j = (cg0 + (size_t)c) & (((size_t)1 << log_s) - 1);
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            u32 p = (u32)(base + (q << LT));
-            u32 kc = gl::bitrev(p, LOGR);
-            u64 ex = ((u64)j * kc) << (ROOT_LOG - log_m);
-            v[q] = gl::mul(v[q], root_pow(wlo, whi, (u32)ex));
-        }
+        for (int q = 1; q < R; q++) v[q] = gl::mul(v[q], w[(u32)jj * gl::bitrev((u32)q, A)]);
     }
 #pragma unroll
-    for (int q = 0; q < R; q++) tile[((base + (q << LT)) << log_c) + c] = v[q];
+    for (int q = 0; q < R; q++) tile[tix(g, base + (q << LT), c)] = v[q];
 }
 
 // Inverse of fwd_task<LOGR, I>: undo the twiddle, then the inverse r-point
 // DFT (DIT, bit-reversed input, natural output).  Unscaled.
 template <int LOGR, int I>
-GL_HD void inv_task(u64 *tile, int L, int log_c, size_t cg0, int log_m, int log_s, const u64 *__restrict__ wlo,
-                    const u64 *__restrict__ whi) {
+GL_HD void inv_task(u64 *tile, const TileGeom &g, int L, const Tw &tw) {
     constexpr int A = Sub<LOGR>::log(I);
     constexpr int R = 1 << A;
     constexpr int LM = Sub<LOGR>::log_m(I);
     constexpr int LT = LM - A;
     constexpr bool LAST = (I == Sub<LOGR>::count - 1);
-    const int c = L & ((1 << log_c) - 1);
-    const int rest = L >> log_c;
+    const int c = L & ((1 << g.log_c) - 1);
+    const int rest = L >> g.log_c;
     const int jj = rest & ((1 << LT) - 1);
     const int blk = rest >> LT;
     const int base = (blk << LM) + jj;
     u64 v[R];
 #pragma unroll
-    for (int q = 0; q < R; q++) v[q] = tile[((base + (q << LT)) << log_c) + c];
+    for (int q = 0; q < R; q++) v[q] = tile[tix(g, base + (q << LT), c)];
     if (!LAST) {
+        const u64 *w = tw.wsm + (1 << LM);
 #pragma unroll
         for (int q = 1; q < R; q++) {
-            u32 k1 = gl::bitrev((u32)q, A);
-            u32 e = ((u32)jj * k1) & ((1u << LM) - 1);
-            // omega^-e = omega^(2^LM - e)
-            v[q] = gl::mul(v[q], whi[(((1u << LM) - e) & ((1u << LM) - 1)) << (TW_SPLIT - LM)]);
-        }
-    } else if (log_s > 0) {
-        size_t j = (cg0 + (size_t)c) & (((size_t)1 << log_s) - 1);
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            u32 p = (u32)(base + (q << LT));
-            u32 kc = gl::bitrev(p, LOGR);
-            u64 ex = ((u64)j * kc) << (ROOT_LOG - log_m);
-            v[q] = gl::mul(v[q], root_pow(wlo, whi, 0u - (u32)ex));
+            u32 e = ((u32)jj * gl::bitrev((u32)q, A)) & ((1u << LM) - 1);
+            v[q] = gl::mul(v[q], w[((1u << LM) - e) & ((1u << LM) - 1)]);  // omega^-e
         }
     }
     gl::dft_dit_inv<R>(v);
 #pragma unroll
-    for (int u = 0; u < R; u++) tile[((base + (u << LT)) << log_c) + c] = v[u];
+    for (int u = 0; u < R; u++) tile[tix(g, base + (u << LT), c)] = v[u];
 }
 
 // ---- tile-level routines ----
-
-struct TileGeom {
-    int log_m, log_c, log_s, tile_n;
-    size_t cg0;
-};
-
-template <int LOGR>
-GL_HD TileGeom tile_geom(const PassDesc &d, size_t tile_index) {
-    TileGeom g;
-    g.log_m = d.log_m;
-    g.log_c = d.log_c;
-    g.log_s = d.log_m - LOGR;
-    g.tile_n = 1 << (LOGR + d.log_c);
-    g.cg0 = tile_index << d.log_c;
-    return g;
-}
 
 // Load a tile from x, or (limbs != nullptr) pack it from the operand's limbs.
 template <class X, int LOGR>
@@ -179,7 +168,7 @@ GL_HD void tile_load(u64 *tile, const TileGeom &g, const u64 *__restrict__ x, co
         int n, c;
         tile_rc(idx, LOGR, g.log_c, g.log_s, n, c);
         size_t a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
-        tile[(n << g.log_c) + c] = limbs ? coeff_from_limbs(limbs, n_limbs, a) : x[a];
+        tile[tix(g, n, c)] = limbs ? coeff_from_limbs(limbs, n_limbs, a) : x[a];
     }
     X::sync();
 }
@@ -189,28 +178,69 @@ GL_HD void tile_store(const u64 *tile, const TileGeom &g, u64 *__restrict__ x) {
     GNT_FOR(X, idx, g.tile_n) {
         int n, c;
         tile_rc(idx, LOGR, g.log_c, g.log_s, n, c);
-        x[tile_addr(g.cg0, n, c, g.log_m, g.log_s)] = tile[(n << g.log_c) + c];
+        x[tile_addr(g.cg0, n, c, g.log_m, g.log_s)] = tile[tix(g, n, c)];
     }
 }
 
+// Pass twiddle omega_M^(j*kc), kc = bitrev_R(p), j = j0 + c the element's
+// column (the tile's columns are consecutive, j0 = cg0 mod s).  Each task owns
+// a run of up to 16 consecutive columns of one row and steps the twiddle by
+// omega_M^kc along it, so a run costs two table lookups.  FWD: tile -> x with
+// the twiddle applied; !FWD: x -> tile with the inverse twiddle.
+template <int LOGR, bool FWD>
+GL_HD void twiddle_run(u64 *tile, const TileGeom &g, u64 *__restrict__ x, const Tw &tw, int L) {
+    const int lc = g.log_c;
+    const int lch = lc < 4 ? lc : 4;
+    const int CH = 1 << lch;
+    const int p = L >> (lc - lch);
+    const int c0 = (L & ((1 << (lc - lch)) - 1)) << lch;
+    const u32 kc = gl::bitrev((u32)p, LOGR);
+    const size_t j = (g.cg0 & (((size_t)1 << g.log_s) - 1)) + (size_t)c0;
+    const int sh = ROOT_LOG - g.log_m;
+    u32 ew = (u32)(((u64)j * kc) << sh);
+    u32 eu = (u32)((u64)kc << sh);
+    if (!FWD) { ew = 0u - ew; eu = 0u - eu; }
+    u64 w = root_pow(tw, ew);
+    const u64 u = root_pow(tw, eu);
+    const size_t a = tile_addr(g.cg0, p, c0, g.log_m, g.log_s);
+    for (int c = 0; c < CH; c++) {
+        if (FWD) x[a + c] = gl::mul(tile[tix(g, p, c0 + c)], w);
+        else tile[tix(g, p, c0 + c)] = gl::mul(x[a + c], w);
+        w = gl::mul(w, u);
+    }
+}
+
+template <class X, int LOGR>
+GL_HD void tile_store_tw(u64 *tile, const TileGeom &g, u64 *__restrict__ x, const Tw &tw) {
+    const int lch = g.log_c < 4 ? g.log_c : 4;
+    GNT_FOR(X, L, g.tile_n >> lch) twiddle_run<LOGR, true>(tile, g, x, tw, L);
+}
+
+template <class X, int LOGR>
+GL_HD void tile_load_tw(u64 *tile, const TileGeom &g, const u64 *__restrict__ x, const Tw &tw) {
+    const int lch = g.log_c < 4 ? g.log_c : 4;
+    GNT_FOR(X, L, g.tile_n >> lch) twiddle_run<LOGR, false>(tile, g, const_cast<u64 *>(x), tw, L);
+    X::sync();
+}
+
 template <class X, int LOGR, int I>
-GL_HD void tile_fwd(u64 *tile, const TileGeom &g, const u64 *wlo, const u64 *whi) {
+GL_HD void tile_fwd(u64 *tile, const TileGeom &g, const Tw &tw) {
     if constexpr (I < Sub<LOGR>::count) {
         const int ntask = g.tile_n >> Sub<LOGR>::log(I);
-        GNT_FOR(X, L, ntask) fwd_task<LOGR, I>(tile, L, g.log_c, g.cg0, g.log_m, g.log_s, wlo, whi);
+        GNT_FOR(X, L, ntask) fwd_task<LOGR, I>(tile, g, L, tw);
         X::sync();
-        tile_fwd<X, LOGR, I + 1>(tile, g, wlo, whi);
+        tile_fwd<X, LOGR, I + 1>(tile, g, tw);
     }
 }
 
 // Inverse sub-steps I, I-1, ..., 0.
 template <class X, int LOGR, int I>
-GL_HD void tile_inv(u64 *tile, const TileGeom &g, const u64 *wlo, const u64 *whi) {
+GL_HD void tile_inv(u64 *tile, const TileGeom &g, const Tw &tw) {
     if constexpr (I >= 0) {
         const int ntask = g.tile_n >> Sub<LOGR>::log(I);
-        GNT_FOR(X, L, ntask) inv_task<LOGR, I>(tile, L, g.log_c, g.cg0, g.log_m, g.log_s, wlo, whi);
+        GNT_FOR(X, L, ntask) inv_task<LOGR, I>(tile, g, L, tw);
         X::sync();
-        tile_inv<X, LOGR, I - 1>(tile, g, wlo, whi);
+        tile_inv<X, LOGR, I - 1>(tile, g, tw);
     }
 }
 
@@ -222,7 +252,7 @@ GL_HD void tile_pointwise(u64 *tile, const TileGeom &g, const u64 *__restrict__ 
     GNT_FOR(X, idx, g.tile_n) {
         int n, c;
         tile_rc(idx, LOGR, g.log_c, g.log_s, n, c);
-        const int t = (n << g.log_c) + c;
+        const int t = tix(g, n, c);
         u64 v = tile[t];
         u64 w = other ? other[tile_addr(g.cg0, n, c, g.log_m, g.log_s)] : v;
         tile[t] = gl::mul_pow2(gl::mul(v, w), e);
@@ -240,17 +270,18 @@ GL_HD void tile_limb_sums(const u64 *tile, const TileGeom &g, size_t n_coeffs, s
                           u64 *__restrict__ Hs) {
     const int nq = g.tile_n >> 2;
     GNT_FOR(X, q, nq) {
-        // four consecutive coefficients: along columns (log_s > 0) or rows
+        // four consecutive coefficients: along a row (log_s > 0) or down a column
         int n, c;
         tile_rc(q << 2, LOGR, g.log_c, g.log_s, n, c);
         size_t a0 = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
         size_t j = a0 >> 2;
         if (j < n_limbs) {
-            const int t0 = (n << g.log_c) + c;
-            const int step = g.log_s > 0 ? 1 : (1 << g.log_c);
             u64 cc[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) cc[k] = (a0 + k < n_coeffs) ? tile[t0 + k * step] : 0;
+            for (int k = 0; k < 4; k++) {
+                int t = g.log_s > 0 ? tix(g, n, c + k) : tix(g, n + k, c);
+                cc[k] = (a0 + k < n_coeffs) ? tile[t] : 0;
+            }
             u64 lo = cc[0], hi = 0, t;
             t = cc[1] << 16; lo += t; hi += (cc[1] >> 48) + (lo < t);
             t = cc[2] << 32; lo += t; hi += (cc[2] >> 32) + (lo < t);
@@ -259,6 +290,44 @@ GL_HD void tile_limb_sums(const u64 *tile, const TileGeom &g, size_t n_coeffs, s
             Hs[j] = hi;
         }
     }
+}
+
+// ---- pass bodies: what each kernel runs on one tile ----
+
+// Forward pass.  With `limbs` set the tile is packed from the operand's limbs.
+template <class X, int LOGR>
+GL_HD void body_fwd(u64 *tile, const TileGeom &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw) {
+    tile_load<X, LOGR>(tile, g, x, limbs, n_limbs);
+    tile_fwd<X, LOGR, 0>(tile, g, tw);
+    if (g.log_s > 0) tile_store_tw<X, LOGR>(tile, g, x, tw);
+    else tile_store<X, LOGR>(tile, g, x);
+}
+
+// Inverse pass; with Ls set it is the last one and writes limb sums.
+template <class X, int LOGR>
+GL_HD void body_inv(u64 *tile, const TileGeom &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
+                    size_t n_limbs) {
+    if (g.log_s > 0) tile_load_tw<X, LOGR>(tile, g, x, tw);
+    else tile_load<X, LOGR>(tile, g, x, nullptr, 0);
+    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, tw);
+    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    else tile_store<X, LOGR>(tile, g, x);
+}
+
+// Last forward pass of B (from xb, or packed from limbs), pointwise with A
+// (a_other at the same positions; null squares), 1/N, first inverse pass;
+// result to xa, or limb sums when this is the only pass.  The last pass has
+// column stride 1, so it carries no pass twiddle.
+template <class X, int LOGR>
+GL_HD void body_fused(u64 *tile, const TileGeom &g, const u64 *xb, const u64 *limbs, size_t n_limbs_in, u64 *xa,
+                      const u64 *a_other, int log_n, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
+                      size_t n_limbs) {
+    tile_load<X, LOGR>(tile, g, xb, limbs, n_limbs_in);
+    tile_fwd<X, LOGR, 0>(tile, g, tw);
+    tile_pointwise<X, LOGR>(tile, g, a_other, log_n);
+    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, tw);
+    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    else tile_store<X, LOGR>(tile, g, xa);
 }
 
 // Carry-lookahead pairs: bit 0 = generate, bit 1 = propagate.  `lo` covers

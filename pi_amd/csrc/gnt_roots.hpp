@@ -28,7 +28,8 @@ inline gl::u64 library_root() {
 }
 
 // lo[i] = r^i, hi[i] = r^(i * 2^16), i < 2^16: r^e = lo[e & 0xFFFF] * hi[e >> 16].
-inline bool build_root_tables(std::vector<gl::u64> &lo, std::vector<gl::u64> &hi) {
+// sm[2^LM + e] = omega_{2^LM}^e = r^(e * 2^(32-LM)) for LM <= TILE_LOG, e < 2^LM.
+inline bool build_root_tables(std::vector<gl::u64> &lo, std::vector<gl::u64> &hi, std::vector<gl::u64> &sm) {
     const gl::u64 r = library_root();
     if (r == 0 || gl::pow(r, (gl::u64)1 << 26) != 8) return false;
     const size_t T = (size_t)1 << TW_SPLIT;
@@ -39,7 +40,14 @@ inline bool build_root_tables(std::vector<gl::u64> &lo, std::vector<gl::u64> &hi
     const gl::u64 step = w;
     w = 1;
     for (size_t i = 0; i < T; i++) { hi[i] = w; w = gl::mul(w, step); }
-    return w == 1;  // r^(2^32) == 1
+    if (w != 1) return false;  // r^(2^32) == 1
+    sm.assign((size_t)2 << TILE_LOG, 0);
+    for (int lm = 0; lm <= TILE_LOG; lm++) {
+        const gl::u64 om = gl::pow(r, (gl::u64)1 << (32 - lm));
+        gl::u64 x = 1;
+        for (size_t e = 0; e < ((size_t)1 << lm); e++) { sm[((size_t)1 << lm) + e] = x; x = gl::mul(x, om); }
+    }
+    return true;
 }
 
 }  // namespace gnt

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/gnt.h"
+#include "gnt_pass.hpp"
 #include "gnt_plan.hpp"
 
 namespace gnt {
@@ -31,7 +32,8 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;  // serialises host entry points on this context
-    uint64_t *wlo = nullptr, *whi = nullptr;
+    uint64_t *wlo = nullptr, *whi = nullptr, *wsm = nullptr;
+    Tw tw() const { return Tw{wlo, whi, wsm}; }
     Pool pool;
     size_t hbm_cap = 0;
     bool timing = false;

@@ -15,11 +15,13 @@ using namespace gnt;
 
 namespace {
 
-std::vector<u64> g_lo, g_hi;
+std::vector<u64> g_lo, g_hi, g_sm;
 
 void tables() {
-    if (g_lo.empty()) build_root_tables(g_lo, g_hi);
+    if (g_lo.empty()) build_root_tables(g_lo, g_hi, g_sm);
 }
+
+Tw tw() { return Tw{g_lo.data(), g_hi.data(), g_sm.data()}; }
 
 using HX = HostExec;
 
@@ -28,25 +30,15 @@ template <int LOGR>
 void k_fwd_pass(const PassDesc &d, int log_n, u64 *x, const u64 *limbs, size_t n_limbs) {
     std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
     const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
-    for (size_t t = 0; t < grid; t++) {
-        const TileGeom g = tile_geom<LOGR>(d, t);
-        tile_load<HX, LOGR>(tile.data(), g, x, limbs, n_limbs);
-        tile_fwd<HX, LOGR, 0>(tile.data(), g, g_lo.data(), g_hi.data());
-        tile_store<HX, LOGR>(tile.data(), g, x);
-    }
+    for (size_t t = 0; t < grid; t++) body_fwd<HX, LOGR>(tile.data(), tile_geom<LOGR>(d, t), x, limbs, n_limbs, tw());
 }
 
 template <int LOGR>
 void k_inv_pass(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
     std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
     const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
-    for (size_t t = 0; t < grid; t++) {
-        const TileGeom g = tile_geom<LOGR>(d, t);
-        tile_load<HX, LOGR>(tile.data(), g, x, nullptr, 0);
-        tile_inv<HX, LOGR, Sub<LOGR>::count - 1>(tile.data(), g, g_lo.data(), g_hi.data());
-        if (Ls) tile_limb_sums<HX, LOGR>(tile.data(), g, n_coeffs, n_limbs, Ls, Hs);
-        else tile_store<HX, LOGR>(tile.data(), g, x);
-    }
+    for (size_t t = 0; t < grid; t++)
+        body_inv<HX, LOGR>(tile.data(), tile_geom<LOGR>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
 }
 
 template <int LOGR>
@@ -54,15 +46,9 @@ void k_last_fused(const PassDesc &d, int log_n, const u64 *xb, const u64 *limbs,
                   const u64 *other, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
     std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
     const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
-    for (size_t t = 0; t < grid; t++) {
-        const TileGeom g = tile_geom<LOGR>(d, t);
-        tile_load<HX, LOGR>(tile.data(), g, xb, limbs, n_in);
-        tile_fwd<HX, LOGR, 0>(tile.data(), g, g_lo.data(), g_hi.data());
-        tile_pointwise<HX, LOGR>(tile.data(), g, other, log_n);
-        tile_inv<HX, LOGR, Sub<LOGR>::count - 1>(tile.data(), g, g_lo.data(), g_hi.data());
-        if (Ls) tile_limb_sums<HX, LOGR>(tile.data(), g, n_coeffs, n_limbs, Ls, Hs);
-        else tile_store<HX, LOGR>(tile.data(), g, xa);
-    }
+    for (size_t t = 0; t < grid; t++)
+        body_fused<HX, LOGR>(tile.data(), tile_geom<LOGR>(d, t), xb, limbs, n_in, xa, other, log_n, tw(), Ls, Hs,
+                             n_coeffs, n_limbs);
 }
 
 #define DISPATCH(LR, CALL)                                                            \
@@ -123,7 +109,7 @@ void emu_inverse(u64 *x, int log_n) {
 u64 emu_root(int log_n) {
     tables();
     u64 e = (u64)1 << (ROOT_LOG - log_n);
-    return root_pow(g_lo.data(), g_hi.data(), (u32)e);
+    return root_pow(tw(), (u32)e);
 }
 
 void emu_plan(int log_n, int *npass, int *log_r, int *log_c) {
