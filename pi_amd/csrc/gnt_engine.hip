@@ -10,8 +10,9 @@ namespace gnt {
 
 namespace {
 
-constexpr int BLOCK = 256;
-using X = DevExec<BLOCK>;
+constexpr int BLOCK = 256;    // carry kernels
+constexpr int TBLOCK = 1024;  // transform kernels: 64 KiB tile, 8 waves per SIMD at two tiles per CU
+using X = DevExec<TBLOCK>;
 constexpr int CARRY_LPT = 4;                  // limbs per thread
 constexpr int CARRY_TILE = BLOCK * CARRY_LPT;  // limbs per carry tile
 constexpr int SCAN_BLOCK = 1024;
@@ -37,7 +38,7 @@ __device__ __forceinline__ int find_item(const T *__restrict__ items, int n, uin
 }
 
 template <int LOGR, int KIND>
-__global__ __launch_bounds__(BLOCK) void k_tpass(const TItem *__restrict__ items, int n_items, uint64_t total, Tw tw) {
+__global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tpass(const TItem *__restrict__ items, int n_items, uint64_t total, Tw tw) {
     extern __shared__ uint64_t tile[];
     for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
         const TItem it = items[find_item(items, n_items, t)];
@@ -321,7 +322,7 @@ static int launch_tgroup(const Group &g, const TItem *dev, const Tw &tw, hipStre
     dim3 grid((unsigned)std::min<uint64_t>(g.tiles, MAX_GRID));
     const TItem *items = dev + g.first;
     const int n = (int)g.count;
-#define TP_CALL(R) hipLaunchKernelGGL((k_tpass<R, KIND>), grid, dim3(BLOCK), lds, st, items, n, g.tiles, tw)
+#define TP_CALL(R) hipLaunchKernelGGL((k_tpass<R, KIND>), grid, dim3(TBLOCK), lds, st, items, n, g.tiles, tw)
     GNT_DISPATCH_R(g.log_r, TP_CALL)
 #undef TP_CALL
     return 0;

@@ -17,11 +17,20 @@
 
 namespace gnt {
 
-constexpr int BLOCK = 256;
-using X = DevExec<BLOCK>;
+constexpr int BLOCK = 256;        // carry kernels
+#ifndef GNT_TBLOCK
+#define GNT_TBLOCK 1024
+#endif
+// Transform kernels: one 64 KiB tile per workgroup of TBLOCK threads; two
+// tiles per CU give 8 waves per SIMD to hide LDS and HBM latency, which
+// needs <= 64 VGPRs per thread.
+constexpr int TBLOCK = GNT_TBLOCK;
+constexpr int TWAVES = TBLOCK / 64 * 2 / 4;  // waves per SIMD at two tiles per CU
+using X = DevExec<TBLOCK>;
+#define GNT_TBOUNDS __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(TWAVES, TWAVES)))
 
 template <int LOGR>
-__global__ __launch_bounds__(BLOCK) void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs,
+__global__ GNT_TBOUNDS void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs,
                                                     size_t n_limbs, PassDesc d, Tw tw) {
     extern __shared__ u64 tile[];
     body_fwd<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), x, limbs, n_limbs, tw);
@@ -29,7 +38,7 @@ __global__ __launch_bounds__(BLOCK) void k_fwd_pass(u64 *__restrict__ x, const u
 
 // Inverse pass; with Ls set it is the last one and writes limb sums.
 template <int LOGR>
-__global__ __launch_bounds__(BLOCK) void k_inv_pass(u64 *__restrict__ x, PassDesc d, Tw tw, u64 *__restrict__ Ls,
+__global__ GNT_TBOUNDS void k_inv_pass(u64 *__restrict__ x, PassDesc d, Tw tw, u64 *__restrict__ Ls,
                                                     u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
     extern __shared__ u64 tile[];
     body_inv<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), x, tw, Ls, Hs, n_coeffs, n_limbs);
@@ -39,7 +48,7 @@ __global__ __launch_bounds__(BLOCK) void k_inv_pass(u64 *__restrict__ x, PassDes
 // A (a_other; null squares), 1/N, first inverse pass; result to xa, or as
 // limb sums when this is the only pass.
 template <int LOGR>
-__global__ __launch_bounds__(BLOCK) void k_last_fused(const u64 *__restrict__ xb, const u64 *__restrict__ limbs,
+__global__ GNT_TBOUNDS void k_last_fused(const u64 *__restrict__ xb, const u64 *__restrict__ limbs,
                                                       size_t n_limbs_in, u64 *xa, const u64 *a_other, PassDesc d,
                                                       int log_n, Tw tw, u64 *__restrict__ Ls,
                                                       u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
@@ -208,7 +217,7 @@ int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t
         const PassDesc d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
 #define FWD_CALL(R) \
-    hipLaunchKernelGGL(k_fwd_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, src, n_limbs, d, tw)
+    hipLaunchKernelGGL(k_fwd_pass<R>, pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, src, n_limbs, d, tw)
         GNT_DISPATCH_R(d.log_r, FWD_CALL)
 #undef FWD_CALL
     }
@@ -224,7 +233,7 @@ int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t 
     u64 *ls = only ? Ls : nullptr;
     u64 *hs = only ? Hs : nullptr;
 #define FUSED_CALL(R)                                                                                            \
-    hipLaunchKernelGGL(k_last_fused<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, xb, src, nb, xa, \
+    hipLaunchKernelGGL(k_last_fused<R>, pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, xb, src, nb, xa, \
                        other, d, pl.log_n, tw, ls, hs, n_coeffs, n_limbs)
     GNT_DISPATCH_R(d.log_r, FUSED_CALL)
 #undef FUSED_CALL
@@ -238,7 +247,7 @@ int launch_inverse_rest(const Plan &pl, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, 
         u64 *ls = p == 0 ? Ls : nullptr;
         u64 *hs = p == 0 ? Hs : nullptr;
 #define INV_CALL(R)                                                                                            \
-    hipLaunchKernelGGL(k_inv_pass<R>, pass_grid(d, pl.log_n), dim3(BLOCK), pass_lds(d), st, x, d, tw, ls, hs, \
+    hipLaunchKernelGGL(k_inv_pass<R>, pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, d, tw, ls, hs, \
                        n_coeffs, n_limbs)
         GNT_DISPATCH_R(d.log_r, INV_CALL)
 #undef INV_CALL

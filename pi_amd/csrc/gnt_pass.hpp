@@ -47,9 +47,11 @@ constexpr int WSM_SIZE = 2 << TILE_LOG;
 // omega_{2^32}^e from the two-level table.
 GL_HD u64 root_pow(const Tw &tw, u32 e) { return gl::mul(tw.wlo[e & ((1u << TW_SPLIT) - 1)], tw.whi[e >> TW_SPLIT]); }
 
+// Sub-steps of an R-point column DFT: radix <= 8 each (8 values in registers
+// keeps a thread within 64 VGPRs), as even as possible, larger ones first.
 template <int LOGR>
 struct Sub {
-    static constexpr int count = (LOGR + 3) / 4;
+    static constexpr int count = (LOGR + 2) / 3;
     static constexpr int log(int i) { return LOGR / count + (i < LOGR % count ? 1 : 0); }
     // block size (log) seen by sub-step i inside the R-point column DFT
     static constexpr int log_m(int i) { return i == 0 ? LOGR : log_m(i - 1) - log(i - 1); }

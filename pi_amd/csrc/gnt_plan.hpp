@@ -10,7 +10,7 @@
 // the inverse passes run in reverse order and return natural order.
 //
 // A pass keeps a tile of R_p x C_p elements (C_p adjacent columns) in LDS;
-// the R_p-point DFT is done as register sub-steps of radix 2..16.
+// the R_p-point DFT is done as register sub-steps of radix 2..8.
 #pragma once
 
 #include <stdint.h>
@@ -37,8 +37,8 @@ struct Plan {
 };
 
 // Sub-step radices of an R-point column DFT: split log_r into parts of at
-// most 4 bits, as evenly as possible, larger parts first.
-static inline int substep_count(int log_r) { return (log_r + 3) / 4; }
+// most 3 bits, as evenly as possible, larger parts first.
+static inline int substep_count(int log_r) { return (log_r + 2) / 3; }
 static inline int substep_log(int log_r, int i) {
     int k = substep_count(log_r);
     int base = log_r / k, extra = log_r % k;
