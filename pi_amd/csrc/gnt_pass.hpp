@@ -80,13 +80,15 @@ GL_HD int tix(const TileGeom &g, int n, int c) {
     return (n << g.log_c) + (c ^ (n & ((1 << g.log_c) - 1)));
 }
 
-// Global address of tile element (row n, tile column c).
-GL_HD size_t tile_addr(size_t cg0, int n, int c, int log_m, int log_s) {
-    size_t cg = cg0 + (size_t)c;
-    size_t b = cg >> log_s;
-    size_t j = cg & (((size_t)1 << log_s) - 1);
-    return (b << log_m) + ((size_t)n << log_s) + j;
+// Global index of tile element (row n, tile column c).  Transforms hold at
+// most 2^MAX_LOG_N elements, so indices fit in 32 bits.
+GL_HD u32 tile_addr(size_t cg0, int n, int c, int log_m, int log_s) {
+    u32 cg = (u32)cg0 + (u32)c;
+    u32 b = cg >> log_s;
+    u32 j = cg & ((1u << log_s) - 1);
+    return (b << log_m) + ((u32)n << log_s) + j;
 }
+static_assert(MAX_LOG_N <= 31, "32-bit element indices");
 
 // Tile element index -> (row, column): columns fastest when columns are
 // adjacent in memory (log_s > 0), rows fastest in the contiguous last pass.
@@ -169,7 +171,7 @@ GL_HD void tile_load(u64 *tile, const TileGeom &g, const u64 *__restrict__ x, co
     GNT_FOR(X, idx, g.tile_n) {
         int n, c;
         tile_rc(idx, LOGR, g.log_c, g.log_s, n, c);
-        size_t a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
+        u32 a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
         tile[tix(g, n, c)] = limbs ? coeff_from_limbs(limbs, n_limbs, a) : x[a];
     }
     X::sync();
@@ -184,44 +186,33 @@ GL_HD void tile_store(const u64 *tile, const TileGeom &g, u64 *__restrict__ x) {
     }
 }
 
-// Pass twiddle omega_M^(j*kc), kc = bitrev_R(p), j = j0 + c the element's
-// column (the tile's columns are consecutive, j0 = cg0 mod s).  Each task owns
-// a run of up to 16 consecutive columns of one row and steps the twiddle by
-// omega_M^kc along it, so a run costs two table lookups.  FWD: tile -> x with
-// the twiddle applied; !FWD: x -> tile with the inverse twiddle.
-template <int LOGR, bool FWD>
-GL_HD void twiddle_run(u64 *tile, const TileGeom &g, u64 *__restrict__ x, const Tw &tw, int L) {
+// Pass twiddle omega_M^(j*kc) on the tile in LDS, kc = bitrev_R(p) for row
+// p, j = j0 + c for column c (a tile's columns are consecutive, j0 = cg0 mod
+// s).  Each task owns a run of up to 16 consecutive columns of one row and
+// steps the twiddle by omega_M^kc along it, so a run costs two table lookups.
+// INV applies the inverse twiddle.  Global loads and stores stay coalesced.
+template <class X, int LOGR, bool INV>
+GL_HD void tile_twiddle(u64 *tile, const TileGeom &g, const Tw &tw) {
     const int lc = g.log_c;
     const int lch = lc < 4 ? lc : 4;
     const int CH = 1 << lch;
-    const int p = L >> (lc - lch);
-    const int c0 = (L & ((1 << (lc - lch)) - 1)) << lch;
-    const u32 kc = gl::bitrev((u32)p, LOGR);
-    const size_t j = (g.cg0 & (((size_t)1 << g.log_s) - 1)) + (size_t)c0;
+    const size_t j0 = g.cg0 & (((size_t)1 << g.log_s) - 1);
     const int sh = ROOT_LOG - g.log_m;
-    u32 ew = (u32)(((u64)j * kc) << sh);
-    u32 eu = (u32)((u64)kc << sh);
-    if (!FWD) { ew = 0u - ew; eu = 0u - eu; }
-    u64 w = root_pow(tw, ew);
-    const u64 u = root_pow(tw, eu);
-    const size_t a = tile_addr(g.cg0, p, c0, g.log_m, g.log_s);
-    for (int c = 0; c < CH; c++) {
-        if (FWD) x[a + c] = gl::mul(tile[tix(g, p, c0 + c)], w);
-        else tile[tix(g, p, c0 + c)] = gl::mul(x[a + c], w);
-        w = gl::mul(w, u);
+    GNT_FOR(X, L, g.tile_n >> lch) {
+        const int p = L >> (lc - lch);
+        const int c0 = (L & ((1 << (lc - lch)) - 1)) << lch;
+        const u32 kc = gl::bitrev((u32)p, LOGR);
+        u32 ew = (u32)(((u64)(j0 + (size_t)c0) * kc) << sh);
+        u32 eu = (u32)((u64)kc << sh);
+        if (INV) { ew = 0u - ew; eu = 0u - eu; }
+        u64 w = root_pow(tw, ew);
+        const u64 u = root_pow(tw, eu);
+        for (int c = 0; c < CH; c++) {
+            const int t = tix(g, p, c0 + c);
+            tile[t] = gl::mul(tile[t], w);
+            w = gl::mul(w, u);
+        }
     }
-}
-
-template <class X, int LOGR>
-GL_HD void tile_store_tw(u64 *tile, const TileGeom &g, u64 *__restrict__ x, const Tw &tw) {
-    const int lch = g.log_c < 4 ? g.log_c : 4;
-    GNT_FOR(X, L, g.tile_n >> lch) twiddle_run<LOGR, true>(tile, g, x, tw, L);
-}
-
-template <class X, int LOGR>
-GL_HD void tile_load_tw(u64 *tile, const TileGeom &g, const u64 *__restrict__ x, const Tw &tw) {
-    const int lch = g.log_c < 4 ? g.log_c : 4;
-    GNT_FOR(X, L, g.tile_n >> lch) twiddle_run<LOGR, false>(tile, g, const_cast<u64 *>(x), tw, L);
     X::sync();
 }
 
@@ -275,8 +266,8 @@ GL_HD void tile_limb_sums(const u64 *tile, const TileGeom &g, size_t n_coeffs, s
         // four consecutive coefficients: along a row (log_s > 0) or down a column
         int n, c;
         tile_rc(q << 2, LOGR, g.log_c, g.log_s, n, c);
-        size_t a0 = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
-        size_t j = a0 >> 2;
+        u32 a0 = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
+        u32 j = a0 >> 2;
         if (j < n_limbs) {
             u64 cc[4];
 #pragma unroll
@@ -301,16 +292,16 @@ template <class X, int LOGR>
 GL_HD void body_fwd(u64 *tile, const TileGeom &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw) {
     tile_load<X, LOGR>(tile, g, x, limbs, n_limbs);
     tile_fwd<X, LOGR, 0>(tile, g, tw);
-    if (g.log_s > 0) tile_store_tw<X, LOGR>(tile, g, x, tw);
-    else tile_store<X, LOGR>(tile, g, x);
+    if (g.log_s > 0) tile_twiddle<X, LOGR, false>(tile, g, tw);
+    tile_store<X, LOGR>(tile, g, x);
 }
 
 // Inverse pass; with Ls set it is the last one and writes limb sums.
 template <class X, int LOGR>
 GL_HD void body_inv(u64 *tile, const TileGeom &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
                     size_t n_limbs) {
-    if (g.log_s > 0) tile_load_tw<X, LOGR>(tile, g, x, tw);
-    else tile_load<X, LOGR>(tile, g, x, nullptr, 0);
+    tile_load<X, LOGR>(tile, g, x, nullptr, 0);
+    if (g.log_s > 0) tile_twiddle<X, LOGR, true>(tile, g, tw);
     tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, tw);
     if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
     else tile_store<X, LOGR>(tile, g, x);
