@@ -37,16 +37,17 @@ __device__ __forceinline__ int find_item(const T *__restrict__ items, int n, uin
     return lo;
 }
 
-template <int LOGR, int KIND>
-__global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tpass(const TItem *__restrict__ items, int n_items, uint64_t total, Tw tw) {
+template <class G, int KIND>
+__global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tpass(
+    const TItem *__restrict__ items, int n_items, uint64_t total, Tw tw) {
     extern __shared__ uint64_t tile[];
     for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
         const TItem it = items[find_item(items, n_items, t)];
-        const TileGeom g = tile_geom<LOGR>(it.d, t - it.tile0);
-        if (KIND == K_FWD) body_fwd<X, LOGR>(tile, g, it.x, it.limbs, it.n_limbs_in, tw);
-        else if (KIND == K_INV) body_inv<X, LOGR>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs);
-        else body_fused<X, LOGR>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls,
-                                 it.Hs, it.n_coeffs, it.n_limbs);
+        const G g = make_geom<G>(it.d, t - it.tile0);
+        if (KIND == K_FWD) body_fwd<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, tw);
+        else if (KIND == K_INV) body_inv<X, G>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs);
+        else body_fused<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls,
+                              it.Hs, it.n_coeffs, it.n_limbs);
         __syncthreads();  // the tile buffer is reused by the next iteration
     }
 }
@@ -271,7 +272,7 @@ void finalize_iteration(Iteration &it) {
     // transform items: group by (kind, radix)
     std::stable_sort(it.titems.begin(), it.titems.end(), [](const TItem &a, const TItem &b) {
         if (a.pad != b.pad) return a.pad < b.pad;
-        return a.d.log_r < b.d.log_r;
+        return shape_key(a.d) < shape_key(b.d);
     });
     it.groups.clear();
     for (size_t i = 0; i < it.titems.size();) {
@@ -279,14 +280,14 @@ void finalize_iteration(Iteration &it) {
         uint64_t tiles = 0;
         int lds_log = 0;
         while (j < it.titems.size() && it.titems[j].pad == it.titems[i].pad &&
-               it.titems[j].d.log_r == it.titems[i].d.log_r) {
+               shape_key(it.titems[j].d) == shape_key(it.titems[i].d)) {
             const TItem &t = it.titems[j];
             it.titems[j].tile0 = tiles;
             tiles += ((uint64_t)1 << t.log_n) >> (t.d.log_r + t.d.log_c);
             lds_log = std::max(lds_log, t.d.log_r + t.d.log_c);
             j++;
         }
-        it.groups.push_back(Group{it.titems[i].pad, it.titems[i].d.log_r, i, j - i, tiles, lds_log});
+        it.groups.push_back(Group{it.titems[i].pad, shape_key(it.titems[i].d), i, j - i, tiles, lds_log});
         i = j;
     }
     // carry items: group by step
@@ -306,25 +307,22 @@ void finalize_iteration(Iteration &it) {
     }
 }
 
-#define GNT_DISPATCH_R(LR, CALL)                                                      \
-    switch (LR) {                                                                     \
-        case 1: CALL(1); break;  case 2: CALL(2); break;  case 3: CALL(3); break;     \
-        case 4: CALL(4); break;  case 5: CALL(5); break;  case 6: CALL(6); break;     \
-        case 7: CALL(7); break;  case 8: CALL(8); break;  case 9: CALL(9); break;     \
-        case 10: CALL(10); break; case 11: CALL(11); break; case 12: CALL(12); break; \
-        case 13: CALL(13); break;                                                     \
-        default: return -1;                                                           \
-    }
-
 template <int KIND>
 static int launch_tgroup(const Group &g, const TItem *dev, const Tw &tw, hipStream_t st) {
     const size_t lds = ((size_t)1 << g.lds_log) * 8;
     dim3 grid((unsigned)std::min<uint64_t>(g.tiles, MAX_GRID));
     const TItem *items = dev + g.first;
     const int n = (int)g.count;
-#define TP_CALL(R) hipLaunchKernelGGL((k_tpass<R, KIND>), grid, dim3(TBLOCK), lds, st, items, n, g.tiles, tw)
-    GNT_DISPATCH_R(g.log_r, TP_CALL)
-#undef TP_CALL
+    switch (g.shape) {
+#define TP_CASE(LR, LC, S)                                                                                  \
+    case shape_key(LR, LC, S):                                                                              \
+        hipLaunchKernelGGL((k_tpass<Geom<LR, LC, S>, KIND>), grid, dim3(TBLOCK), lds, st, items, n, g.tiles, \
+                           tw);                                                                             \
+        break;
+        GNT_FOR_EACH_SHAPE(TP_CASE)
+#undef TP_CASE
+        default: return -1;
+    }
     return 0;
 }
 

@@ -75,7 +75,7 @@ bool prod_init(Prod &p);
 // Launch groups of one iteration.
 struct Group {
     int kind;   // K_FWD/K_INV/K_FUSED, or 10+k for carry step k
-    int log_r;
+    int shape;  // shape_key of the group's tiles
     size_t first, count;  // range in the iteration's item table
     uint64_t tiles;       // total tiles of the launch
     int lds_log;          // log2 of the largest tile (u64 elements) in the group

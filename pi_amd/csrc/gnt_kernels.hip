@@ -29,32 +29,32 @@ constexpr int TWAVES = TBLOCK / 64 * 2 / 4;  // waves per SIMD at two tiles per 
 using X = DevExec<TBLOCK>;
 #define GNT_TBOUNDS __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(TWAVES, TWAVES)))
 
-template <int LOGR>
-__global__ GNT_TBOUNDS void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs,
-                                                    size_t n_limbs, PassDesc d, Tw tw) {
+template <class G>
+__global__ GNT_TBOUNDS void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs, size_t n_limbs,
+                                       PassDesc d, Tw tw) {
     extern __shared__ u64 tile[];
-    body_fwd<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), x, limbs, n_limbs, tw);
+    body_fwd<X, G>(tile, make_geom<G>(d, blockIdx.x), x, limbs, n_limbs, tw);
 }
 
 // Inverse pass; with Ls set it is the last one and writes limb sums.
-template <int LOGR>
+template <class G>
 __global__ GNT_TBOUNDS void k_inv_pass(u64 *__restrict__ x, PassDesc d, Tw tw, u64 *__restrict__ Ls,
-                                                    u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
+                                       u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
     extern __shared__ u64 tile[];
-    body_inv<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), x, tw, Ls, Hs, n_coeffs, n_limbs);
+    body_inv<X, G>(tile, make_geom<G>(d, blockIdx.x), x, tw, Ls, Hs, n_coeffs, n_limbs);
 }
 
 // Last forward pass of B (read from xb, or packed from limbs), pointwise with
 // A (a_other; null squares), 1/N, first inverse pass; result to xa, or as
 // limb sums when this is the only pass.
-template <int LOGR>
+template <class G>
 __global__ GNT_TBOUNDS void k_last_fused(const u64 *__restrict__ xb, const u64 *__restrict__ limbs,
-                                                      size_t n_limbs_in, u64 *xa, const u64 *a_other, PassDesc d,
-                                                      int log_n, Tw tw, u64 *__restrict__ Ls,
-                                                      u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
+                                         size_t n_limbs_in, u64 *xa, const u64 *a_other, PassDesc d, int log_n,
+                                         Tw tw, u64 *__restrict__ Ls, u64 *__restrict__ Hs, size_t n_coeffs,
+                                         size_t n_limbs) {
     extern __shared__ u64 tile[];
-    body_fused<X, LOGR>(tile, tile_geom<LOGR>(d, blockIdx.x), xb, limbs, n_limbs_in, xa, a_other, log_n, tw, Ls,
-                        Hs, n_coeffs, n_limbs);
+    body_fused<X, G>(tile, make_geom<G>(d, blockIdx.x), xb, limbs, n_limbs_in, xa, a_other, log_n, tw, Ls, Hs,
+                     n_coeffs, n_limbs);
 }
 
 // ---------------------------------------------------------------------------
@@ -201,25 +201,21 @@ static inline dim3 pass_grid(const PassDesc &d, int log_n) {
 }
 static inline size_t pass_lds(const PassDesc &d) { return ((size_t)1 << (d.log_r + d.log_c)) * sizeof(u64); }
 
-#define GNT_DISPATCH_R(LR, CALL)                                                      \
-    switch (LR) {                                                                     \
-        case 1: CALL(1); break;  case 2: CALL(2); break;  case 3: CALL(3); break;     \
-        case 4: CALL(4); break;  case 5: CALL(5); break;  case 6: CALL(6); break;     \
-        case 7: CALL(7); break;  case 8: CALL(8); break;  case 9: CALL(9); break;     \
-        case 10: CALL(10); break; case 11: CALL(11); break; case 12: CALL(12); break; \
-        case 13: CALL(13); break;                                                     \
-        default: return -1;                                                           \
-    }
-
 int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw,
                    hipStream_t st) {
     for (int p = 0; p < npasses; p++) {
         const PassDesc d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
-#define FWD_CALL(R) \
-    hipLaunchKernelGGL(k_fwd_pass<R>, pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, src, n_limbs, d, tw)
-        GNT_DISPATCH_R(d.log_r, FWD_CALL)
-#undef FWD_CALL
+        switch (shape_key(d)) {
+#define FWD_CASE(LR, LC, S)                                                                                     \
+    case shape_key(LR, LC, S):                                                                                  \
+        hipLaunchKernelGGL((k_fwd_pass<Geom<LR, LC, S>>), pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, \
+                           src, n_limbs, d, tw);                                                                \
+        break;
+            GNT_FOR_EACH_SHAPE(FWD_CASE)
+#undef FWD_CASE
+            default: return -1;
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -232,11 +228,16 @@ int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t 
     const u64 *other = square ? nullptr : xa;
     u64 *ls = only ? Ls : nullptr;
     u64 *hs = only ? Hs : nullptr;
-#define FUSED_CALL(R)                                                                                            \
-    hipLaunchKernelGGL(k_last_fused<R>, pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, xb, src, nb, xa, \
-                       other, d, pl.log_n, tw, ls, hs, n_coeffs, n_limbs)
-    GNT_DISPATCH_R(d.log_r, FUSED_CALL)
-#undef FUSED_CALL
+    switch (shape_key(d)) {
+#define FUSED_CASE(LR, LC, S)                                                                                   \
+    case shape_key(LR, LC, S):                                                                                  \
+        hipLaunchKernelGGL((k_last_fused<Geom<LR, LC, S>>), pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, \
+                           xb, src, nb, xa, other, d, pl.log_n, tw, ls, hs, n_coeffs, n_limbs);                 \
+        break;
+        GNT_FOR_EACH_SHAPE(FUSED_CASE)
+#undef FUSED_CASE
+        default: return -1;
+    }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -246,11 +247,16 @@ int launch_inverse_rest(const Plan &pl, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, 
         const PassDesc d = pl.pass[p];
         u64 *ls = p == 0 ? Ls : nullptr;
         u64 *hs = p == 0 ? Hs : nullptr;
-#define INV_CALL(R)                                                                                            \
-    hipLaunchKernelGGL(k_inv_pass<R>, pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, d, tw, ls, hs, \
-                       n_coeffs, n_limbs)
-        GNT_DISPATCH_R(d.log_r, INV_CALL)
-#undef INV_CALL
+        switch (shape_key(d)) {
+#define INV_CASE(LR, LC, S)                                                                                     \
+    case shape_key(LR, LC, S):                                                                                  \
+        hipLaunchKernelGGL((k_inv_pass<Geom<LR, LC, S>>), pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, \
+                           d, tw, ls, hs, n_coeffs, n_limbs);                                                   \
+        break;
+            GNT_FOR_EACH_SHAPE(INV_CASE)
+#undef INV_CASE
+            default: return -1;
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

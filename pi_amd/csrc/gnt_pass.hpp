@@ -57,27 +57,34 @@ struct Sub {
     static constexpr int log_m(int i) { return i == 0 ? LOGR : log_m(i - 1) - log(i - 1); }
 };
 
-struct TileGeom {
-    int log_m, log_c, log_s, tile_n;
-    size_t cg0;
+// Tile shape, fixed at compile time: R = 2^LOGR rows, 2^LOGC columns;
+// STRIDED: columns are adjacent in memory (a pass with column stride > 1);
+// otherwise the pass is the contiguous last one (rows adjacent).
+template <int LOGR_, int LOGC_, bool STRIDED_>
+struct Geom {
+    static constexpr int LOGR = LOGR_;
+    static constexpr int log_c = LOGC_;
+    static constexpr bool strided = STRIDED_;
+    static constexpr int tile_n = 1 << (LOGR_ + LOGC_);
+    int log_m, log_s;  // pass block size, column stride (logs)
+    size_t cg0;        // first column of the tile
 };
 
-template <int LOGR>
-GL_HD TileGeom tile_geom(const PassDesc &d, size_t tile_index) {
-    TileGeom g;
+template <class G>
+GL_HD G make_geom(const PassDesc &d, size_t tile_index) {
+    G g;
     g.log_m = d.log_m;
-    g.log_c = d.log_c;
-    g.log_s = d.log_m - LOGR;
-    g.tile_n = 1 << (LOGR + d.log_c);
-    g.cg0 = tile_index << d.log_c;
+    g.log_s = d.log_m - G::LOGR;
+    g.cg0 = tile_index << G::log_c;
     return g;
 }
 
 // LDS position of tile element (row n, column c).  Columns are XOR-swizzled
 // by the row so that walking down a column (the contiguous last pass) and
 // walking along a row both hit distinct banks.
-GL_HD int tix(const TileGeom &g, int n, int c) {
-    return (n << g.log_c) + (c ^ (n & ((1 << g.log_c) - 1)));
+template <class G>
+GL_HD int tix(const G &, int n, int c) {
+    return (n << G::log_c) + (c ^ (n & ((1 << G::log_c) - 1)));
 }
 
 // Global index of tile element (row n, tile column c).  Transforms hold at
@@ -91,10 +98,11 @@ GL_HD u32 tile_addr(size_t cg0, int n, int c, int log_m, int log_s) {
 static_assert(MAX_LOG_N <= 31, "32-bit element indices");
 
 // Tile element index -> (row, column): columns fastest when columns are
-// adjacent in memory (log_s > 0), rows fastest in the contiguous last pass.
-GL_HD void tile_rc(int idx, int log_r, int log_c, int log_s, int &n, int &c) {
-    if (log_s > 0) { c = idx & ((1 << log_c) - 1); n = idx >> log_c; }
-    else { n = idx & ((1 << log_r) - 1); c = idx >> log_r; }
+// adjacent in memory (strided passes), rows fastest in the contiguous last pass.
+template <class G>
+GL_HD void tile_rc(int idx, int &n, int &c) {
+    if (G::strided) { c = idx & ((1 << G::log_c) - 1); n = idx >> G::log_c; }
+    else { n = idx & ((1 << G::LOGR) - 1); c = idx >> G::LOGR; }
 }
 
 // Coefficient i of an operand packed 16 bits per coefficient (ntt.rs:640-665).
@@ -107,15 +115,16 @@ GL_HD u64 coeff_from_limbs(const u64 *__restrict__ limbs, size_t n_limbs, size_t
 // Sub-step I of the forward R-point column DFT for task L (DIF): an r-point
 // DFT over rows base + u*t of column c, then the twiddle omega_{2^LM}^(jj*k1)
 // (none after the last sub-step; the pass twiddle is applied on the way out).
-template <int LOGR, int I>
-GL_HD void fwd_task(u64 *tile, const TileGeom &g, int L, const Tw &tw) {
+template <class G, int I>
+GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
+    constexpr int LOGR = G::LOGR;
     constexpr int A = Sub<LOGR>::log(I);
     constexpr int R = 1 << A;
     constexpr int LM = Sub<LOGR>::log_m(I);
     constexpr int LT = LM - A;
     constexpr bool LAST = (I == Sub<LOGR>::count - 1);
-    const int c = L & ((1 << g.log_c) - 1);
-    const int rest = L >> g.log_c;
+    const int c = L & ((1 << G::log_c) - 1);
+    const int rest = L >> G::log_c;
     const int jj = rest & ((1 << LT) - 1);
     const int blk = rest >> LT;
     const int base = (blk << LM) + jj;
@@ -134,15 +143,16 @@ GL_HD void fwd_task(u64 *tile, const TileGeom &g, int L, const Tw &tw) {
 
 // Inverse of fwd_task<LOGR, I>: undo the twiddle, then the inverse r-point
 // DFT (DIT, bit-reversed input, natural output).  Unscaled.
-template <int LOGR, int I>
-GL_HD void inv_task(u64 *tile, const TileGeom &g, int L, const Tw &tw) {
+template <class G, int I>
+GL_HD void inv_task(u64 *tile, const G &g, int L, const Tw &tw) {
+    constexpr int LOGR = G::LOGR;
     constexpr int A = Sub<LOGR>::log(I);
     constexpr int R = 1 << A;
     constexpr int LM = Sub<LOGR>::log_m(I);
     constexpr int LT = LM - A;
     constexpr bool LAST = (I == Sub<LOGR>::count - 1);
-    const int c = L & ((1 << g.log_c) - 1);
-    const int rest = L >> g.log_c;
+    const int c = L & ((1 << G::log_c) - 1);
+    const int rest = L >> G::log_c;
     const int jj = rest & ((1 << LT) - 1);
     const int blk = rest >> LT;
     const int base = (blk << LM) + jj;
@@ -165,23 +175,23 @@ GL_HD void inv_task(u64 *tile, const TileGeom &g, int L, const Tw &tw) {
 // ---- tile-level routines ----
 
 // Load a tile from x, or (limbs != nullptr) pack it from the operand's limbs.
-template <class X, int LOGR>
-GL_HD void tile_load(u64 *tile, const TileGeom &g, const u64 *__restrict__ x, const u64 *__restrict__ limbs,
+template <class X, class G>
+GL_HD void tile_load(u64 *tile, const G &g, const u64 *__restrict__ x, const u64 *__restrict__ limbs,
                      size_t n_limbs) {
-    GNT_FOR(X, idx, g.tile_n) {
+    GNT_FOR(X, idx, G::tile_n) {
         int n, c;
-        tile_rc(idx, LOGR, g.log_c, g.log_s, n, c);
+        tile_rc<G>(idx, n, c);
         u32 a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
         tile[tix(g, n, c)] = limbs ? coeff_from_limbs(limbs, n_limbs, a) : x[a];
     }
     X::sync();
 }
 
-template <class X, int LOGR>
-GL_HD void tile_store(const u64 *tile, const TileGeom &g, u64 *__restrict__ x) {
-    GNT_FOR(X, idx, g.tile_n) {
+template <class X, class G>
+GL_HD void tile_store(const u64 *tile, const G &g, u64 *__restrict__ x) {
+    GNT_FOR(X, idx, G::tile_n) {
         int n, c;
-        tile_rc(idx, LOGR, g.log_c, g.log_s, n, c);
+        tile_rc<G>(idx, n, c);
         x[tile_addr(g.cg0, n, c, g.log_m, g.log_s)] = tile[tix(g, n, c)];
     }
 }
@@ -191,14 +201,15 @@ GL_HD void tile_store(const u64 *tile, const TileGeom &g, u64 *__restrict__ x) {
 // s).  Each task owns a run of up to 16 consecutive columns of one row and
 // steps the twiddle by omega_M^kc along it, so a run costs two table lookups.
 // INV applies the inverse twiddle.  Global loads and stores stay coalesced.
-template <class X, int LOGR, bool INV>
-GL_HD void tile_twiddle(u64 *tile, const TileGeom &g, const Tw &tw) {
-    const int lc = g.log_c;
+template <class X, class G, bool INV>
+GL_HD void tile_twiddle(u64 *tile, const G &g, const Tw &tw) {
+    constexpr int LOGR = G::LOGR;
+    const int lc = G::log_c;
     const int lch = lc < 4 ? lc : 4;
     const int CH = 1 << lch;
     const size_t j0 = g.cg0 & (((size_t)1 << g.log_s) - 1);
     const int sh = ROOT_LOG - g.log_m;
-    GNT_FOR(X, L, g.tile_n >> lch) {
+    GNT_FOR(X, L, G::tile_n >> lch) {
         const int p = L >> (lc - lch);
         const int c0 = (L & ((1 << (lc - lch)) - 1)) << lch;
         const u32 kc = gl::bitrev((u32)p, LOGR);
@@ -216,35 +227,35 @@ GL_HD void tile_twiddle(u64 *tile, const TileGeom &g, const Tw &tw) {
     X::sync();
 }
 
-template <class X, int LOGR, int I>
-GL_HD void tile_fwd(u64 *tile, const TileGeom &g, const Tw &tw) {
-    if constexpr (I < Sub<LOGR>::count) {
-        const int ntask = g.tile_n >> Sub<LOGR>::log(I);
-        GNT_FOR(X, L, ntask) fwd_task<LOGR, I>(tile, g, L, tw);
+template <class X, class G, int I>
+GL_HD void tile_fwd(u64 *tile, const G &g, const Tw &tw) {
+    if constexpr (I < Sub<G::LOGR>::count) {
+        const int ntask = G::tile_n >> Sub<G::LOGR>::log(I);
+        GNT_FOR(X, L, ntask) fwd_task<G, I>(tile, g, L, tw);
         X::sync();
-        tile_fwd<X, LOGR, I + 1>(tile, g, tw);
+        tile_fwd<X, G, I + 1>(tile, g, tw);
     }
 }
 
 // Inverse sub-steps I, I-1, ..., 0.
-template <class X, int LOGR, int I>
-GL_HD void tile_inv(u64 *tile, const TileGeom &g, const Tw &tw) {
+template <class X, class G, int I>
+GL_HD void tile_inv(u64 *tile, const G &g, const Tw &tw) {
     if constexpr (I >= 0) {
-        const int ntask = g.tile_n >> Sub<LOGR>::log(I);
-        GNT_FOR(X, L, ntask) inv_task<LOGR, I>(tile, g, L, tw);
+        const int ntask = G::tile_n >> Sub<G::LOGR>::log(I);
+        GNT_FOR(X, L, ntask) inv_task<G, I>(tile, g, L, tw);
         X::sync();
-        tile_inv<X, LOGR, I - 1>(tile, g, tw);
+        tile_inv<X, G, I - 1>(tile, g, tw);
     }
 }
 
 // tile *= other (at the same global positions, or the tile itself) * 2^(192 - log_n):
 // the pointwise product with the 1/N of the inverse (ntt.rs:845-853, 376-391).
-template <class X, int LOGR>
-GL_HD void tile_pointwise(u64 *tile, const TileGeom &g, const u64 *__restrict__ other, int log_n) {
+template <class X, class G>
+GL_HD void tile_pointwise(u64 *tile, const G &g, const u64 *__restrict__ other, int log_n) {
     const int e = 192 - log_n;
-    GNT_FOR(X, idx, g.tile_n) {
+    GNT_FOR(X, idx, G::tile_n) {
         int n, c;
-        tile_rc(idx, LOGR, g.log_c, g.log_s, n, c);
+        tile_rc<G>(idx, n, c);
         const int t = tix(g, n, c);
         u64 v = tile[t];
         u64 w = other ? other[tile_addr(g.cg0, n, c, g.log_m, g.log_s)] : v;
@@ -258,21 +269,21 @@ GL_HD void tile_pointwise(u64 *tile, const TileGeom &g, const u64 *__restrict__ 
 // indices): S_j = sum_t c[4j+t] 2^(16t) = L_j + 2^64 H_j for every limb whose
 // four coefficients lie in this tile, j < n_limbs.  Coefficients at or past
 // n_coeffs are zero by construction of the product; they are masked anyway.
-template <class X, int LOGR>
-GL_HD void tile_limb_sums(const u64 *tile, const TileGeom &g, size_t n_coeffs, size_t n_limbs, u64 *__restrict__ Ls,
+template <class X, class G>
+GL_HD void tile_limb_sums(const u64 *tile, const G &g, size_t n_coeffs, size_t n_limbs, u64 *__restrict__ Ls,
                           u64 *__restrict__ Hs) {
-    const int nq = g.tile_n >> 2;
+    const int nq = G::tile_n >> 2;
     GNT_FOR(X, q, nq) {
         // four consecutive coefficients: along a row (log_s > 0) or down a column
         int n, c;
-        tile_rc(q << 2, LOGR, g.log_c, g.log_s, n, c);
+        tile_rc<G>(q << 2, n, c);
         u32 a0 = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
         u32 j = a0 >> 2;
         if (j < n_limbs) {
             u64 cc[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                int t = g.log_s > 0 ? tix(g, n, c + k) : tix(g, n + k, c);
+                int t = G::strided ? tix(g, n, c + k) : tix(g, n + k, c);
                 cc[k] = (a0 + k < n_coeffs) ? tile[t] : 0;
             }
             u64 lo = cc[0], hi = 0, t;
@@ -288,39 +299,39 @@ GL_HD void tile_limb_sums(const u64 *tile, const TileGeom &g, size_t n_coeffs, s
 // ---- pass bodies: what each kernel runs on one tile ----
 
 // Forward pass.  With `limbs` set the tile is packed from the operand's limbs.
-template <class X, int LOGR>
-GL_HD void body_fwd(u64 *tile, const TileGeom &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw) {
-    tile_load<X, LOGR>(tile, g, x, limbs, n_limbs);
-    tile_fwd<X, LOGR, 0>(tile, g, tw);
-    if (g.log_s > 0) tile_twiddle<X, LOGR, false>(tile, g, tw);
-    tile_store<X, LOGR>(tile, g, x);
+template <class X, class G>
+GL_HD void body_fwd(u64 *tile, const G &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw) {
+    tile_load<X, G>(tile, g, x, limbs, n_limbs);
+    tile_fwd<X, G, 0>(tile, g, tw);
+    if constexpr (G::strided) tile_twiddle<X, G, false>(tile, g, tw);
+    tile_store<X, G>(tile, g, x);
 }
 
 // Inverse pass; with Ls set it is the last one and writes limb sums.
-template <class X, int LOGR>
-GL_HD void body_inv(u64 *tile, const TileGeom &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
+template <class X, class G>
+GL_HD void body_inv(u64 *tile, const G &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
                     size_t n_limbs) {
-    tile_load<X, LOGR>(tile, g, x, nullptr, 0);
-    if (g.log_s > 0) tile_twiddle<X, LOGR, true>(tile, g, tw);
-    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, tw);
-    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
-    else tile_store<X, LOGR>(tile, g, x);
+    tile_load<X, G>(tile, g, x, nullptr, 0);
+    if constexpr (G::strided) tile_twiddle<X, G, true>(tile, g, tw);
+    tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
+    if (Ls) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    else tile_store<X, G>(tile, g, x);
 }
 
 // Last forward pass of B (from xb, or packed from limbs), pointwise with A
 // (a_other at the same positions; null squares), 1/N, first inverse pass;
 // result to xa, or limb sums when this is the only pass.  The last pass has
 // column stride 1, so it carries no pass twiddle.
-template <class X, int LOGR>
-GL_HD void body_fused(u64 *tile, const TileGeom &g, const u64 *xb, const u64 *limbs, size_t n_limbs_in, u64 *xa,
+template <class X, class G>
+GL_HD void body_fused(u64 *tile, const G &g, const u64 *xb, const u64 *limbs, size_t n_limbs_in, u64 *xa,
                       const u64 *a_other, int log_n, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
                       size_t n_limbs) {
-    tile_load<X, LOGR>(tile, g, xb, limbs, n_limbs_in);
-    tile_fwd<X, LOGR, 0>(tile, g, tw);
-    tile_pointwise<X, LOGR>(tile, g, a_other, log_n);
-    tile_inv<X, LOGR, Sub<LOGR>::count - 1>(tile, g, tw);
-    if (Ls) tile_limb_sums<X, LOGR>(tile, g, n_coeffs, n_limbs, Ls, Hs);
-    else tile_store<X, LOGR>(tile, g, xa);
+    tile_load<X, G>(tile, g, xb, limbs, n_limbs_in);
+    tile_fwd<X, G, 0>(tile, g, tw);
+    tile_pointwise<X, G>(tile, g, a_other, log_n);
+    tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
+    if (Ls) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    else tile_store<X, G>(tile, g, xa);
 }
 
 // Carry-lookahead pairs: bit 0 = generate, bit 1 = propagate.  `lo` covers

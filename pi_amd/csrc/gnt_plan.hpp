@@ -72,3 +72,17 @@ static inline Plan make_plan(int log_n) {
 }
 
 }  // namespace gnt
+
+// Every tile shape the planner can produce, as (log_r, log_c, strided):
+// single-tile transforms (log_n <= TILE_LOG) and the 7..10-bit passes of the
+// 2- and 3-pass plans, whose tiles are always 2^TILE_LOG elements.  Kernels are
+// instantiated for exactly these; shape_key() selects one at run time.
+#define GNT_FOR_EACH_SHAPE(M)                                                                      \
+    M(1, 0, 0) M(2, 0, 0) M(3, 0, 0) M(4, 0, 0) M(5, 0, 0) M(6, 0, 0) M(7, 0, 0) M(8, 0, 0)        \
+    M(9, 0, 0) M(10, 0, 0) M(11, 0, 0) M(12, 0, 0) M(13, 0, 0)                                     \
+    M(7, 6, 0) M(7, 6, 1) M(8, 5, 0) M(8, 5, 1) M(9, 4, 0) M(9, 4, 1) M(10, 3, 0) M(10, 3, 1)
+
+namespace gnt {
+constexpr int shape_key(int log_r, int log_c, int strided) { return (log_r << 8) | (log_c << 1) | strided; }
+static inline int shape_key(const PassDesc &d) { return shape_key(d.log_r, d.log_c, d.log_s > 0 ? 1 : 0); }
+}  // namespace gnt

@@ -26,47 +26,44 @@ Tw tw() { return Tw{g_lo.data(), g_hi.data(), g_sm.data()}; }
 using HX = HostExec;
 
 // The three kernel bodies of gnt_kernels.hip, run tile after tile.
-template <int LOGR>
+template <class G>
 void k_fwd_pass(const PassDesc &d, int log_n, u64 *x, const u64 *limbs, size_t n_limbs) {
-    std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
-    const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
-    for (size_t t = 0; t < grid; t++) body_fwd<HX, LOGR>(tile.data(), tile_geom<LOGR>(d, t), x, limbs, n_limbs, tw());
+    std::vector<u64> tile(G::tile_n);
+    const size_t grid = ((size_t)1 << log_n) / G::tile_n;
+    for (size_t t = 0; t < grid; t++) body_fwd<HX, G>(tile.data(), make_geom<G>(d, t), x, limbs, n_limbs, tw());
 }
 
-template <int LOGR>
+template <class G>
 void k_inv_pass(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
-    std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
-    const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
+    std::vector<u64> tile(G::tile_n);
+    const size_t grid = ((size_t)1 << log_n) / G::tile_n;
     for (size_t t = 0; t < grid; t++)
-        body_inv<HX, LOGR>(tile.data(), tile_geom<LOGR>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
+        body_inv<HX, G>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
 }
 
-template <int LOGR>
+template <class G>
 void k_last_fused(const PassDesc &d, int log_n, const u64 *xb, const u64 *limbs, size_t n_in, u64 *xa,
                   const u64 *other, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
-    std::vector<u64> tile((size_t)1 << (LOGR + d.log_c));
-    const size_t grid = ((size_t)1 << log_n) >> (LOGR + d.log_c);
+    std::vector<u64> tile(G::tile_n);
+    const size_t grid = ((size_t)1 << log_n) / G::tile_n;
     for (size_t t = 0; t < grid; t++)
-        body_fused<HX, LOGR>(tile.data(), tile_geom<LOGR>(d, t), xb, limbs, n_in, xa, other, log_n, tw(), Ls, Hs,
-                             n_coeffs, n_limbs);
+        body_fused<HX, G>(tile.data(), make_geom<G>(d, t), xb, limbs, n_in, xa, other, log_n, tw(), Ls, Hs,
+                          n_coeffs, n_limbs);
 }
 
-#define DISPATCH(LR, CALL)                                                            \
-    switch (LR) {                                                                     \
-        case 1: CALL(1); break;  case 2: CALL(2); break;  case 3: CALL(3); break;     \
-        case 4: CALL(4); break;  case 5: CALL(5); break;  case 6: CALL(6); break;     \
-        case 7: CALL(7); break;  case 8: CALL(8); break;  case 9: CALL(9); break;     \
-        case 10: CALL(10); break; case 11: CALL(11); break; case 12: CALL(12); break; \
-        case 13: CALL(13); break;                                                     \
-        default: break;                                                               \
+#define DISPATCH(D, CALL)                                            \
+    switch (shape_key(D)) {                                          \
+        GNT_FOR_EACH_SHAPE(CALL)                                     \
+        default: break;                                              \
     }
 
 void forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs) {
     for (int p = 0; p < npasses; p++) {
         const PassDesc &d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
-#define F(R) k_fwd_pass<R>(d, pl.log_n, x, src, n_limbs)
-        DISPATCH(d.log_r, F)
+#define F(LR, LC, S) \
+    case shape_key(LR, LC, S): k_fwd_pass<Geom<LR, LC, S>>(d, pl.log_n, x, src, n_limbs); break;
+        DISPATCH(d, F)
 #undef F
     }
 }
@@ -76,8 +73,9 @@ void inverse(const Plan &pl, int from, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs
         const PassDesc &d = pl.pass[p];
         u64 *ls = p == 0 ? Ls : nullptr;
         u64 *hs = p == 0 ? Hs : nullptr;
-#define F(R) k_inv_pass<R>(d, pl.log_n, x, ls, hs, n_coeffs, n_limbs)
-        DISPATCH(d.log_r, F)
+#define F(LR, LC, S) \
+    case shape_key(LR, LC, S): k_inv_pass<Geom<LR, LC, S>>(d, pl.log_n, x, ls, hs, n_coeffs, n_limbs); break;
+        DISPATCH(d, F)
 #undef F
     }
 }
@@ -149,9 +147,12 @@ size_t emu_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
         const u64 *bsrc = sqr ? xa.data() : xb.data();
         const u64 *blimbs = only ? (sqr ? a : b) : nullptr;
         const u64 *other = sqr ? nullptr : xa.data();
-#define F(R) k_last_fused<R>(d, log_n, bsrc, blimbs, sqr ? na : nb, xa.data(), other, only ? Ls.data() : nullptr, \
-                             only ? Hs.data() : nullptr, n_out, nl)
-        DISPATCH(d.log_r, F)
+#define F(LR, LC, S)                                                                                       \
+    case shape_key(LR, LC, S):                                                                             \
+        k_last_fused<Geom<LR, LC, S>>(d, log_n, bsrc, blimbs, sqr ? na : nb, xa.data(), other,             \
+                                      only ? Ls.data() : nullptr, only ? Hs.data() : nullptr, n_out, nl); \
+        break;
+        DISPATCH(d, F)
 #undef F
     }
     inverse(pl, P - 2, xa.data(), Ls.data(), Hs.data(), n_out, nl);
