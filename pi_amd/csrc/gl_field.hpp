@@ -31,20 +31,23 @@ typedef uint32_t u32;
 constexpr u64 P = 0xFFFFFFFF00000001ull;
 constexpr u64 EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
-// (a + b) mod p for a, b in [0, p).  s = a + b; t = s - p = s + EPS (mod 2^64).
-// The answer is t when a + b >= p, i.e. when either add carried.
+// (a + b) mod p for a, b in [0, p).  s = a + b; when the add carried or
+// s >= p the answer is s - p = s + EPS (mod 2^64).  The correction is a 32-bit
+// mask (EPS < 2^32), so it costs one select and one add, not a 64-bit select.
 GL_HD u64 add(u64 a, u64 b) {
-    u64 s = a + b;
-    bool c1 = s < a;
-    u64 t = s + EPS;
-    bool c2 = t < s;
-    return (c1 | c2) ? t : s;
+    u64 s;
+    bool c = __builtin_add_overflow(a, b, &s);
+    u32 m = (c | (s >= P)) ? (u32)EPS : 0u;
+    return s + (u64)m;
 }
 
-// (a - b) mod p for a, b in [0, p).  On borrow add p, i.e. subtract EPS.
+// (a - b) mod p for a, b in [0, p).  On borrow add p, i.e. subtract EPS
+// (cannot underflow again: a - b + 2^64 >= 2^64 - p + 1 > EPS).
 GL_HD u64 sub(u64 a, u64 b) {
-    u64 d = a - b;
-    return (a < b) ? d - EPS : d;
+    u64 d;
+    bool bw = __builtin_sub_overflow(a, b, &d);
+    u32 m = bw ? (u32)EPS : 0u;
+    return d - (u64)m;
 }
 
 GL_HD u64 neg(u64 a) { return sub(0, a); }
@@ -54,14 +57,14 @@ GL_HD u64 neg(u64 a) { return sub(0, a); }
 GL_HD u64 reduce128(u64 lo, u64 hi) {
     u64 hh = hi >> 32;
     u64 hl = hi & 0xFFFFFFFFull;
-    u64 t0 = lo - hh;
-    if (lo < hh) t0 -= EPS;             // lo - hh + p; cannot underflow
-    u64 t1 = (hl << 32) - hl;           // hl * EPS < p
-    u64 s = t0 + t1;
-    bool c1 = s < t0;                   // wrapped: true value s + 2^64 = s + EPS (mod p)
-    u64 t = s + EPS;                    // s - p (mod 2^64); also s + 2^64 - p
-    bool c2 = t < s;                    // s >= p
-    return (c1 | c2) ? t : s;           // both branches land in [0, p)
+    u64 t0;
+    bool bw = __builtin_sub_overflow(lo, hh, &t0);  // on borrow: lo - hh + p, no second borrow
+    t0 -= (u64)(bw ? (u32)EPS : 0u);
+    u64 t1 = (hl << 32) - hl;                   // hl * EPS < p
+    u64 s;
+    bool c = __builtin_add_overflow(t0, t1, &s);    // wrapped: true value s + 2^64 = s + EPS (mod p)
+    u32 m = (c | (s >= P)) ? (u32)EPS : 0u;
+    return s + (u64)m;                          // lands in [0, p) in both cases
 }
 
 // Full 64x64 -> 128 product from four 32x32 -> 64 pieces.
