@@ -100,6 +100,24 @@ int gnt_mul_batch(gnt_ctx *ctx, gnt_job *jobs, size_t n);
 /* Same jobs, one product at a time (the unbatched path, for comparison). */
 int gnt_mul_batch_serial(gnt_ctx *ctx, gnt_job *jobs, size_t n);
 
+/* Batch plans: a fixed set of product shapes with device-resident operand,
+ * scratch and result buffers, scheduled once (every level of every product
+ * grouped into one launch per kernel shape) and optionally captured as a
+ * hipGraph; replay with new operands of the same shapes.  No reference
+ * counterpart: they serve the per-level products of binary splitting. */
+typedef struct gnt_batch gnt_batch;
+int gnt_batch_create(gnt_ctx *ctx, const size_t *na, const size_t *nb, const int *is_sqr, size_t n, int use_graph,
+                     gnt_batch **out);
+/* Copy jobs[i].a / .b (host) into the plan's operand buffers (shapes must match). */
+int gnt_batch_upload(gnt_batch *b, const gnt_job *jobs);
+/* Enqueue one run: mode 0 = batched (graph replay when captured), 1 = one
+ * launch sequence per product.  Returns without waiting. */
+int gnt_batch_run(gnt_batch *b, int mode);
+/* Wait, then copy results into jobs[i].out (trimmed; jobs[i].out_len set). */
+int gnt_batch_download(gnt_batch *b, gnt_job *jobs);
+int gnt_batch_info(gnt_batch *b, size_t *launches_per_run, size_t *iterations, size_t *device_bytes);
+int gnt_batch_destroy(gnt_batch *b);
+
 /* ---- introspection ---- */
 /* Transform plan the library uses for an na x nb product. */
 int gnt_plan_info(size_t na, size_t nb, int *log_n, int *npass, int *log_r /* [4] */);

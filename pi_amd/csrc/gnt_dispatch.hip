@@ -1,5 +1,4 @@
-// Host-side multiply dispatch with the GPU product as its transform leaf, and
-// the batched entry point.
+// Host-side multiply dispatch with the GPU product as its transform leaf.
 //
 // gnt_mul_mag restates bignum's mul_mag / mul_mag_dispatch
 // (crates/bignum/src/integer.rs:445-486): schoolbook below the Karatsuba
@@ -158,62 +157,6 @@ struct Dispatch {
 };
 
 }  // namespace
-
-int run_batch(Ctx *c, gnt_job *jobs, size_t n) {
-    if (!c) return set_error(GNT_E_ARG, "null context");
-    if (n && !jobs) return set_error(GNT_E_ARG, "null jobs");
-    if (hipSetDevice(c->device) != hipSuccess) return set_error(GNT_E_DEVICE, "hipSetDevice failed");
-    std::lock_guard<std::mutex> g(c->mu);
-    struct Slot {
-        u64 *da = nullptr, *db = nullptr, *dout = nullptr;
-        uint32_t *dlen = nullptr;
-        uint32_t len = 0;
-        size_t na = 0, nb = 0;
-    };
-    std::vector<Slot> slots(n);
-    int first = GNT_OK;
-    for (size_t i = 0; i < n; i++) {
-        gnt_job &j = jobs[i];
-        Slot &s = slots[i];
-        j.status = GNT_OK;
-        j.out_len = 0;
-        const bool sqr = j.is_sqr != 0;
-        s.na = j.na;
-        s.nb = sqr ? j.na : j.nb;
-        if (s.na == 0 || s.nb == 0) continue;
-        Plan pl;
-        if (!j.a || (!sqr && !j.b) || !j.out || j.out_cap < s.na + s.nb) { j.status = GNT_E_ARG; continue; }
-        if (!plan_for(s.na, s.nb, pl)) { j.status = GNT_E_TOO_LARGE; continue; }
-        s.da = (u64 *)c->pool.acquire(s.na * 8);
-        s.db = sqr ? nullptr : (u64 *)c->pool.acquire(s.nb * 8);
-        s.dout = (u64 *)c->pool.acquire((s.na + s.nb) * 8);
-        s.dlen = (uint32_t *)c->pool.acquire(sizeof(uint32_t));
-        if (!s.da || (!sqr && !s.db) || !s.dout || !s.dlen) { j.status = GNT_E_NOMEM; continue; }
-        if (hipMemcpyAsync(s.da, j.a, s.na * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-            (!sqr && hipMemcpyAsync(s.db, j.b, s.nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
-            j.status = GNT_E_DEVICE;
-            continue;
-        }
-        j.status = c->enqueue_product(s.da, s.na, s.db, s.nb, s.dout, s.dlen);
-        if (j.status) continue;
-        if (hipMemcpyAsync(&s.len, s.dlen, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipMemcpyAsync(j.out, s.dout, (s.na + s.nb) * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-            j.status = GNT_E_DEVICE;
-    }
-    hipError_t se = hipStreamSynchronize(c->stream);
-    for (size_t i = 0; i < n; i++) {
-        Slot &s = slots[i];
-        if (se != hipSuccess && jobs[i].status == GNT_OK && s.da) jobs[i].status = GNT_E_DEVICE;
-        if (jobs[i].status == GNT_OK) jobs[i].out_len = s.dout ? s.len : 0;
-        c->pool.release(s.da, s.na * 8);
-        c->pool.release(s.db, s.nb * 8);
-        c->pool.release(s.dout, (s.na + s.nb) * 8);
-        c->pool.release(s.dlen, sizeof(uint32_t));
-        if (jobs[i].status && !first) first = jobs[i].status;
-    }
-    if (first) set_error(first, "one or more batch jobs failed");
-    return first;
-}
 
 }  // namespace gnt
 

@@ -34,6 +34,8 @@ EXPORTS = (
     "gnt_version", "gnt_last_error", "gnt_init", "gnt_shutdown", "gnt_mul", "gnt_sqr", "gnt_mul_mag",
     "gnt_buf_upload", "gnt_buf_download", "gnt_buf_len", "gnt_buf_release", "gnt_mul_buf", "gnt_sqr_buf",
     "gnt_sync", "gnt_mul_batch", "gnt_mul_batch_serial", "gnt_plan_info", "gnt_set_timing", "gnt_get_timing",
+    "gnt_batch_create", "gnt_batch_upload", "gnt_batch_run", "gnt_batch_download", "gnt_batch_info",
+    "gnt_batch_destroy",
 )
 
 
@@ -82,6 +84,13 @@ def load() -> ctypes.CDLL:
     L.gnt_mul_batch_serial.argtypes = [ctypes.c_void_p, ctypes.POINTER(GntJob), SZ]
     L.gnt_plan_info.argtypes = [SZ, SZ, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_int)]
+    L.gnt_batch_create.argtypes = [ctypes.c_void_p, PSZ, PSZ, ctypes.POINTER(ctypes.c_int), SZ, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_void_p)]
+    L.gnt_batch_upload.argtypes = [ctypes.c_void_p, ctypes.POINTER(GntJob)]
+    L.gnt_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.gnt_batch_download.argtypes = [ctypes.c_void_p, ctypes.POINTER(GntJob)]
+    L.gnt_batch_info.argtypes = [ctypes.c_void_p, PSZ, PSZ, PSZ]
+    L.gnt_batch_destroy.argtypes = [ctypes.c_void_p]
     L.gnt_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.gnt_get_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_uint64)]
@@ -226,6 +235,9 @@ class Context:
         _check(rc)
         return [o[:j.out_len].copy() for o, j in zip(outs, jobs)]
 
+    def batch_plan(self, pairs: Sequence, graph: bool = True) -> "BatchPlan":
+        return BatchPlan(self, pairs, graph)
+
     # ---- timing ----
     def set_timing(self, on: bool):
         _check(load().gnt_set_timing(self.h, int(on)))
@@ -235,6 +247,39 @@ class Context:
         n = ctypes.c_uint64()
         _check(load().gnt_get_timing(self.h, ms, 5, ctypes.byref(n)))
         return list(ms), n.value
+
+
+class BatchPlan:
+    """gnt_batch: fixed product shapes, device-resident buffers, optional hipGraph."""
+
+    def __init__(self, ctx: Context, pairs: Sequence, graph: bool = True):
+        self.ctx = ctx
+        self.jobs, self._keep, self.outs = ctx._jobs(pairs)
+        n = len(pairs)
+        na = (SZ * n)(*[j.na for j in self.jobs])
+        nb = (SZ * n)(*[j.nb for j in self.jobs])
+        sq = (ctypes.c_int * n)(*[j.is_sqr for j in self.jobs])
+        h = ctypes.c_void_p()
+        _check(load().gnt_batch_create(ctx.h, na, nb, sq, n, int(graph), ctypes.byref(h)))
+        self.h = h
+        _check(load().gnt_batch_upload(self.h, self.jobs))
+
+    def run(self, serial: bool = False):
+        _check(load().gnt_batch_run(self.h, 1 if serial else 0))
+
+    def results(self):
+        _check(load().gnt_batch_download(self.h, self.jobs))
+        return [o[:j.out_len].copy() for o, j in zip(self.outs, self.jobs)]
+
+    def info(self):
+        a, b, c = SZ(), SZ(), SZ()
+        _check(load().gnt_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"launches_per_run": a.value, "iterations": b.value, "device_bytes": c.value}
+
+    def close(self):
+        if self.h:
+            load().gnt_batch_destroy(self.h)
+            self.h = None
 
 
 _default: Optional[Context] = None

@@ -238,3 +238,59 @@ def test_mul_mag_dispatch(ctx, orc, la, lb):
     check(orc, ctx.mul_mag(a, b), a, b)
     # a low NTT threshold routes the Karatsuba leaves to the GPU too
     check(orc, ctx.mul_mag(a, b, ntt_threshold=64), a, b)
+
+
+# ---- batch plans: level-grouped launches, hipGraph capture and replay ----
+
+def _mixed_pairs(orc, n, lo, hi, seed, sqr_every=4):
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for i in range(n):
+        m = int(round(2 ** rng.uniform(lo, hi)))
+        if i % 11 == 5:
+            a = np.full(m, 2**64 - 1, dtype=np.uint64)   # maximal carries inside a batch
+        else:
+            a = orc.splitmix64(seed * 1000 + i, m)
+        b = None if i % sqr_every == 0 else orc.splitmix64(seed * 1000 + 500 + i, int(round(2 ** rng.uniform(lo, hi))))
+        pairs.append((a, b))
+    return pairs
+
+
+def _want(orc, a, b):
+    return orc.gmp_sqr(a) if b is None else orc.gmp_mul(a, b)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_plan_replay(ctx, orc, graph):
+    from pi_amd import gnt
+    pairs = _mixed_pairs(orc, 48, 1, 15, 3)
+    log(f"batch plan graph={graph} n={len(pairs)}")
+    plan = ctx.batch_plan(pairs, graph=graph)
+    info = plan.info()
+    assert info["launches_per_run"] < 6 * len(pairs)   # levels grouped, not one sequence per product
+    plan.run()
+    for (a, b), got in zip(pairs, plan.results()):
+        assert np.array_equal(got, _want(orc, a, b))
+    # same shapes, new operands: upload and replay
+    pairs2 = [(orc.splitmix64(77 + i, a.size), None if b is None else orc.splitmix64(99 + i, b.size))
+              for i, (a, b) in enumerate(pairs)]
+    jobs, keep, outs = ctx._jobs(pairs2)
+    gnt._check(gnt.load().gnt_batch_upload(plan.h, jobs))
+    plan.run()
+    plan.run()   # a second replay on the same inputs gives the same results
+    for (a, b), got in zip(pairs2, plan.results()):
+        assert np.array_equal(got, _want(orc, a, b))
+    plan.run(serial=True)  # one launch sequence per product, same buffers
+    for (a, b), got in zip(pairs2, plan.results()):
+        assert np.array_equal(got, _want(orc, a, b))
+    plan.close()
+
+
+def test_batch_plan_uneven_large(ctx, orc):
+    """Products from 2^8 to 2^20 limbs in one plan (1-, 2- and 3-pass plans together)."""
+    pairs = _mixed_pairs(orc, 24, 8, 20, 5, sqr_every=3)
+    plan = ctx.batch_plan(pairs, graph=True)
+    plan.run()
+    for (a, b), got in zip(pairs, plan.results()):
+        assert np.array_equal(got, _want(orc, a, b))
+    plan.close()
