@@ -93,8 +93,11 @@ def main():
     ap.add_argument("--op", choices=["mul", "sqr"], default="mul")
     ap.add_argument("--cpu-log-size", type=int, default=22, help="log2 limbs of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batched"], default="single",
-                    help="single: the metric (one 2^24 product per step); batched: 1024 uneven pairs per step")
+    ap.add_argument("--workload", choices=["single", "batched", "serve"], default="single",
+                    help="single: the metric (one 2^24 product per step); batched: 1024 uneven pairs per step; "
+                         "serve: open-loop Poisson request stream through the serving loop")
+    ap.add_argument("--rates", default="100,200,400,800,1600", help="serve: arrival rates (requests/s) to sweep")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--pairs", type=int, default=1024)
     args = ap.parse_args()
 
@@ -107,6 +110,8 @@ def main():
 
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     n = 1 << args.log_size
     a, b = operand(3, n), operand(4, n)
@@ -247,6 +252,125 @@ def bench_batched(args):
                    "total_operand_limbs": limbs, "modes": res},
     }
     ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Config: serving.  Open-loop Poisson arrivals; n = round(2^U[8,24]) limbs;
+    70% multiply / 30% square; 20% of requests take a finished result that is
+    still on the device as operand; HBM admission cap 200 GB.  Latency is
+    submit-to-completion as recorded by the library."""
+    import threading
+
+    from pi_amd import gnt
+
+    rng = np.random.default_rng(8)
+    pool_limbs = (1 << 25) + (1 << 24)
+    src = operand(8, pool_limbs)  # operands are slices of one random buffer
+    ctx = gnt.Context(0, hbm_cap_bytes=200 * 10**9)
+    srv = gnt.Server(ctx)
+    points = []
+    q61 = (1 << 61) - 1
+    for rate in [float(r) for r in args.rates.split(",")]:
+        lock = threading.Lock()
+        ready = []          # finished results available for reuse: (DeviceBuf, limbs)
+        lat, limbs_done, checks = [], [0], [0, 0]
+        tickets = []
+        meta = {}
+        consumed = []       # operands taken from `ready`, released after the run
+        done_evt = threading.Event()
+
+        def waiter():
+            for k in range(args.requests):
+                while k >= len(tickets):
+                    time.sleep(0.0002)
+                t = tickets[k]
+                buf, us = srv.wait(t)
+                lat.append(us)
+                n_in, want, keep = meta[t]
+                limbs_done[0] += n_in
+                res = None
+                if want is not None:  # sampled exactness check on the residue mod 2^61-1
+                    out = buf.download()
+                    res = int.from_bytes(out.astype("<u8").tobytes(), "little") % q61
+                    checks[0] += 1
+                    checks[1] += int(res == want)
+                if keep and n_in <= (1 << 24):
+                    with lock:
+                        ready.append((buf, n_in, res))
+                        while len(ready) > 8:
+                            ready.pop(0)[0].release()
+                else:
+                    buf.release()
+            done_evt.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        next_t = t0
+        for i in range(args.requests):
+            next_t += rng.exponential(1.0 / rate)
+            now = time.perf_counter()
+            if next_t > now:
+                time.sleep(next_t - now)
+            n = int(round(2 ** rng.uniform(8, 24)))
+            sq = rng.uniform() < 0.3
+            reuse = rng.uniform() < 0.2
+            off = int(rng.integers(0, pool_limbs - n))
+            a = src[off:off + n]
+            a_buf = None
+            with lock:
+                if reuse and ready:
+                    a_buf, n_a, a_res = ready[int(rng.integers(0, len(ready)))]
+                    ready.remove((a_buf, n_a, a_res))  # consumed: released after this request completes
+            sample = i % 25 == 0
+            if a_buf is None:
+                n_a = n
+                a_res = int.from_bytes(a.astype("<u8").tobytes(), "little") % q61 if sample else None
+            if sq:
+                t = srv.submit(None if a_buf is not None else a, square=True, a_buf=a_buf)
+                n_in = 2 * n_a
+                want = (a_res * a_res) % q61 if (sample and a_res is not None) else None
+            else:
+                off2 = int(rng.integers(0, pool_limbs - n))
+                b = src[off2:off2 + n]
+                t = srv.submit(None if a_buf is not None else a, b, a_buf=a_buf)
+                n_in = n_a + n
+                want = None
+                if sample and a_res is not None:
+                    want = a_res * (int.from_bytes(b.astype("<u8").tobytes(), "little") % q61) % q61
+            meta[t] = (n_in, want, rng.uniform() < 0.5)
+            if a_buf is not None:
+                consumed.append((t, a_buf))
+            tickets.append(t)
+        done_evt.wait()
+        th.join()
+        wall = time.perf_counter() - t0
+        for item in ready:
+            item[0].release()
+        for _, buf in consumed:
+            buf.release()
+        lat_ms = np.array(lat) / 1e3
+        points.append({"rate_req_per_s": rate, "requests": args.requests,
+                       "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                       "throughput_Mlimbs_per_s": round(limbs_done[0] / wall / 1e6, 1),
+                       "achieved_req_per_s": round(args.requests / wall, 1),
+                       "residue_checks_passed": f"{checks[1]}/{checks[0]}"})
+        print(json.dumps({"serve_point": points[-1]}), file=sys.stderr, flush=True)
+    stats = srv.stats()
+    srv.stop()
+    ctx.close()
+    best = max(points, key=lambda p: p["throughput_Mlimbs_per_s"])
+    line = {
+        "metric": "served multiply/square p50/p99 latency and throughput",
+        "value": best["throughput_Mlimbs_per_s"], "unit": "Mlimbs/s", "n_gpus": 1,
+        "steps": args.requests, "warmup": 0, "ms_per_step": best["p50_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (SplitMix64 limbs, Poisson arrivals, seed 8)",
+        "config": {"workload": "open-loop Poisson; n = round(2^U[8,24]); 70/30 mul/sqr; 20% reuse of device results; "
+                               "HBM cap 200 GB", "sweep": points, "server": stats},
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -118,6 +118,32 @@ int gnt_batch_download(gnt_batch *b, gnt_job *jobs);
 int gnt_batch_info(gnt_batch *b, size_t *launches_per_run, size_t *iterations, size_t *device_bytes);
 int gnt_batch_destroy(gnt_batch *b);
 
+/* ---- serving loop (continuous batching) ----
+ * A scheduler thread advances all admitted requests one stage per iteration
+ * (one launch per kernel shape per iteration) and admits waiting requests, in
+ * arrival order, between iterations while the bytes in flight stay under
+ * gnt_config.hbm_cap_bytes.  Host operands must stay valid until the request
+ * completes.  An operand may instead be a gnt_buf (e.g. an earlier result).
+ * No reference counterpart: the reference has no serving mode. */
+typedef struct gnt_req {
+    int op;             /* 0 multiply, 1 square (b ignored) */
+    const uint64_t *a;  /* host operand, or NULL when a_buf is set */
+    size_t na;
+    const uint64_t *b;
+    size_t nb;
+    gnt_buf *a_buf;     /* device operand (a finished result), or NULL */
+    gnt_buf *b_buf;
+} gnt_req;
+int gnt_serve_start(gnt_ctx *ctx);
+int gnt_serve_stop(gnt_ctx *ctx);   /* drains in-flight requests */
+int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket);
+int gnt_poll(gnt_ctx *ctx, uint64_t ticket, int *done);
+/* Wait for a request (timeout_us < 0: forever).  On success *result is a new
+ * device buffer owned by the caller; *latency_us = submit-to-completion. */
+int gnt_wait(gnt_ctx *ctx, uint64_t ticket, int64_t timeout_us, gnt_buf **result, double *latency_us);
+/* [0] iterations [1] launches [2] completed [3] peak admitted [4] peak bytes in flight [5] bytes in flight */
+int gnt_serve_stats(gnt_ctx *ctx, uint64_t *stats, int n);
+
 /* ---- introspection ---- */
 /* Transform plan the library uses for an na x nb product. */
 int gnt_plan_info(size_t na, size_t nb, int *log_n, int *npass, int *log_r /* [4] */);

@@ -205,11 +205,30 @@ using gnt::Ctx;
 
 struct gnt_buf {
     u64 *d = nullptr;
-    size_t cap = 0;      // limbs allocated (untrimmed length of the value)
-    u32 *dlen = nullptr; // device-side trimmed length
+    size_t cap = 0;         // limbs allocated (untrimmed length of the value)
+    u32 *dlen = nullptr;    // device-side trimmed length
     size_t len = 0;
     bool len_known = false;
+    bool complete = false;  // contents final (no pending work writes them)
 };
+
+namespace gnt {
+gnt_buf *buf_wrap(Ctx *c, u64 *d, size_t cap, u32 *dlen) {
+    (void)c;
+    gnt_buf *b = new gnt_buf();
+    b->d = d;
+    b->cap = cap;
+    b->dlen = dlen;
+    b->complete = true;  // handed out only after its product finished
+    return b;
+}
+bool buf_info(gnt_buf *b, u64 **d, size_t *cap) {
+    if (!b) return false;
+    *d = b->d;
+    *cap = b->len_known ? b->len : b->cap;
+    return true;
+}
+}  // namespace gnt
 
 extern "C" {
 
@@ -226,7 +245,8 @@ int gnt_init(gnt_ctx **out, const gnt_config *cfg) {
     Ctx *c = new Ctx();
     c->device = dev;
     c->hbm_cap = (cfg && cfg->hbm_cap_bytes) ? cfg->hbm_cap_bytes : (size_t)200 * 1000 * 1000 * 1000;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(GNT_E_DEVICE, "stream creation failed");
     }
@@ -242,6 +262,7 @@ int gnt_init(gnt_ctx **out, const gnt_config *cfg) {
 int gnt_shutdown(gnt_ctx *ctx) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c) return GNT_OK;
+    gnt_serve_stop(ctx);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->pool.trim();
@@ -249,6 +270,7 @@ int gnt_shutdown(gnt_ctx *ctx) {
     (void)hipFree(c->whi);
     (void)hipFree(c->wsm);
     (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->copy_stream);
     delete c;
     return GNT_OK;
 }
@@ -345,15 +367,19 @@ int gnt_buf_upload(gnt_ctx *ctx, const uint64_t *limbs, size_t n, gnt_buf **out)
     HIP_TRY(hipStreamSynchronize(c->stream));
     b->len = len;
     b->len_known = true;
+    b->complete = true;
     *out = b;
     return GNT_OK;
 }
 
+// Length of a buffer.  A finished buffer is read over the copy stream without
+// waiting for unrelated work on the compute stream.
 static int buf_len_locked(Ctx *c, gnt_buf *b, size_t *len) {
     if (!b->len_known) {
         u32 l = 0;
-        HIP_TRY(hipMemcpyAsync(&l, b->dlen, sizeof(u32), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
+        hipStream_t st = b->complete ? c->copy_stream : c->stream;
+        HIP_TRY(hipMemcpyAsync(&l, b->dlen, sizeof(u32), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
         b->len = l;
         b->len_known = true;
     }
@@ -365,6 +391,7 @@ int gnt_buf_len(gnt_ctx *ctx, gnt_buf *b, size_t *len) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (int rc = check_ctx(c)) return rc;
     if (!b || !len) return fail(GNT_E_ARG, "null argument");
+    if (b->complete) return buf_len_locked(c, b, len);
     std::lock_guard<std::mutex> g(c->mu);
     return buf_len_locked(c, b, len);
 }
@@ -373,13 +400,15 @@ int gnt_buf_download(gnt_ctx *ctx, gnt_buf *b, uint64_t *out, size_t cap, size_t
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (int rc = check_ctx(c)) return rc;
     if (!b || !out_len) return fail(GNT_E_ARG, "null argument");
-    std::lock_guard<std::mutex> g(c->mu);
+    std::unique_lock<std::mutex> g(c->mu, std::defer_lock);
+    if (!b->complete) g.lock();
     size_t len;
     if (int rc = buf_len_locked(c, b, &len)) return rc;
     if (cap < len || (len && !out)) return fail(GNT_E_ARG, "capacity %zu < %zu", cap, len);
     if (len) {
-        HIP_TRY(hipMemcpyAsync(out, b->d, len * 8, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
+        hipStream_t st = b->complete ? c->copy_stream : c->stream;
+        HIP_TRY(hipMemcpyAsync(out, b->d, len * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
     }
     *out_len = len;
     return GNT_OK;

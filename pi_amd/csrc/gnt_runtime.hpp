@@ -31,6 +31,7 @@ private:
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // downloads of finished buffers, off the compute stream
     std::mutex mu;  // serialises host entry points on this context
     uint64_t *wlo = nullptr, *whi = nullptr, *wsm = nullptr;
     Tw tw() const { return Tw{wlo, whi, wsm}; }
@@ -39,6 +40,7 @@ struct Ctx {
     bool timing = false;
     double phase_ms[5] = {0, 0, 0, 0, 0};
     uint64_t products = 0;
+    void *server = nullptr;  // serving loop (gnt_serve.hip) while running
 
     int init_tables();
     int enqueue_product(const uint64_t *da, size_t na, const uint64_t *db, size_t nb, uint64_t *dout,

@@ -35,7 +35,8 @@ EXPORTS = (
     "gnt_buf_upload", "gnt_buf_download", "gnt_buf_len", "gnt_buf_release", "gnt_mul_buf", "gnt_sqr_buf",
     "gnt_sync", "gnt_mul_batch", "gnt_mul_batch_serial", "gnt_plan_info", "gnt_set_timing", "gnt_get_timing",
     "gnt_batch_create", "gnt_batch_upload", "gnt_batch_run", "gnt_batch_download", "gnt_batch_info",
-    "gnt_batch_destroy",
+    "gnt_batch_destroy", "gnt_serve_start", "gnt_serve_stop", "gnt_submit", "gnt_poll", "gnt_wait",
+    "gnt_serve_stats",
 )
 
 
@@ -48,6 +49,11 @@ class GntError(RuntimeError):
 class GntConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_cap_bytes", SZ), ("plan_cache_path", ctypes.c_char_p),
                 ("reserved", SZ * 4)]
+
+
+class GntReq(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int), ("a", PU64), ("na", SZ), ("b", PU64), ("nb", SZ),
+                ("a_buf", ctypes.c_void_p), ("b_buf", ctypes.c_void_p)]
 
 
 class GntJob(ctypes.Structure):
@@ -91,6 +97,13 @@ def load() -> ctypes.CDLL:
     L.gnt_batch_download.argtypes = [ctypes.c_void_p, ctypes.POINTER(GntJob)]
     L.gnt_batch_info.argtypes = [ctypes.c_void_p, PSZ, PSZ, PSZ]
     L.gnt_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.gnt_serve_start.argtypes = [ctypes.c_void_p]
+    L.gnt_serve_stop.argtypes = [ctypes.c_void_p]
+    L.gnt_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(GntReq), ctypes.POINTER(ctypes.c_uint64)]
+    L.gnt_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
+    L.gnt_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p),
+                           ctypes.POINTER(ctypes.c_double)]
+    L.gnt_serve_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
     L.gnt_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.gnt_get_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_uint64)]
@@ -280,6 +293,59 @@ class BatchPlan:
         if self.h:
             load().gnt_batch_destroy(self.h)
             self.h = None
+
+
+class Server:
+    """The serving loop (continuous batching) of a context."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self._keep = {}
+        _check(load().gnt_serve_start(ctx.h))
+
+    def submit(self, a=None, b=None, square: bool = False, a_buf: Optional[DeviceBuf] = None,
+               b_buf: Optional[DeviceBuf] = None) -> int:
+        req = GntReq()
+        req.op = 1 if square else 0
+        keep = []
+        if a_buf is not None:
+            req.a_buf = a_buf.handle
+        else:
+            x = _u64(a)
+            keep.append(x)
+            req.a, req.na = _p(x), x.size
+        if not square:
+            if b_buf is not None:
+                req.b_buf = b_buf.handle
+            else:
+                y = _u64(b)
+                keep.append(y)
+                req.b, req.nb = _p(y), y.size
+        t = ctypes.c_uint64()
+        _check(load().gnt_submit(self.ctx.h, ctypes.byref(req), ctypes.byref(t)))
+        self._keep[t.value] = keep  # host operands live until the request completes
+        return t.value
+
+    def done(self, ticket: int) -> bool:
+        d = ctypes.c_int()
+        _check(load().gnt_poll(self.ctx.h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int, timeout_us: int = -1):
+        h = ctypes.c_void_p()
+        lat = ctypes.c_double()
+        _check(load().gnt_wait(self.ctx.h, ticket, timeout_us, ctypes.byref(h), ctypes.byref(lat)))
+        self._keep.pop(ticket, None)
+        return DeviceBuf(self.ctx, h), lat.value
+
+    def stats(self):
+        v = (ctypes.c_uint64 * 6)()
+        _check(load().gnt_serve_stats(self.ctx.h, v, 6))
+        return dict(zip(["iterations", "launches", "completed", "peak_admitted", "peak_bytes", "bytes_in_flight"],
+                        list(v)))
+
+    def stop(self):
+        _check(load().gnt_serve_stop(self.ctx.h))
 
 
 _default: Optional[Context] = None

@@ -294,3 +294,66 @@ def test_batch_plan_uneven_large(ctx, orc):
     for (a, b), got in zip(pairs, plan.results()):
         assert np.array_equal(got, _want(orc, a, b))
     plan.close()
+
+
+# ---- serving loop: continuous batching ----
+
+def test_serving_mix_equals_alone(ctx, orc):
+    """Requests submitted together (work overlapping on the device) give the
+    same bytes as the same request alone, and the exact product."""
+    from pi_amd import gnt
+    rng = np.random.default_rng(11)
+    reqs = []
+    for i in range(40):
+        n = int(round(2 ** rng.uniform(4, 17)))
+        a = orc.splitmix64(3000 + i, n)
+        sq = i % 3 == 0
+        b = None if sq else orc.splitmix64(4000 + i, int(round(2 ** rng.uniform(4, 17))))
+        reqs.append((a, b))
+    srv = gnt.Server(ctx)
+    try:
+        tickets = [srv.submit(a, b, square=b is None) for a, b in reqs]
+        results = []
+        for t in tickets:
+            buf, lat = srv.wait(t)
+            assert lat > 0
+            results.append(buf.download())
+            buf.release()
+        st = srv.stats()
+        assert st["completed"] >= len(reqs) and st["peak_admitted"] > 1   # they were in flight together
+        # a result kept on the device as the next request's operand (no host round trip)
+        a, b = reqs[1]
+        t = srv.submit(a, b)
+        prod, _ = srv.wait(t)
+        t2 = srv.submit(a_buf=prod, square=True)
+        sq, _ = srv.wait(t2)
+        A, B = orc.to_int(a), orc.to_int(b)
+        assert orc.to_int(sq.download()) == (A * B) ** 2
+        prod.release(); sq.release()
+    finally:
+        srv.stop()
+    for (a, b), got in zip(reqs, results):
+        alone = ctx.sqr(a) if b is None else ctx.mul_mag_ntt(a, b)
+        assert np.array_equal(got, alone)
+        assert np.array_equal(got, orc.gmp_sqr(a) if b is None else orc.gmp_mul(a, b))
+
+
+def test_serving_admission_cap(orc):
+    """With a tiny HBM cap the loop admits one request at a time and still completes all."""
+    from pi_amd import gnt
+    c = gnt.Context(0, hbm_cap_bytes=1 << 20)
+    srv = gnt.Server(c)
+    try:
+        pairs = [(orc.splitmix64(50 + i, 3000), orc.splitmix64(60 + i, 2000)) for i in range(6)]
+        ts = [srv.submit(a, b) for a, b in pairs]
+        outs = []
+        for t in ts:
+            buf, _ = srv.wait(t)
+            outs.append(buf.download())
+            buf.release()
+        assert srv.stats()["peak_bytes"] <= (1 << 20) or srv.stats()["peak_admitted"] == 1
+        for (a, b), o in zip(pairs, outs):
+            assert np.array_equal(o, orc.gmp_mul(a, b))
+    finally:
+        srv.stop()
+        c.close()
