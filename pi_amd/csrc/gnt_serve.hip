@@ -180,7 +180,9 @@ struct Server {
                 // admission in arrival order while the bytes in flight fit the cap
                 while (!waiting.empty()) {
                     Req *q = waiting.front();
-                    if (!(active.empty() && draining.empty()) && bytes_in_flight + q->bytes > c->hbm_cap) break;
+                    // bytes_in_flight counts everything admitted and not retired, including
+                    // requests admitted earlier in this pass; an empty device admits anything
+                    if (bytes_in_flight > 0 && bytes_in_flight + q->bytes > c->hbm_cap) break;
                     waiting.pop_front();
                     bytes_in_flight += q->bytes;
                     peak_bytes = std::max(peak_bytes, bytes_in_flight);
