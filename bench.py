@@ -259,7 +259,8 @@ def bench_serve(args):
     """Config: serving.  Open-loop Poisson arrivals; n = round(2^U[8,24]) limbs;
     70% multiply / 30% square; 20% of requests take a finished result that is
     still on the device as operand; HBM admission cap 200 GB.  Latency is
-    submit-to-completion as recorded by the library."""
+    submit-to-completion as recorded by the library.  Every 25th request with
+    fresh operands is checked exactly (residue mod 2^61-1) after the run."""
     import threading
 
     from pi_amd import gnt
@@ -267,37 +268,51 @@ def bench_serve(args):
     rng = np.random.default_rng(8)
     pool_limbs = (1 << 25) + (1 << 24)
     src = operand(8, pool_limbs)  # operands are slices of one random buffer
+    q61 = (1 << 61) - 1
+
+    def residue(x):
+        return int.from_bytes(np.ascontiguousarray(x, dtype="<u8").tobytes(), "little") % q61
+
     ctx = gnt.Context(0, hbm_cap_bytes=200 * 10**9)
     srv = gnt.Server(ctx)
+    # warm-up: first-use allocations of every size class
+    for e in range(8, 25, 2):
+        n = 1 << e
+        t = srv.submit(src[:n], src[n:2 * n])
+        buf, _ = srv.wait(t)
+        buf.release()
+
     points = []
-    q61 = (1 << 61) - 1
     for rate in [float(r) for r in args.rates.split(",")]:
+        # the request schedule, drawn before the clock starts
+        sched = []
+        gap = rng.exponential(1.0 / rate, size=args.requests)
+        for i in range(args.requests):
+            n = int(round(2 ** rng.uniform(8, 24)))
+            sched.append(dict(at=float(gap[:i + 1].sum()), n=n, sq=bool(rng.uniform() < 0.3),
+                              reuse=bool(rng.uniform() < 0.2), keep=bool(rng.uniform() < 0.5),
+                              oa=int(rng.integers(0, pool_limbs - n)), ob=int(rng.integers(0, pool_limbs - n))))
         lock = threading.Lock()
-        ready = []          # finished results available for reuse: (DeviceBuf, limbs)
-        lat, limbs_done, checks = [], [0], [0, 0]
-        tickets = []
-        meta = {}
-        consumed = []       # operands taken from `ready`, released after the run
+        ready = []                      # finished results kept on the device for reuse
+        consumed, sampled = [], []
+        lat, limbs_done = [], [0]
+        tickets, meta = [], {}
         done_evt = threading.Event()
 
         def waiter():
             for k in range(args.requests):
                 while k >= len(tickets):
-                    time.sleep(0.0002)
+                    time.sleep(0.0001)
                 t = tickets[k]
                 buf, us = srv.wait(t)
                 lat.append(us)
-                n_in, want, keep = meta[t]
+                n_in, check, keep = meta[t]
                 limbs_done[0] += n_in
-                res = None
-                if want is not None:  # sampled exactness check on the residue mod 2^61-1
-                    out = buf.download()
-                    res = int.from_bytes(out.astype("<u8").tobytes(), "little") % q61
-                    checks[0] += 1
-                    checks[1] += int(res == want)
-                if keep and n_in <= (1 << 24):
+                if check is not None:
+                    sampled.append((buf, check))
+                elif keep and n_in <= (1 << 24):
                     with lock:
-                        ready.append((buf, n_in, res))
+                        ready.append((buf, n_in))
                         while len(ready) > 8:
                             ready.pop(0)[0].release()
                 else:
@@ -307,48 +322,49 @@ def bench_serve(args):
         th = threading.Thread(target=waiter)
         th.start()
         t0 = time.perf_counter()
-        next_t = t0
-        for i in range(args.requests):
-            next_t += rng.exponential(1.0 / rate)
-            now = time.perf_counter()
-            if next_t > now:
-                time.sleep(next_t - now)
-            n = int(round(2 ** rng.uniform(8, 24)))
-            sq = rng.uniform() < 0.3
-            reuse = rng.uniform() < 0.2
-            off = int(rng.integers(0, pool_limbs - n))
-            a = src[off:off + n]
+        for i, r in enumerate(sched):
+            wait_s = t0 + r["at"] - time.perf_counter()
+            if wait_s > 0:
+                time.sleep(wait_s)
+            n = r["n"]
+            a = src[r["oa"]:r["oa"] + n]
+            b = src[r["ob"]:r["ob"] + n]
             a_buf = None
-            with lock:
-                if reuse and ready:
-                    a_buf, n_a, a_res = ready[int(rng.integers(0, len(ready)))]
-                    ready.remove((a_buf, n_a, a_res))  # consumed: released after this request completes
-            sample = i % 25 == 0
+            if r["reuse"]:
+                with lock:
+                    if ready:
+                        a_buf, n_a = ready.pop(int(rng.integers(0, len(ready))))
             if a_buf is None:
                 n_a = n
-                a_res = int.from_bytes(a.astype("<u8").tobytes(), "little") % q61 if sample else None
-            if sq:
+            check = None
+            if a_buf is None and i % 25 == 0:
+                check = ("sq", r["oa"], n) if r["sq"] else ("mul", r["oa"], r["ob"], n)
+            if r["sq"]:
                 t = srv.submit(None if a_buf is not None else a, square=True, a_buf=a_buf)
                 n_in = 2 * n_a
-                want = (a_res * a_res) % q61 if (sample and a_res is not None) else None
             else:
-                off2 = int(rng.integers(0, pool_limbs - n))
-                b = src[off2:off2 + n]
                 t = srv.submit(None if a_buf is not None else a, b, a_buf=a_buf)
                 n_in = n_a + n
-                want = None
-                if sample and a_res is not None:
-                    want = a_res * (int.from_bytes(b.astype("<u8").tobytes(), "little") % q61) % q61
-            meta[t] = (n_in, want, rng.uniform() < 0.5)
+            meta[t] = (n_in, check, r["keep"])
             if a_buf is not None:
-                consumed.append((t, a_buf))
+                consumed.append(a_buf)
             tickets.append(t)
         done_evt.wait()
         th.join()
         wall = time.perf_counter() - t0
-        for item in ready:
-            item[0].release()
-        for _, buf in consumed:
+        # exactness of the sampled requests, off the clock
+        ok = 0
+        for buf, chk in sampled:
+            c = residue(buf.download())
+            if chk[0] == "sq":
+                want = residue(src[chk[1]:chk[1] + chk[2]]) ** 2 % q61
+            else:
+                want = residue(src[chk[1]:chk[1] + chk[3]]) * residue(src[chk[2]:chk[2] + chk[3]]) % q61
+            ok += int(c == want)
+            buf.release()
+        for buf, _ in ready:
+            buf.release()
+        for buf in consumed:
             buf.release()
         lat_ms = np.array(lat) / 1e3
         points.append({"rate_req_per_s": rate, "requests": args.requests,
@@ -356,7 +372,7 @@ def bench_serve(args):
                        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
                        "throughput_Mlimbs_per_s": round(limbs_done[0] / wall / 1e6, 1),
                        "achieved_req_per_s": round(args.requests / wall, 1),
-                       "residue_checks_passed": f"{checks[1]}/{checks[0]}"})
+                       "exact_checks_passed": f"{ok}/{len(sampled)}"})
         print(json.dumps({"serve_point": points[-1]}), file=sys.stderr, flush=True)
     stats = srv.stats()
     srv.stop()
@@ -365,7 +381,7 @@ def bench_serve(args):
     line = {
         "metric": "served multiply/square p50/p99 latency and throughput",
         "value": best["throughput_Mlimbs_per_s"], "unit": "Mlimbs/s", "n_gpus": 1,
-        "steps": args.requests, "warmup": 0, "ms_per_step": best["p50_ms"], "higher_is_better": True,
+        "steps": args.requests, "warmup": 9, "ms_per_step": best["p50_ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (SplitMix64 limbs, Poisson arrivals, seed 8)",
         "config": {"workload": "open-loop Poisson; n = round(2^U[8,24]); 70/30 mul/sqr; 20% reuse of device results; "
