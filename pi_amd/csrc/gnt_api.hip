@@ -266,6 +266,7 @@ int gnt_shutdown(gnt_ctx *ctx) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->pool.trim();
+    c->upload_pool.trim();
     (void)hipFree(c->wlo);
     (void)hipFree(c->whi);
     (void)hipFree(c->wsm);

@@ -35,7 +35,8 @@ struct Ctx {
     std::mutex mu;  // serialises host entry points on this context
     uint64_t *wlo = nullptr, *whi = nullptr, *wsm = nullptr;
     Tw tw() const { return Tw{wlo, whi, wsm}; }
-    Pool pool;
+    Pool pool;         // buffers used in compute-stream order
+    Pool upload_pool;  // serving operands written on the copy stream; freed only after completion
     size_t hbm_cap = 0;
     bool timing = false;
     double phase_ms[5] = {0, 0, 0, 0, 0};

@@ -9,7 +9,8 @@
 // run join at the next iteration, so their work overlaps on the device.
 // Results stay in HBM as gnt_buf handles (handed out once complete, and not
 // counted against the cap afterwards); a request may take such a result as an
-// operand without a host round trip.
+// operand without a host round trip.  Host operands are uploaded by
+// gnt_submit on a separate copy stream, so PCIe transfers overlap compute.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -64,7 +65,8 @@ namespace gnt {
 struct Server {
     Ctx *c;
     std::thread th;
-    std::mutex mu;  // queue, tickets
+    std::mutex mu;         // queue, tickets
+    std::mutex upload_mu;  // submissions share the copy stream
     std::condition_variable cv_work, cv_done;
     bool stop = false;
     std::deque<Req *> waiting;
@@ -85,35 +87,18 @@ struct Server {
         return p.scratch_bytes() + p.nl * 8 + (q->own_a ? p.na * 8 : 0) + (q->own_b ? p.nb * 8 : 0);
     }
 
-    // Buffers of an admitted request (scheduler thread only).
+    // Buffers of an admitted request (scheduler thread only).  Host operands
+    // were uploaded at submission (copy stream), so only scratch is allocated.
     int admit(Req *q) {
         gnt::Prod &p = q->p;
         const bool sq = q->r.op == 1;
-        if (q->r.a_buf) {
-            u64 *d;
-            size_t cap;
-            buf_info(q->r.a_buf, &d, &cap);
-            p.da = d;
-        } else {
-            q->own_a = (u64 *)c->pool.acquire(p.na * 8);
-            if (!q->own_a) return GNT_E_NOMEM;
-            if (hipMemcpyAsync(q->own_a, q->r.a, p.na * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-                return GNT_E_DEVICE;
-            p.da = q->own_a;
-        }
+        u64 *d;
+        size_t cap;
+        if (q->r.a_buf) { buf_info(q->r.a_buf, &d, &cap); p.da = d; }
+        else p.da = q->own_a;
         if (sq) p.db = nullptr;
-        else if (q->r.b_buf) {
-            u64 *d;
-            size_t cap;
-            buf_info(q->r.b_buf, &d, &cap);
-            p.db = d;
-        } else {
-            q->own_b = (u64 *)c->pool.acquire(p.nb * 8);
-            if (!q->own_b) return GNT_E_NOMEM;
-            if (hipMemcpyAsync(q->own_b, q->r.b, p.nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-                return GNT_E_DEVICE;
-            p.db = q->own_b;
-        }
+        else if (q->r.b_buf) { buf_info(q->r.b_buf, &d, &cap); p.db = d; }
+        else p.db = q->own_b;
         p.dout = (u64 *)c->pool.acquire(p.nl * 8);
         p.dlen = (uint32_t *)c->pool.acquire(4);
         p.xa = (u64 *)c->pool.acquire(p.N * 8);
@@ -130,10 +115,15 @@ struct Server {
         c->pool.release(p.xb, p.sqr() ? 0 : p.N * 8);
         c->pool.release(p.lh, p.nl * 16);
         c->pool.release(p.cs, p.cs_bytes());
-        c->pool.release(q->own_a, p.na * 8);
-        c->pool.release(q->own_b, p.nb * 8);
         p.xa = p.xb = p.lh = nullptr;
         p.cs = nullptr;
+    }
+
+    // Operands uploaded on the copy stream: released only once the product
+    // has finished, so no later upload can overwrite them under a kernel.
+    void free_operands(Req *q) {
+        c->upload_pool.release(q->own_a, q->p.na * 8);
+        c->upload_pool.release(q->own_b, q->p.nb * 8);
         q->own_a = q->own_b = nullptr;
     }
 
@@ -154,7 +144,9 @@ struct Server {
     }
 
     void fail_request(Req *q, int code) {
+        (void)hipStreamSynchronize(c->stream);
         free_scratch(q);
+        free_operands(q);
         c->pool.release(q->p.dout, q->p.nl * 8);
         c->pool.release(q->p.dlen, 4);
         std::lock_guard<std::mutex> g(mu);
@@ -240,6 +232,7 @@ struct Server {
             if (hipEventQuery(q->last) != hipSuccess) { keep.push_back(q); continue; }
             (void)hipEventDestroy(q->last);
             q->last = nullptr;
+            free_operands(q);
             {
                 std::lock_guard<std::mutex> g(mu);
                 bytes_in_flight -= q->bytes;
@@ -321,6 +314,29 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
         return gnt::set_error(GNT_E_NOMEM, "request exceeds the HBM admission cap");
     }
     q->t_submit = Clock::now();
+    // Upload host operands now, on the copy stream, so PCIe transfers overlap
+    // the scheduler's kernels.  They count against the cap once admitted.
+    {
+        std::lock_guard<std::mutex> ug(s->upload_mu);
+        int rc = GNT_OK;
+        if (!req->a_buf) {
+            q->own_a = (u64 *)c->upload_pool.acquire(p.na * 8);
+            if (!q->own_a || hipMemcpyAsync(q->own_a, req->a, p.na * 8, hipMemcpyHostToDevice, c->copy_stream))
+                rc = GNT_E_DEVICE;
+        }
+        if (!rc && !sq && !req->b_buf) {
+            q->own_b = (u64 *)c->upload_pool.acquire(p.nb * 8);
+            if (!q->own_b || hipMemcpyAsync(q->own_b, req->b, p.nb * 8, hipMemcpyHostToDevice, c->copy_stream))
+                rc = GNT_E_DEVICE;
+        }
+        if (!rc && hipStreamSynchronize(c->copy_stream) != hipSuccess) rc = GNT_E_DEVICE;
+        if (rc) {
+            c->upload_pool.release(q->own_a, p.na * 8);
+            c->upload_pool.release(q->own_b, p.nb * 8);
+            delete q;
+            return gnt::set_error(rc, "operand upload failed");
+        }
+    }
     {
         std::lock_guard<std::mutex> g(s->mu);
         q->ticket = s->next_ticket++;
