@@ -125,8 +125,15 @@ int gnt_batch_destroy(gnt_batch *b);
  * gnt_config.hbm_cap_bytes.  Host operands must stay valid until the request
  * completes.  An operand may instead be a gnt_buf (e.g. an earlier result).
  * No reference counterpart: the reference has no serving mode. */
+enum {
+    GNT_OP_MUL = 0, /* a * b */
+    GNT_OP_SQR = 1, /* a * a (b ignored) */
+    GNT_OP_ADD = 2, /* a + b */
+    GNT_OP_SUB = 3  /* a - b; requires a >= b (the result is a - b mod 2^(64*na) otherwise) */
+};
 typedef struct gnt_req {
-    int op;             /* 0 multiply, 1 square (b ignored) */
+    int op;             /* GNT_OP_*; ADD/SUB serve binary splitting's t = t_l*q_r +- p_l*t_r
+                           (chudnovsky.rs:411) without a host round trip */
     const uint64_t *a;  /* host operand, or NULL when a_buf is set */
     size_t na;
     const uint64_t *b;
@@ -139,7 +146,8 @@ int gnt_serve_stop(gnt_ctx *ctx);   /* drains in-flight requests */
 int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket);
 int gnt_poll(gnt_ctx *ctx, uint64_t ticket, int *done);
 /* Wait for a request (timeout_us < 0: forever).  On success *result is a new
- * device buffer owned by the caller; *latency_us = submit-to-completion. */
+ * device buffer owned by the caller, its trimmed length already known (no
+ * device round trip in gnt_buf_len); *latency_us = submit-to-completion. */
 int gnt_wait(gnt_ctx *ctx, uint64_t ticket, int64_t timeout_us, gnt_buf **result, double *latency_us);
 /* [0] iterations [1] launches [2] completed [3] peak admitted [4] peak bytes in flight [5] bytes in flight */
 int gnt_serve_stats(gnt_ctx *ctx, uint64_t *stats, int n);

@@ -57,7 +57,22 @@ def build(verbose: bool = False) -> str:
     calib = os.path.join(HERE, "gnt_calib")
     subprocess.run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "gnt_calib.hip"), "-o", calib + ".tmp"], check=True)
     os.replace(calib + ".tmp", calib)
+    build_pi_driver()
     return LIB
+
+
+GMP = "/usr/lib/x86_64-linux-gnu/libgmp.so.10"
+PI_BIN = os.path.join(HERE, "pi_chudnovsky")
+
+
+def build_pi_driver() -> str:
+    """Host driver of the Chudnovsky run (binary splitting over libgnt, GMP for host nodes)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-pthread", os.path.join(CSRC, "pi_chudnovsky.cpp"),
+           "-o", PI_BIN + ".tmp", "-L" + HERE, "-lgnt", GMP, "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    subprocess.run(cmd, check=True)
+    os.replace(PI_BIN + ".tmp", PI_BIN)
+    return PI_BIN
 
 
 if __name__ == "__main__":

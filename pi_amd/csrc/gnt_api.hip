@@ -63,8 +63,20 @@ int set_error(int code, const char *msg) {
 // Buffers are handed back in stream order, so reuse by later work on the same
 // stream is safe without a sync.
 // ---------------------------------------------------------------------------
-void *Pool::acquire(size_t bytes) {
+// Size classes: 256-byte granules up to 64 KiB, then eight classes per
+// octave (at most 12.5% slack), so products of uneven sizes reuse each
+// other's buffers instead of going back to hipMalloc/hipFree (hipFree waits
+// for the device).
+size_t Pool::size_class(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes <= ((size_t)1 << 16)) return bytes;
+    int top = 63 - __builtin_clzll(bytes);
+    size_t step = (size_t)1 << (top - 3);
+    return (bytes + step - 1) & ~(step - 1);
+}
+
+void *Pool::acquire(size_t bytes) {
+    bytes = size_class(bytes);
     {
         std::lock_guard<std::mutex> g(mu);
         auto it = free_.find(bytes);
@@ -88,11 +100,11 @@ void *Pool::acquire(size_t bytes) {
 
 void Pool::release(void *p, size_t bytes) {
     if (!p) return;
-    bytes = (bytes + 255) & ~(size_t)255;
+    bytes = size_class(bytes);
     std::lock_guard<std::mutex> g(mu);
     auto &v = free_[bytes];
-    size_t cap = bytes >= ((size_t)1 << 30) ? 1 : 4;
-    if (v.size() < cap) {
+    // idle buffers are kept while the idle total stays under POOL_IDLE_MAX
+    if (pooled_bytes + bytes <= POOL_IDLE_MAX && v.size() < 4096) {
         v.push_back(p);
         pooled_bytes += bytes;
     } else {
@@ -213,13 +225,17 @@ struct gnt_buf {
 };
 
 namespace gnt {
-gnt_buf *buf_wrap(Ctx *c, u64 *d, size_t cap, u32 *dlen) {
+gnt_buf *buf_wrap(Ctx *c, u64 *d, size_t cap, u32 *dlen, int64_t len) {
     (void)c;
     gnt_buf *b = new gnt_buf();
     b->d = d;
     b->cap = cap;
     b->dlen = dlen;
     b->complete = true;  // handed out only after its product finished
+    if (len >= 0) {
+        b->len = (size_t)len;
+        b->len_known = true;
+    }
     return b;
 }
 bool buf_info(gnt_buf *b, u64 **d, size_t *cap) {

@@ -61,17 +61,41 @@ __global__ __launch_bounds__(BLOCK) void k_ctiles(const CItem *__restrict__ item
         const size_t j0 = (size_t)k * CARRY_TILE + (size_t)threadIdx.x * CARRY_LPT;
         uint64_t y[CARRY_LPT];
         uint32_t gbits = 0, pbits = 0;
-        uint64_t Hprev = (j0 > 0 && j0 - 1 < it.n_limbs) ? it.Hs[j0 - 1] : 0;
+        if (it.mode == C_SUMS) {
+            uint64_t Hprev = (j0 > 0 && j0 - 1 < it.n_limbs) ? it.Hs[j0 - 1] : 0;
 #pragma unroll
-        for (int q = 0; q < CARRY_LPT; q++) {
-            size_t j = j0 + q;
-            uint64_t L = 0, H = 0;
-            if (j < it.n_limbs) { L = it.Ls[j]; H = it.Hs[j]; }
-            uint64_t s = L + Hprev;
-            gbits |= (uint32_t)(s < L) << q;
-            pbits |= (uint32_t)(s == ~0ull) << q;
-            y[q] = s;
-            Hprev = H;
+            for (int q = 0; q < CARRY_LPT; q++) {
+                size_t j = j0 + q;
+                uint64_t L = 0, H = 0;
+                if (j < it.n_limbs) { L = it.Ls[j]; H = it.Hs[j]; }
+                uint64_t s = L + Hprev;
+                gbits |= (uint32_t)(s < L) << q;
+                pbits |= (uint32_t)(s == ~0ull) << q;
+                y[q] = s;
+                Hprev = H;
+            }
+        } else {
+            const bool sub = it.mode == C_SUB;
+#pragma unroll
+            for (int q = 0; q < CARRY_LPT; q++) {
+                size_t j = j0 + q;
+                uint64_t s = 0;
+                uint32_t g = 0;
+                if (j < it.n_limbs) {
+                    uint64_t a = j < it.na ? it.Ls[j] : 0;
+                    uint64_t b = j < it.nb ? it.Hs[j] : 0;
+                    if (sub) b = ~b;  // zero-extended, then complemented
+                    s = a + b;
+                    g = s < a;
+                    if (sub && j == 0) {  // the +1 of the two's complement
+                        s += 1;
+                        g |= s == 0;
+                    }
+                }
+                gbits |= g << q;
+                pbits |= (uint32_t)(s == ~0ull) << q;
+                y[q] = s;
+            }
         }
         uint32_t agg = 2;
 #pragma unroll
@@ -117,14 +141,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_cscan(const CItem *__restrict__ 
     const size_t per = (nt + SCAN_BLOCK - 1) / SCAN_BLOCK;
     const size_t t0 = (size_t)threadIdx.x * per;
     uint32_t agg = 2, mh = 0;
-    for (size_t t = t0; t < t0 + per && t < nt; t++) {
-        agg = gp_combine(agg, it.gp[t]);
-        mh = max(mh, it.hi[t]);
-    }
+    for (size_t t = t0; t < t0 + per && t < nt; t++) agg = gp_combine(agg, it.gp[t]);
     scan[threadIdx.x] = agg;
     if (threadIdx.x == 0) maxhi = 0;
     __syncthreads();
-    if (mh) atomicMax(&maxhi, mh);
     for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
         uint32_t v = scan[threadIdx.x];
         uint32_t w = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 2u;
@@ -136,8 +156,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_cscan(const CItem *__restrict__ 
     for (size_t t = t0; t < t0 + per && t < nt; t++) {
         it.cin[t] = (unsigned char)carry;
         uint32_t gp = it.gp[t];
+        // a tile of all-ones limbs that receives a carry becomes zeros (only
+        // at the top of a difference, where the carry leaves the result)
+        if (!(carry && (gp >> 1))) mh = max(mh, it.hi[t]);
         carry = (gp & 1) | ((gp >> 1) & carry);
     }
+    if (mh) atomicMax(&maxhi, mh);
+    __syncthreads();
     if (threadIdx.x == 0) *it.out_len = maxhi;
 }
 
@@ -187,10 +212,19 @@ __global__ __launch_bounds__(BLOCK) void k_cripple(const CItem *__restrict__ ite
 size_t Prod::cs_bytes() const { return (2 * nt + 15) / 16 * 16 + 4 * nt; }
 
 size_t Prod::scratch_bytes() const {
+    if (linear()) return cs_bytes();
     return N * 8 * (sqr() ? 1 : 2) + 2 * nl * 8 + cs_bytes();
 }
 
 bool prod_init(Prod &p) {
+    p.stage = 0;
+    if (p.linear()) {
+        p.nl = p.op == OP_ADD ? std::max(p.na, p.nb) + 1 : p.na;
+        p.N = p.n_out = 0;
+        p.nt = (p.nl + CARRY_TILE - 1) / CARRY_TILE;
+        p.nstages = 3;
+        return p.nl <= 0xFFFFFFFFull;
+    }
     if (p.sqr()) p.nb = p.na;
     p.nl = p.na + p.nb;
     p.n_out = 4 * p.nl - 1;
@@ -224,6 +258,24 @@ void add_stage(Iteration &it, Prod &p) {
     uint64_t *Ls = p.lh, *Hs = p.lh + p.nl;
     // forward passes before the fused one
     const int nfwd = sq ? P - 1 : P;  // stages that hold forward passes
+    if (p.linear()) {
+        CItem c{};
+        c.Ls = p.da;
+        c.Hs = p.db;
+        c.na = p.na;
+        c.nb = p.nb;
+        c.mode = p.op == OP_ADD ? C_ADD : C_SUB;
+        c.out = p.dout;
+        c.gp = p.cs;
+        c.cin = p.cs + p.nt;
+        c.hi = (uint32_t *)(p.cs + (2 * p.nt + 15) / 16 * 16);
+        c.out_len = p.dlen;
+        c.n_limbs = p.nl;
+        c.ntiles = p.nt;
+        c.tile0 = (uint64_t)s;  // the carry step, resolved in finalize_iteration
+        it.citems.push_back(c);
+        return;
+    }
     if (s < nfwd) {
         TItem a = titem(p, s, p.xa, s == 0 ? p.da : nullptr, p.na);
         a.pad = K_FWD;

@@ -40,7 +40,13 @@ struct TItem {
     int pad;
 };
 
-// One product's carry (device descriptor).
+// Carry modes: limb sums of a product, or the sum / difference of two numbers.
+enum CarryMode : int { C_SUMS = 0, C_ADD = 1, C_SUB = 2 };
+
+// One product's carry (device descriptor).  C_SUMS: limb j of the result is
+// Ls[j] + Hs[j-1] plus carries.  C_ADD / C_SUB: Ls and Hs are the operands
+// (na and nb limbs); C_SUB computes Ls - Hs over n_limbs = na limbs as
+// Ls + ~Hs + 1, which needs Ls >= Hs.
 struct CItem {
     const uint64_t *Ls, *Hs;
     uint64_t *out;
@@ -50,9 +56,14 @@ struct CItem {
     uint64_t n_limbs;
     uint64_t ntiles;
     uint64_t tile0;
+    uint64_t na, nb;  // C_ADD / C_SUB operand lengths
+    int mode;
+    int pad;
 };
 
-// A product in flight (host side).
+enum ProdOp : int { OP_MUL = 0, OP_SQR = 1, OP_ADD = 2, OP_SUB = 3 };
+
+// A product (or a sum / difference) in flight (host side).
 struct Prod {
     const uint64_t *da = nullptr, *db = nullptr;  // device limbs (db null: square)
     size_t na = 0, nb = 0;
@@ -64,7 +75,9 @@ struct Prod {
     unsigned char *cs = nullptr;  // carry scratch: gp[nt] cin[nt] pad, hi[nt] u32
     int stage = 0;
     int nstages = 0;
+    int op = OP_MUL;  // OP_ADD / OP_SUB: carry stages only, no transform
     bool sqr() const { return db == nullptr; }
+    bool linear() const { return op == OP_ADD || op == OP_SUB; }
     size_t scratch_bytes() const;            // xa + xb + lh + cs
     size_t cs_bytes() const;
 };

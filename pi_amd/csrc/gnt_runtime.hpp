@@ -16,8 +16,12 @@ namespace gnt {
 
 int set_error(int code, const char *msg);
 
+// Idle device memory a pool keeps for reuse.
+constexpr size_t POOL_IDLE_MAX = (size_t)32 << 30;
+
 class Pool {
 public:
+    static size_t size_class(size_t bytes);
     void *acquire(size_t bytes);
     void release(void *p, size_t bytes);
     void trim();

@@ -31,7 +31,7 @@ using gnt::Ctx;
 // gnt_buf is defined by the runtime; the serving loop only builds new ones
 // through these helpers (gnt_api.hip).
 namespace gnt {
-gnt_buf *buf_wrap(Ctx *c, u64 *d, size_t cap, uint32_t *dlen);
+gnt_buf *buf_wrap(Ctx *c, u64 *d, size_t cap, uint32_t *dlen, int64_t len);
 bool buf_info(gnt_buf *b, u64 **d, size_t *cap);
 }
 
@@ -49,6 +49,8 @@ struct Req {
     bool done = false;
     int status = GNT_OK;
     hipEvent_t last = nullptr;               // recorded after the product's last stage
+    int len_slot = -1;                       // pinned word the trimmed length is copied to
+    int64_t len = -1;                        // trimmed result length once known
 };
 
 struct Slot {  // descriptor staging for one iteration in flight
@@ -79,8 +81,18 @@ struct Server {
     static constexpr int NSLOT = 2;
     Slot slots[NSLOT];
     int slot_next = 0;
+    // Pinned words that receive each finished product's trimmed length on the
+    // compute stream, so results are handed out with their length known.
+    static constexpr int NLEN = 1 << 16;
+    uint32_t *lens = nullptr;
+    std::vector<int> lens_free;
 
-    explicit Server(Ctx *ctx) : c(ctx) {}
+    explicit Server(Ctx *ctx) : c(ctx) {
+        if (hipHostMalloc((void **)&lens, NLEN * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+            lens = nullptr;
+        if (lens)
+            for (int i = NLEN - 1; i >= 0; i--) lens_free.push_back(i);
+    }
 
     size_t footprint(Req *q) {
         gnt::Prod &p = q->p;
@@ -91,7 +103,7 @@ struct Server {
     // were uploaded at submission (copy stream), so only scratch is allocated.
     int admit(Req *q) {
         gnt::Prod &p = q->p;
-        const bool sq = q->r.op == 1;
+        const bool sq = q->r.op == GNT_OP_SQR;
         u64 *d;
         size_t cap;
         if (q->r.a_buf) { buf_info(q->r.a_buf, &d, &cap); p.da = d; }
@@ -101,6 +113,10 @@ struct Server {
         else p.db = q->own_b;
         p.dout = (u64 *)c->pool.acquire(p.nl * 8);
         p.dlen = (uint32_t *)c->pool.acquire(4);
+        if (p.linear()) {
+            p.cs = (unsigned char *)c->pool.acquire(p.cs_bytes());
+            return (p.dout && p.dlen && p.cs) ? GNT_OK : GNT_E_NOMEM;
+        }
         p.xa = (u64 *)c->pool.acquire(p.N * 8);
         p.xb = sq ? nullptr : (u64 *)c->pool.acquire(p.N * 8);
         p.lh = (u64 *)c->pool.acquire(p.nl * 16);
@@ -112,7 +128,7 @@ struct Server {
     void free_scratch(Req *q) {
         gnt::Prod &p = q->p;
         c->pool.release(p.xa, p.N * 8);
-        c->pool.release(p.xb, p.sqr() ? 0 : p.N * 8);
+        c->pool.release(p.xb, p.N * 8);
         c->pool.release(p.lh, p.nl * 16);
         c->pool.release(p.cs, p.cs_bytes());
         p.xa = p.xb = p.lh = nullptr;
@@ -212,6 +228,12 @@ struct Server {
         std::vector<Req *> still;
         for (Req *q : active) {
             if (q->p.stage >= q->p.nstages) {
+                if (!lens_free.empty()) {
+                    q->len_slot = lens_free.back();
+                    lens_free.pop_back();
+                    (void)hipMemcpyAsync(lens + q->len_slot, q->p.dlen, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                         c->stream);
+                }
                 (void)hipEventCreateWithFlags(&q->last, hipEventDisableTiming);
                 (void)hipEventRecord(q->last, c->stream);
                 free_scratch(q);  // later work is stream-ordered after this product
@@ -235,6 +257,11 @@ struct Server {
             free_operands(q);
             {
                 std::lock_guard<std::mutex> g(mu);
+                if (q->len_slot >= 0) {
+                    q->len = lens[q->len_slot];
+                    lens_free.push_back(q->len_slot);
+                    q->len_slot = -1;
+                }
                 bytes_in_flight -= q->bytes;
                 q->t_done = Clock::now();
                 q->done = true;
@@ -278,6 +305,7 @@ int gnt_serve_stop(gnt_ctx *ctx) {
         if (slot.dev) (void)hipFree(slot.dev);
     }
     for (auto &kv : s->tickets) delete kv.second;  // unclaimed results stay owned by their gnt_buf
+    if (s->lens) (void)hipHostFree(s->lens);
     delete s;
     c->server = nullptr;
     return GNT_OK;
@@ -291,7 +319,12 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
     Req *q = new Req();
     q->r = *req;
     gnt::Prod &p = q->p;
-    const bool sq = req->op == 1;
+    if (req->op < GNT_OP_MUL || req->op > GNT_OP_SUB) {
+        delete q;
+        return gnt::set_error(GNT_E_ARG, "unknown request op");
+    }
+    const bool sq = req->op == GNT_OP_SQR;
+    p.op = req->op;
     size_t cap;
     u64 *d;
     p.na = req->a_buf ? (gnt::buf_info(req->a_buf, &d, &cap), cap) : req->na;
@@ -299,6 +332,10 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
     if (p.na == 0 || p.nb == 0 || (!req->a_buf && !req->a) || (!sq && !req->b_buf && !req->b)) {
         delete q;
         return gnt::set_error(GNT_E_ARG, "empty or null operand");
+    }
+    if (req->op == GNT_OP_SUB && p.nb > p.na) {
+        delete q;
+        return gnt::set_error(GNT_E_ARG, "difference needs a >= b (b longer than a)");
     }
     p.db = sq ? nullptr : (const u64 *)1;  // shape only; real pointers at admission
     if (!gnt::prod_init(p)) {
@@ -376,7 +413,7 @@ int gnt_wait(gnt_ctx *ctx, uint64_t ticket, int64_t timeout_us, gnt_buf **result
     s->tickets.erase(it);
     const int st = q->status;
     if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(q->t_done - q->t_submit).count();
-    if (st == GNT_OK) *result = gnt::buf_wrap(c, q->p.dout, q->p.nl, q->p.dlen);
+    if (st == GNT_OK) *result = gnt::buf_wrap(c, q->p.dout, q->p.nl, q->p.dlen, q->len);
     delete q;
     if (st != GNT_OK) return gnt::set_error(st, "request failed");
     return GNT_OK;

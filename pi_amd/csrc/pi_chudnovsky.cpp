@@ -1,0 +1,469 @@
+// pi to D decimal digits by the Chudnovsky series, binary splitting with the
+// tree's large merges on the GPU.
+//
+// Restates pi-core's Chudnovsky (crates/pi-core/src/algorithm/chudnovsky.rs):
+//   n_terms = ceil(D / 14.181647462725478) + 8                 (:50, :91)
+//   leaf k: p = -(6k-5)(2k-1)(6k-1), q = k^3 C^3/24, t = p (A + B k)   (:379-393)
+//   merge:  t = t_l q_r + p_l t_r, p = p_l p_r, q = q_l q_r   (:411-413)
+//   root:   p skipped                                          (:339-369)
+//   pi = 426880 sqrt(10005) Q / (A Q + T)                      (:146-211)
+// at prec = ceil(D log2 10) + 256 bits (precision.rs:19, :64), truncated to D
+// digits (util.rs:25-83) and written as "3." + (D-1) digits + "\n"
+// (output.rs:33-61).
+//
+// The split tree is the reference's (m = (a+b)/2).  Nodes whose smaller
+// operand is under a limb threshold run whole on host threads (GMP, as the
+// reference runs its small products on the CPU); every node above is merged
+// on the GPU, a tree level at a time, through the serving loop: all products
+// of a level are submitted together (continuous batching groups them into
+// shared launches), each t = x +- y is submitted as soon as its two products
+// finish, and the level's P, Q, T stay in HBM as the next level's operands.
+//
+// Sign: T(a,b) < 0 for every range (an alternating series whose terms shrink,
+// led by the negative k = a term), so the device holds |T| and
+//   |T| = |T_l| Q_r + (-1)^(m-a) |P_l| |T_r|,
+// a sum when the left half has an even number of terms and otherwise a
+// difference that is positive by the same argument.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gnt.h"
+#include "gmp_abi.hpp"
+
+namespace {
+
+constexpr double DIGITS_PER_TERM = 14.181647462725478;  // chudnovsky.rs:50
+constexpr uint64_t SAFETY_TERMS = 8;                    // chudnovsky.rs:56
+constexpr uint64_t SAFETY_BITS = 256;                   // precision.rs:19
+constexpr unsigned long CH_A = 13591409, CH_B = 545140134;
+constexpr unsigned long C3_OVER_24 = 10939058860032000ul;
+
+using Clock = std::chrono::steady_clock;
+double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+[[noreturn]] void die(const char *what, const char *detail = "") {
+    std::fprintf(stderr, "pi_chudnovsky: %s %s\n", what, detail);
+    std::exit(1);
+}
+
+void check(int rc, const char *what) {
+    if (rc != GNT_OK) die(what, gnt_last_error());
+}
+
+// Serial binary splitting of [a, b) on the host (chudnovsky.rs:377-417).
+void bs_host(uint64_t a, uint64_t b, gmp_mpz *P, gmp_mpz *Q, gmp_mpz *T, bool need_p) {
+    if (b - a == 1) {
+        __gmpz_set_ui(P, 6 * a - 5);
+        __gmpz_mul_ui(P, P, 2 * a - 1);
+        __gmpz_mul_ui(P, P, 6 * a - 1);
+        __gmpz_neg(P, P);
+        __gmpz_set_ui(Q, a);
+        __gmpz_mul_ui(Q, Q, a);
+        __gmpz_mul_ui(Q, Q, a);
+        __gmpz_mul_ui(Q, Q, C3_OVER_24);
+        __gmpz_mul_ui(T, P, CH_A + CH_B * a);
+        return;
+    }
+    const uint64_t m = (a + b) / 2;
+    bs_host(a, m, P, Q, T, true);
+    Mpz P2, Q2, T2;
+    bs_host(m, b, &P2, &Q2, &T2, need_p);
+    __gmpz_mul(T, T, &Q2);
+    __gmpz_mul(&T2, P, &T2);
+    __gmpz_add(T, T, &T2);
+    if (need_p) __gmpz_mul(P, P, &P2);
+    __gmpz_mul(Q, Q, &Q2);
+}
+
+// log2 |P(a,b)| estimate: sum of log2(72 k^3) over the range.
+double p_bits(uint64_t a, uint64_t b) {
+    return (double)(b - a) * std::log2(72.0) + 3.0 * (std::lgamma((double)b) - std::lgamma((double)a)) / std::log(2.0);
+}
+
+struct Node {
+    uint64_t a = 0, b = 0;
+    int depth = 0;
+    int left = -1, right = -1;  // children of a merged node
+    bool host = true;           // computed whole on the host
+    std::unique_ptr<Mpz> P, Q, T;
+    gnt_buf *bp = nullptr, *bq = nullptr, *bt = nullptr;
+};
+
+struct LevelStat {
+    int depth;
+    size_t nodes, products;
+    double limbs;  // operand limbs (na + nb) summed over the level's products
+    double secs;
+};
+
+struct Options {
+    uint64_t digits = 0;
+    std::string out;
+    double gpu_min_limbs = 2048;  // a node merges on the GPU when both children's P reach this
+    int threads = 0;
+    bool host_merge = false;      // merge the upper tree on the host too (comparison run)
+    int device = 0;
+};
+
+struct Driver {
+    Options o;
+    std::vector<Node> nodes;
+    std::vector<int> host_nodes;
+    int max_depth = 0;
+    gnt_ctx *ctx = nullptr;
+    std::vector<LevelStat> levels;
+
+    bool merges_on_device() const { return !o.host_merge; }
+
+    int build(uint64_t a, uint64_t b, int depth) {
+        const int id = (int)nodes.size();
+        nodes.emplace_back();
+        nodes[id].a = a;
+        nodes[id].b = b;
+        nodes[id].depth = depth;
+        max_depth = std::max(max_depth, depth);
+        const uint64_t m = (a + b) / 2;
+        bool split = b - a >= 2 && std::min(p_bits(a, m), p_bits(m, b)) / 64.0 >= o.gpu_min_limbs;
+        if (split) {
+            nodes[id].host = false;
+            int l = build(a, m, depth + 1);
+            int r = build(m, b, depth + 1);
+            nodes[id].left = l;
+            nodes[id].right = r;
+        } else {
+            host_nodes.push_back(id);
+        }
+        return id;
+    }
+
+    void run_host_nodes(int threads) {
+        std::vector<int> order = host_nodes;
+        std::sort(order.begin(), order.end(), [&](int x, int y) { return nodes[x].b - nodes[x].a > nodes[y].b - nodes[y].a; });
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (;;) {
+                size_t i = next.fetch_add(1);
+                if (i >= order.size()) return;
+                Node &n = nodes[order[i]];
+                n.P.reset(new Mpz);
+                n.Q.reset(new Mpz);
+                n.T.reset(new Mpz);
+                bs_host(n.a, n.b, &*n.P, &*n.Q, &*n.T, order[i] != 0);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; t++) pool.emplace_back(work);
+        for (auto &t : pool) t.join();
+    }
+
+    gnt_buf *upload(const Mpz &z) {
+        gnt_buf *b = nullptr;
+        check(gnt_buf_upload(ctx, z.data(), z.limbs(), &b), "upload");
+        return b;
+    }
+
+    static size_t buf_len(gnt_ctx *c, gnt_buf *b) {
+        size_t n = 0;
+        check(gnt_buf_len(c, b, &n), "length");
+        return n;
+    }
+
+    uint64_t submit(int op, gnt_buf *x, gnt_buf *y, double *limbs) {
+        gnt_req r{};
+        r.op = op;
+        r.a_buf = x;
+        r.b_buf = y;
+        uint64_t t = 0;
+        check(gnt_submit(ctx, &r, &t), "submit");
+        *limbs += (double)(buf_len(ctx, x) + buf_len(ctx, y));
+        return t;
+    }
+
+    gnt_buf *wait(uint64_t t) {
+        gnt_buf *b = nullptr;
+        check(gnt_wait(ctx, t, -1, &b, nullptr), "wait");
+        return b;
+    }
+
+    // Merge every device node at `depth` (children finished).
+    void merge_level_device(int depth) {
+        std::vector<int> ids;
+        for (int i = 0; i < (int)nodes.size(); i++)
+            if (!nodes[i].host && nodes[i].depth == depth) ids.push_back(i);
+        if (ids.empty()) return;
+        auto t0 = Clock::now();
+        struct Tk { uint64_t x, y, q, p, t; };
+        std::vector<Tk> tk(ids.size());
+        LevelStat st{depth, ids.size(), 0, 0.0, 0.0};
+        for (size_t i = 0; i < ids.size(); i++) {
+            Node &n = nodes[ids[i]];
+            Node &L = nodes[n.left], &R = nodes[n.right];
+            tk[i].x = submit(GNT_OP_MUL, L.bt, R.bq, &st.limbs);
+            tk[i].y = submit(GNT_OP_MUL, L.bp, R.bt, &st.limbs);
+            tk[i].q = submit(GNT_OP_MUL, L.bq, R.bq, &st.limbs);
+            tk[i].p = ids[i] != 0 ? submit(GNT_OP_MUL, L.bp, R.bp, &st.limbs) : 0;
+            st.products += ids[i] != 0 ? 4 : 3;
+        }
+        std::vector<gnt_buf *> xs(ids.size()), ys(ids.size());
+        for (size_t i = 0; i < ids.size(); i++) {
+            Node &n = nodes[ids[i]];
+            xs[i] = wait(tk[i].x);
+            ys[i] = wait(tk[i].y);
+            const uint64_t left_terms = nodes[n.left].b - nodes[n.left].a;
+            tk[i].t = submit(left_terms % 2 == 0 ? GNT_OP_ADD : GNT_OP_SUB, xs[i], ys[i], &st.limbs);
+        }
+        for (size_t i = 0; i < ids.size(); i++) {
+            Node &n = nodes[ids[i]];
+            n.bq = wait(tk[i].q);
+            if (ids[i] != 0) n.bp = wait(tk[i].p);
+            n.bt = wait(tk[i].t);
+            gnt_buf_release(ctx, xs[i]);
+            gnt_buf_release(ctx, ys[i]);
+            for (int c : {n.left, n.right}) {
+                Node &ch = nodes[c];
+                gnt_buf_release(ctx, ch.bp);
+                gnt_buf_release(ctx, ch.bq);
+                gnt_buf_release(ctx, ch.bt);
+                ch.bp = ch.bq = ch.bt = nullptr;
+            }
+        }
+        st.secs = since(t0);
+        levels.push_back(st);
+    }
+
+    // Host merge of the same level (comparison run).
+    void merge_level_host(int depth) {
+        std::vector<int> ids;
+        for (int i = 0; i < (int)nodes.size(); i++)
+            if (!nodes[i].host && nodes[i].depth == depth) ids.push_back(i);
+        if (ids.empty()) return;
+        auto t0 = Clock::now();
+        LevelStat st{depth, ids.size(), 0, 0.0, 0.0};
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (;;) {
+                size_t i = next.fetch_add(1);
+                if (i >= ids.size()) return;
+                Node &n = nodes[ids[i]];
+                Node &L = nodes[n.left], &R = nodes[n.right];
+                n.P.reset(new Mpz);
+                n.Q.reset(new Mpz);
+                n.T.reset(new Mpz);
+                Mpz y;
+                __gmpz_mul(&*n.T, &*L.T, &*R.Q);
+                __gmpz_mul(&y, &*L.P, &*R.T);
+                __gmpz_add(&*n.T, &*n.T, &y);
+                if (ids[i] != 0) __gmpz_mul(&*n.P, &*L.P, &*R.P);
+                __gmpz_mul(&*n.Q, &*L.Q, &*R.Q);
+                L.P.reset(); L.Q.reset(); L.T.reset();
+                R.P.reset(); R.Q.reset(); R.T.reset();
+            }
+        };
+        const int nt = std::max(1, std::min<int>(o.threads, (int)ids.size()));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; t++) pool.emplace_back(work);
+        for (auto &t : pool) t.join();
+        st.products = ids.size() * 4 - (ids[0] == 0 ? 1 : 0);
+        st.secs = since(t0);
+        levels.push_back(st);
+    }
+};
+
+// floor(pi * 10^(D-1)) as D decimal characters, from the root's Q and |T|.
+std::string final_assembly(const Mpz &Q, const Mpz &Tabs, uint64_t digits, uint64_t prec, double *t_fa,
+                           double *t_dec) {
+    auto t0 = Clock::now();
+    // denom = A Q + T = A Q - |T|
+    Mpz den;
+    __gmpz_mul_ui(&den, &Q, CH_A);
+    __gmpz_sub(&den, &den, &Tabs);
+    // Q / den needs only prec + 64 bits of each
+    Mpz q;
+    __gmpz_set(&q, &Q);
+    const size_t dbits = __gmpz_sizeinbase(&den, 2);
+    if (dbits > prec + 64) {
+        const unsigned long s = (unsigned long)(dbits - (prec + 64));
+        __gmpz_tdiv_q_2exp(&q, &q, s);
+        __gmpz_tdiv_q_2exp(&den, &den, s);
+    }
+    // S = floor(sqrt(10005) 2^prec)
+    Mpz S;
+    __gmpz_set_ui(&S, 10005);
+    __gmpz_mul_2exp(&S, &S, 2 * prec);
+    __gmpz_sqrt(&S, &S);
+    // pi_fix = floor(426880 S q / den), within 2 units below pi 2^prec
+    Mpz pifix;
+    __gmpz_mul(&pifix, &S, &q);
+    __gmpz_mul_ui(&pifix, &pifix, 426880);
+    __gmpz_tdiv_q(&pifix, &pifix, &den);
+    *t_fa = since(t0);
+    t0 = Clock::now();
+    // floor(pi 10^(D-1)): scale, then drop prec bits; the dropped bits must
+    // stay 2*10^(D-1) away from the next integer (the error bound above)
+    Mpz p10, scaled, intpart, frac, bound, top;
+    __gmpz_ui_pow_ui(&p10, 10, (unsigned long)(digits - 1));
+    __gmpz_mul(&scaled, &pifix, &p10);
+    __gmpz_tdiv_q_2exp(&intpart, &scaled, prec);
+    __gmpz_tdiv_r_2exp(&frac, &scaled, prec);
+    __gmpz_mul_ui(&bound, &p10, 2);
+    __gmpz_set_ui(&top, 1);
+    __gmpz_mul_2exp(&top, &top, prec);
+    __gmpz_sub(&top, &top, &bound);
+    if (__gmpz_cmp(&frac, &top) >= 0) die("truncation undecided at this precision", "");
+    char *s = __gmpz_get_str(nullptr, 10, &intpart);
+    std::string out(s);
+    std::free(s);
+    *t_dec = since(t0);
+    if (out.size() != digits) die("unexpected digit count", "");
+    return out;
+}
+
+void usage() {
+    std::fprintf(stderr,
+                 "usage: pi_chudnovsky DIGITS OUT_FILE [--gpu-min-limbs L] [--threads T] [--host-merge] [--device D]\n");
+    std::exit(2);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc < 3) usage();
+    Options o;
+    o.digits = std::strtoull(argv[1], nullptr, 10);
+    o.out = argv[2];
+    for (int i = 3; i < argc; i++) {
+        std::string a = argv[i];
+        if (a == "--gpu-min-limbs" && i + 1 < argc) o.gpu_min_limbs = std::atof(argv[++i]);
+        else if (a == "--threads" && i + 1 < argc) o.threads = std::atoi(argv[++i]);
+        else if (a == "--host-merge") o.host_merge = true;
+        else if (a == "--device" && i + 1 < argc) o.device = std::atoi(argv[++i]);
+        else usage();
+    }
+    if (o.threads <= 0) {
+        unsigned hc = std::thread::hardware_concurrency();
+        o.threads = (int)std::min(16u, hc ? hc : 1u);
+    }
+    auto t_start = Clock::now();
+    FILE *f = std::fopen(o.out.c_str(), "wb");
+    if (!f) die("cannot open", o.out.c_str());
+    if (o.digits == 0) {  // chudnovsky.rs:82-85: only the sink's finish
+        std::fputc('\n', f);
+        std::fclose(f);
+        std::printf("{\"digits\": 0}\n");
+        return 0;
+    }
+    const uint64_t n_terms = (uint64_t)std::ceil((double)o.digits / DIGITS_PER_TERM) + SAFETY_TERMS;
+    const uint64_t prec = (uint64_t)std::ceil((double)o.digits * std::log2(10.0)) + SAFETY_BITS;
+
+    Driver d;
+    d.o = o;
+    d.nodes.reserve(1024);
+    d.build(1, n_terms + 1, 0);
+    const bool any_merge = !d.nodes[0].host;
+
+    double t_init = 0, t_upload = 0, t_download = 0;
+    if (any_merge && d.merges_on_device()) {
+        auto t0 = Clock::now();
+        gnt_config cfg{};
+        cfg.device = o.device;
+        check(gnt_init(&d.ctx, &cfg), "init");
+        check(gnt_serve_start(d.ctx), "serve start");
+        t_init = since(t0);
+    }
+    auto t0 = Clock::now();
+    d.run_host_nodes(o.threads);
+    const double t_host = since(t0);
+
+    Mpz Q, Tabs;
+    double t_merge = 0;
+    if (!any_merge) {
+        __gmpz_set(&Q, &*d.nodes[0].Q);
+        __gmpz_neg(&Tabs, &*d.nodes[0].T);
+    } else if (!d.merges_on_device()) {
+        t0 = Clock::now();
+        for (int depth = d.max_depth; depth >= 0; depth--) d.merge_level_host(depth);
+        t_merge = since(t0);
+        __gmpz_set(&Q, &*d.nodes[0].Q);
+        __gmpz_neg(&Tabs, &*d.nodes[0].T);
+    } else {
+        t0 = Clock::now();
+        for (int id : d.host_nodes) {
+            Node &n = d.nodes[id];
+            if (n.T->sign() >= 0) die("T(a,b) expected negative", "");
+            n.bp = d.upload(*n.P);
+            n.bq = d.upload(*n.Q);
+            n.bt = d.upload(*n.T);
+            n.P.reset();
+            n.Q.reset();
+            n.T.reset();
+        }
+        t_upload = since(t0);
+        t0 = Clock::now();
+        for (int depth = d.max_depth; depth >= 0; depth--) d.merge_level_device(depth);
+        t_merge = since(t0);
+        t0 = Clock::now();
+        Node &root = d.nodes[0];
+        size_t nq = Driver::buf_len(d.ctx, root.bq), nt = Driver::buf_len(d.ctx, root.bt);
+        std::vector<uint64_t> hq(nq), ht(nt);
+        size_t got = 0;
+        check(gnt_buf_download(d.ctx, root.bq, hq.data(), nq, &got), "download");
+        check(gnt_buf_download(d.ctx, root.bt, ht.data(), nt, &got), "download");
+        Q.set_limbs(hq.data(), nq);
+        Tabs.set_limbs(ht.data(), nt);
+        gnt_buf_release(d.ctx, root.bq);
+        gnt_buf_release(d.ctx, root.bt);
+        t_download = since(t0);
+    }
+
+    double t_fa = 0, t_dec = 0;
+    std::string s = final_assembly(Q, Tabs, o.digits, prec, &t_fa, &t_dec);
+    t0 = Clock::now();
+    std::fwrite(s.data(), 1, 1, f);
+    if (o.digits > 1) {
+        std::fputc('.', f);
+        std::fwrite(s.data() + 1, 1, s.size() - 1, f);
+    }
+    std::fputc('\n', f);
+    if (std::fclose(f) != 0) die("write failed", o.out.c_str());
+    const double t_write = since(t0);
+    const double t_total = since(t_start);
+
+    uint64_t sstats[6] = {0, 0, 0, 0, 0, 0};
+    if (d.ctx) {
+        gnt_serve_stats(d.ctx, sstats, 6);
+        gnt_serve_stop(d.ctx);
+        gnt_shutdown(d.ctx);
+    }
+    double t_levels = 0;
+    std::string lv = "[";
+    for (size_t i = 0; i < d.levels.size(); i++) {
+        const LevelStat &L = d.levels[i];
+        t_levels += L.secs;
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "%s{\"depth\": %d, \"nodes\": %zu, \"products\": %zu, \"operand_limbs\": %.0f, \"s\": %.4f}",
+                      i ? ", " : "", L.depth, L.nodes, L.products, L.limbs, L.secs);
+        lv += buf;
+    }
+    lv += "]";
+    std::printf("{\"digits\": %llu, \"n_terms\": %llu, \"prec_bits\": %llu, \"merge\": \"%s\", \"threads\": %d, "
+                "\"gpu_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
+                "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
+                "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_total\": %.3f, "
+                "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"levels\": %s}\n",
+                (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
+                !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"), o.threads, o.gpu_min_limbs,
+                d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
+                t_download, t_fa, t_dec, t_write, t_total, (unsigned long long)sstats[0],
+                (unsigned long long)sstats[1], lv.c_str());
+    (void)t_levels;
+    return 0;
+}

@@ -303,10 +303,16 @@ class Server:
         self._keep = {}
         _check(load().gnt_serve_start(ctx.h))
 
+    OPS = {"mul": 0, "sqr": 1, "add": 2, "sub": 3}
+
     def submit(self, a=None, b=None, square: bool = False, a_buf: Optional[DeviceBuf] = None,
-               b_buf: Optional[DeviceBuf] = None) -> int:
+               b_buf: Optional[DeviceBuf] = None, op: Optional[str] = None) -> int:
+        """Queue a*b (or a*a with square=True); op="add"/"sub" queues a+b / a-b (a >= b)."""
         req = GntReq()
-        req.op = 1 if square else 0
+        if op is None:
+            op = "sqr" if square else "mul"
+        req.op = self.OPS[op]
+        square = op == "sqr"
         keep = []
         if a_buf is not None:
             req.a_buf = a_buf.handle

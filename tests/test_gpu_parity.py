@@ -357,3 +357,65 @@ def test_serving_admission_cap(orc):
     finally:
         srv.stop()
         c.close()
+
+
+# ---- serving loop: sums and differences (binary splitting's t = x +- y) ----
+
+def _rand_int(rng, n):
+    return int.from_bytes(rng.bytes(8 * n), "little") | (1 << (64 * n - 1))
+
+
+def _limbs(x):
+    from oracle import oracle
+    return oracle.from_int(x)
+
+
+def test_serving_add_sub(ctx, orc):
+    """a + b and a - b (a >= b) through the serving loop, exact against CPython
+    int, with carries and borrows that cross carry tiles (1024 limbs) and
+    differences whose top limbs cancel."""
+    from pi_amd import gnt
+    rng = np.random.default_rng(5)
+    ones = (1 << (64 * 5000)) - 1
+    X = _rand_int(rng, 6000)
+    cases = [
+        ("add", _rand_int(rng, 3000), _rand_int(rng, 3000)),
+        ("add", _rand_int(rng, 5000), _rand_int(rng, 17)),
+        ("add", _rand_int(rng, 3), _rand_int(rng, 4100)),
+        ("add", ones, 1),                                   # carry through 5000 limbs
+        ("add", ones, ones),
+        ("sub", _rand_int(rng, 5000), _rand_int(rng, 4999)),
+        ("sub", _rand_int(rng, 5000), _rand_int(rng, 12)),
+        ("sub", 1 << (64 * 4500), 1),                       # borrow through 4500 limbs
+        ("sub", X + 12345, X),                              # top 6000 limbs cancel
+        ("sub", X + (1 << (64 * 2500)), X),                 # cancellation above a carry tile
+        ("sub", X, X),                                      # zero
+        ("add", 5, 7),
+        ("sub", 7, 5),
+    ]
+    srv = gnt.Server(ctx)
+    try:
+        ts = [srv.submit(_limbs(a), _limbs(b), op=op) for op, a, b in cases]
+        for (op, a, b), t in zip(cases, ts):
+            buf, _ = srv.wait(t)
+            want = a + b if op == "add" else a - b
+            got = buf.download()
+            assert len(buf) == len(_limbs(want)) if want else len(buf) == 0
+            assert orc.to_int(got) == want, f"{op} {a.bit_length()} {b.bit_length()}"
+            buf.release()
+        # device-resident chain: (x*y) - (z*w) and (x*y) + (z*w) from product results
+        x, y, z, w = (_rand_int(rng, n) for n in (4000, 3900, 2000, 1500))
+        tp = srv.submit(_limbs(x), _limbs(y))
+        tq = srv.submit(_limbs(z), _limbs(w))
+        p, _ = srv.wait(tp)
+        q, _ = srv.wait(tq)
+        td = srv.submit(a_buf=p, b_buf=q, op="sub")
+        ts_ = srv.submit(a_buf=p, b_buf=q, op="add")
+        d, _ = srv.wait(td)
+        s, _ = srv.wait(ts_)
+        assert orc.to_int(d.download()) == x * y - z * w
+        assert orc.to_int(s.download()) == x * y + z * w
+        for b_ in (p, q, d, s):
+            b_.release()
+    finally:
+        srv.stop()

@@ -1,0 +1,52 @@
+"""End-to-end Chudnovsky on the GPU: the binary-splitting merges run through the
+serving loop (products, sums and differences on device-resident P, Q, T) and
+the digits must equal the pinned fixtures byte for byte.
+
+Fixtures (tests/golden/pi_digits.json, pi_10000.txt) come from an independent
+GMP program that agrees with a Gauss-Legendre run; the first 100 digits are
+the reference's FIRST_100 (crates/pi-core/src/algorithm/util.rs:126-127).
+"""
+import hashlib
+import json
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PI = os.path.join(ROOT, "pi_amd", "pi_chudnovsky")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _digest(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def _run(digits, tmp_path, *extra):
+    out = tmp_path / f"pi_{digits}.txt"
+    print(f"pi_chudnovsky {digits} {' '.join(extra)}", flush=True)
+    r = subprocess.run([PI, str(digits), str(out), *extra], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    stats = json.loads(r.stdout.strip().splitlines()[-1])
+    return out, stats
+
+
+@pytest.mark.parametrize("gpu_min_limbs", ["1", "8", "64"])
+def test_pi_10000_gpu_merges(tmp_path, gpu_min_limbs):
+    """Almost the whole tree merged on the device (tiny limb threshold)."""
+    out, st = _run(10000, tmp_path, "--gpu-min-limbs", gpu_min_limbs)
+    assert st["merge"] == "gpu" and st["merge_nodes"] > 0
+    with open(os.path.join(GOLD, "pi_10000.txt"), "rb") as f:
+        assert out.read_bytes() == f.read()
+
+
+@pytest.mark.parametrize("digits,gpu_min_limbs", [(100000, "32"), (1000000, "256"), (10000000, "1024")])
+def test_pi_sha256(tmp_path, digits, gpu_min_limbs):
+    gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
+    out, st = _run(digits, tmp_path, "--gpu-min-limbs", gpu_min_limbs)
+    assert st["merge"] == "gpu" and st["merge_nodes"] > 0
+    assert os.path.getsize(out) == gold[str(digits)]["bytes"]
+    assert _digest(out) == gold[str(digits)]["sha256"]

@@ -1,0 +1,52 @@
+"""The Chudnovsky driver's host logic (no GPU): the same split tree merged on
+host threads, final assembly and output format, against the pinned fixtures;
+and the fixtures themselves against the reference's FIRST_100
+(crates/pi-core/src/algorithm/util.rs:126-127)."""
+import hashlib
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PI = os.path.join(ROOT, "pi_amd", "pi_chudnovsky")
+GOLD = os.path.join(ROOT, "tests", "golden")
+FIRST_100 = ("3.141592653589793238462643383279502884197169399375105820974944592307816406286208998628034825342117067")
+
+
+def test_fixture_first_100():
+    data = open(os.path.join(GOLD, "pi_10000.txt"), "rb").read()
+    assert data.startswith(FIRST_100.encode())
+    assert len(data) == 10000 + 2 and data.endswith(b"\n")   # "3." + 9999 digits + "\n"
+    gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
+    assert hashlib.sha256(data).hexdigest() == gold["10000"]["sha256"]
+    # Feynman point: six 9s starting at decimal place 762
+    assert data[2 + 761:2 + 767] == b"999999"
+
+
+def _run(tmp_path, digits, *extra):
+    out = tmp_path / "pi.txt"
+    r = subprocess.run([PI, str(digits), str(out), *extra], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return out.read_bytes(), json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@pytest.mark.parametrize("digits", [0, 1, 2, 15, 100, 101, 1000])
+def test_small_digit_counts(tmp_path, digits):
+    data, _ = _run(tmp_path, digits)
+    ref = open(os.path.join(GOLD, "pi_10000.txt"), "rb").read()
+    if digits == 0:
+        assert data == b"\n"
+    elif digits == 1:
+        assert data == b"3\n"
+    else:
+        assert data == ref[:digits + 1] + b"\n"
+
+
+@pytest.mark.parametrize("digits,limbs", [(10000, "1"), (10000, "8"), (100000, "64")])
+def test_host_merged_tree(tmp_path, digits, limbs):
+    data, st = _run(tmp_path, digits, "--host-merge", "--gpu-min-limbs", limbs)
+    assert st["merge"] == "host" and st["merge_nodes"] > 0
+    gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
+    assert hashlib.sha256(data).hexdigest() == gold[str(digits)]["sha256"]
