@@ -152,6 +152,21 @@ int gnt_wait(gnt_ctx *ctx, uint64_t ticket, int64_t timeout_us, gnt_buf **result
 /* [0] iterations [1] launches [2] completed [3] peak admitted [4] peak bytes in flight [5] bytes in flight */
 int gnt_serve_stats(gnt_ctx *ctx, uint64_t *stats, int n);
 
+/* ---- transform plans and the autotuner (no reference counterpart: the
+ * reference's only transform knob is four_step_threshold, config.rs:81-89) ----
+ * A plan splits a length-2^log_n transform into passes of radix 2^log_r[p]:
+ * one pass up to 2^13, else 2..4 passes of radix 2^7..2^10.  Choices are
+ * process-wide and apply to every context; change them while no product runs. */
+/* Install a split for one length (npass 0 restores the built-in rule). */
+int gnt_set_plan(int log_n, int npass, const int *log_r);
+/* Load a plan cache written by gnt_tune (also done by gnt_init when
+ * gnt_config.plan_cache_path is set). */
+int gnt_load_plans(const char *path);
+/* Time every valid split of each length in [min_log_n, max_log_n] (4..28) on
+ * the device (median of reps products), install the fastest, and write the
+ * choices and timings to cache_path (JSON) when it is not NULL. */
+int gnt_tune(gnt_ctx *ctx, int min_log_n, int max_log_n, int reps, const char *cache_path);
+
 /* ---- introspection ---- */
 /* Transform plan the library uses for an na x nb product. */
 int gnt_plan_info(size_t na, size_t nb, int *log_n, int *npass, int *log_r /* [4] */);

@@ -48,6 +48,14 @@ struct Mpz {
     ~Mpz() { __gmpz_clear(&z); }
     Mpz(const Mpz &) = delete;
     Mpz &operator=(const Mpz &) = delete;
+    Mpz(Mpz &&o) noexcept {
+        __gmpz_init(&z);
+        __gmpz_swap(&z, &o.z);
+    }
+    Mpz &operator=(Mpz &&o) noexcept {
+        __gmpz_swap(&z, &o.z);
+        return *this;
+    }
     gmp_mpz *operator&() { return &z; }
     const gmp_mpz *operator&() const { return &z; }
     size_t limbs() const { return (size_t)(z.size < 0 ? -z.size : z.size); }

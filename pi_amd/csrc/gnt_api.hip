@@ -58,15 +58,13 @@ int set_error(int code, const char *msg) {
 }
 
 // ---------------------------------------------------------------------------
-// Device memory pool: exact-size free lists, at most 4 buffers per size and 1
-// per size >= 1 GiB (the reference's scratch pool policy, ntt.rs:51-155).
+// Device memory pool (the reference keeps a host scratch pool, ntt.rs:51-155).
 // Buffers are handed back in stream order, so reuse by later work on the same
-// stream is safe without a sync.
+// stream is safe without a sync.  Size classes: 256-byte granules up to
+// 64 KiB, then eight classes per octave (at most 12.5% slack), so products of
+// uneven sizes reuse each other's buffers instead of going back to
+// hipMalloc/hipFree (hipFree waits for the device).
 // ---------------------------------------------------------------------------
-// Size classes: 256-byte granules up to 64 KiB, then eight classes per
-// octave (at most 12.5% slack), so products of uneven sizes reuse each
-// other's buffers instead of going back to hipMalloc/hipFree (hipFree waits
-// for the device).
 size_t Pool::size_class(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     if (bytes <= ((size_t)1 << 16)) return bytes;
@@ -267,7 +265,10 @@ int gnt_init(gnt_ctx **out, const gnt_config *cfg) {
         return fail(GNT_E_DEVICE, "stream creation failed");
     }
     int rc = c->init_tables();
+    if (!rc && cfg && cfg->plan_cache_path && *cfg->plan_cache_path) rc = gnt_load_plans(cfg->plan_cache_path);
     if (rc) {
+        (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->copy_stream);
         delete c;
         return rc;
     }

@@ -70,7 +70,9 @@ __global__ __launch_bounds__(BLOCK) void k_ctiles(const CItem *__restrict__ item
                 if (j < it.n_limbs) { L = it.Ls[j]; H = it.Hs[j]; }
                 uint64_t s = L + Hprev;
                 gbits |= (uint32_t)(s < L) << q;
-                pbits |= (uint32_t)(s == ~0ull) << q;
+                // limbs past the end propagate, so a tile is all-ones exactly
+                // when its limbs inside the result are
+                pbits |= (uint32_t)(s == ~0ull || j >= it.n_limbs) << q;
                 y[q] = s;
                 Hprev = H;
             }
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(BLOCK) void k_ctiles(const CItem *__restrict__ item
                     }
                 }
                 gbits |= g << q;
-                pbits |= (uint32_t)(s == ~0ull) << q;
+                pbits |= (uint32_t)(s == ~0ull || j >= it.n_limbs) << q;
                 y[q] = s;
             }
         }

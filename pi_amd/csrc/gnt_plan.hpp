@@ -45,19 +45,18 @@ static inline int substep_log(int log_r, int i) {
     return base + (i < extra ? 1 : 0);
 }
 
-static inline Plan make_plan(int log_n) {
+// Fill pass geometry for the radix split lr[0..np) of a length-2^log_n
+// transform (earlier passes are strided, the last is contiguous).
+static inline Plan make_plan_split(int log_n, int np, const int *lr) {
     Plan pl;
     pl.log_n = log_n;
-    int np = log_n <= TILE_LOG ? 1 : (log_n <= 20 ? 2 : 3);
     pl.npass = np;
     int rem = log_n;
     for (int p = 0; p < np; p++) {
-        int left = np - p;
-        int lr = rem / left;             // earlier passes get the smaller share
-        pl.pass[p].log_r = lr;
+        pl.pass[p].log_r = lr[p];
         pl.pass[p].log_m = rem;
-        pl.pass[p].log_s = rem - lr;
-        rem -= lr;
+        pl.pass[p].log_s = rem - lr[p];
+        rem -= lr[p];
     }
     for (int p = 0; p < np; p++) {
         PassDesc &d = pl.pass[p];
@@ -69,6 +68,51 @@ static inline Plan make_plan(int log_n) {
         d.log_c = cols;
     }
     return pl;
+}
+
+// Built-in rule: one pass up to a tile, else 2 or 3 passes with the smaller
+// radices first (strided passes then load longer contiguous column runs).
+static inline void default_split(int log_n, int *np_out, int *lr) {
+    int np = log_n <= TILE_LOG ? 1 : (log_n <= 20 ? 2 : 3);
+    int rem = log_n;
+    for (int p = 0; p < np; p++) {
+        lr[p] = rem / (np - p);
+        rem -= lr[p];
+    }
+    *np_out = np;
+}
+
+// A split the compiled kernels cover: one pass of any radix up to a tile, or
+// 2..4 passes of radix 7..10 (the 2^TILE_LOG-element tile shapes).
+static inline bool split_valid(int log_n, int np, const int *lr) {
+    if (np < 1 || np > MAX_PASSES || log_n < 1 || log_n > MAX_LOG_N) return false;
+    int sum = 0;
+    for (int p = 0; p < np; p++) sum += lr[p];
+    if (sum != log_n) return false;
+    if (np == 1) return log_n <= TILE_LOG;
+    for (int p = 0; p < np; p++)
+        if (lr[p] < 7 || lr[p] > 10) return false;
+    return true;
+}
+
+// Per-length choices installed by the autotuner or a plan cache (host side;
+// npass 0 keeps the built-in rule).
+struct PlanChoice {
+    int npass;
+    int log_r[MAX_PASSES];
+};
+inline PlanChoice g_plan_choice[MAX_LOG_N + 1] = {};
+
+static inline Plan make_plan(int log_n) {
+    int lr[MAX_PASSES] = {0, 0, 0, 0};
+    int np = 0;
+    if (log_n >= 0 && log_n <= MAX_LOG_N && g_plan_choice[log_n].npass) {
+        np = g_plan_choice[log_n].npass;
+        for (int p = 0; p < np; p++) lr[p] = g_plan_choice[log_n].log_r[p];
+    } else {
+        default_split(log_n, &np, lr);
+    }
+    return make_plan_split(log_n, np, lr);
 }
 
 }  // namespace gnt

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <thread>
@@ -111,6 +112,8 @@ struct Options {
     double gpu_min_limbs = 2048;  // a node merges on the GPU when both children's P reach this
     int threads = 0;
     bool host_merge = false;      // merge the upper tree on the host too (comparison run)
+    bool fa_gmp = false;          // final assembly and radix conversion by GMP (comparison run)
+    double fa_min_limbs = 1024;   // final-assembly / conversion products on the GPU from this size
     int device = 0;
 };
 
@@ -278,8 +281,327 @@ struct Driver {
     }
 };
 
-// floor(pi * 10^(D-1)) as D decimal characters, from the root's Q and |T|.
-std::string final_assembly(const Mpz &Q, const Mpz &Tabs, uint64_t digits, uint64_t prec, double *t_fa,
+template <class F>
+void parallel_for(size_t n, int threads, F f) {
+    std::atomic<size_t> next{0};
+    auto w = [&] {
+        for (;;) {
+            size_t i = next.fetch_add(1);
+            if (i >= n) return;
+            f(i);
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 1; t < threads && (size_t)t < n; t++) ts.emplace_back(w);
+    w();
+    for (auto &t : ts) t.join();
+}
+
+// Products of the final assembly and the decimal conversion: on the GPU when
+// both operands reach min_limbs limbs (libgnt, host buffers), otherwise GMP.
+struct Mul {
+    gnt_ctx *ctx = nullptr;
+    double min_limbs = 1024;
+    int threads = 1;
+    size_t gpu_products = 0, host_products = 0;
+    double gpu_limbs = 0;
+
+    static size_t nl(const gmp_mpz *x) { return (size_t)(x->size < 0 ? -x->size : x->size); }
+    bool on_gpu(size_t na, size_t nb) const { return ctx && (double)std::min(na, nb) >= min_limbs; }
+
+    // r = a * b (r may alias a or b; a == b squares)
+    void mul(gmp_mpz *r, const gmp_mpz *a, const gmp_mpz *b) {
+        const size_t na = nl(a), nb = nl(b);
+        if (!on_gpu(na, nb)) {
+            __gmpz_mul(r, a, b);
+            host_products++;
+            return;
+        }
+        const bool neg = (a->size < 0) != (b->size < 0);
+        Mpz t;
+        __gmpz_realloc2(&t, (unsigned long)(na + nb) * 64);
+        size_t len = 0;
+        if (a == b) check(gnt_sqr(ctx, a->d, na, t.z.d, na + nb, &len), "square");
+        else check(gnt_mul(ctx, a->d, na, b->d, nb, t.z.d, na + nb, &len), "multiply");
+        t.z.size = neg ? -(int)len : (int)len;
+        __gmpz_swap(r, &t);
+        gpu_products++;
+        gpu_limbs += (double)(na + nb);
+    }
+
+    struct Job {
+        gmp_mpz *r;
+        const gmp_mpz *a, *b;  // nonnegative
+    };
+    // Independent products: the large ones in one batched GPU call while host
+    // threads do the small ones.
+    void batch(std::vector<Job> &jobs) {
+        std::vector<size_t> g, h;
+        for (size_t i = 0; i < jobs.size(); i++) (on_gpu(nl(jobs[i].a), nl(jobs[i].b)) ? g : h).push_back(i);
+        std::vector<gnt_job> gj(g.size());
+        for (size_t k = 0; k < g.size(); k++) {
+            Job &j = jobs[g[k]];
+            const size_t na = nl(j.a), nb = nl(j.b);
+            __gmpz_realloc2(j.r, (unsigned long)(na + nb) * 64);
+            gj[k] = gnt_job{j.a->d, na, j.b->d, nb, j.r->d, na + nb, 0, 0, 0};
+            gpu_limbs += (double)(na + nb);
+        }
+        std::thread host([&] { parallel_for(h.size(), std::max(1, threads - 1), [&](size_t k) {
+                                   __gmpz_mul(jobs[h[k]].r, jobs[h[k]].a, jobs[h[k]].b);
+                               }); });
+        if (!gj.empty()) check(gnt_mul_batch(ctx, gj.data(), gj.size()), "batched multiply");
+        host.join();
+        for (size_t k = 0; k < g.size(); k++) jobs[g[k]].r->size = (int)gj[k].out_len;
+        gpu_products += g.size();
+        host_products += h.size();
+    }
+};
+
+unsigned long bits_of(const gmp_mpz *x) { return x->size ? (unsigned long)__gmpz_sizeinbase(x, 2) : 0; }
+
+// r = x * 2^s (s may be negative: truncation toward zero)
+void shift(gmp_mpz *r, const gmp_mpz *x, long s) {
+    if (s >= 0) __gmpz_mul_2exp(r, x, (unsigned long)s);
+    else __gmpz_tdiv_q_2exp(r, x, (unsigned long)(-s));
+}
+
+// Newton precisions from k down to a start the hardware double gives exactly
+// enough for (each step at most doubles, minus a margin for rounding).
+std::vector<unsigned long> newton_chain(unsigned long k) {
+    std::vector<unsigned long> ps;
+    for (unsigned long p = k;; p = p / 2 + 6) {
+        ps.push_back(p);
+        if (p <= 48) break;
+    }
+    return ps;
+}
+
+// R ~ 2^(n+k) / D, n = bits(D): Newton r' = r + r(1 - d r) with precision
+// doubling (the reference's reciprocal_at_prec, float.rs:417-474, does the
+// same iteration on Floats).  Within a few units of the last place.
+void reciprocal(Mul &M, const gmp_mpz *D, unsigned long k, gmp_mpz *R) {
+    const unsigned long n = bits_of(D);
+    std::vector<unsigned long> ps = newton_chain(k);
+    unsigned long p = ps.back();
+    {
+        Mpz top;
+        shift(&top, D, 53 - (long)n);  // 53 leading bits
+        const double dt = (double)top.z.d[0];
+        const long double r0 = std::ldexp(1.0L / (long double)dt, (int)(p + 53));
+        __gmpz_set_ui(R, (unsigned long)r0);
+    }
+    const unsigned long g = 32;
+    Mpz dt, t, e, c;
+    for (size_t i = ps.size() - 1; i-- > 0;) {
+        const unsigned long q = ps[i];  // next precision
+        shift(&dt, D, (long)(q + g) - (long)n);
+        M.mul(&t, &dt, R);                              // ~ 2^(q+g+p)
+        __gmpz_set_ui(&e, 1);
+        __gmpz_mul_2exp(&e, &e, q + g + p);
+        __gmpz_sub(&e, &e, &t);                         // ~ err 2^(q+g), signed
+        const long eb = (long)bits_of(&e);
+        const long drop = std::max(0L, eb - (long)(q - p + 48));
+        shift(&e, &e, -drop);
+        M.mul(&c, R, &e);
+        shift(&c, &c, drop - (long)(2 * p + g));        // r e / 2^(2p+g)
+        shift(R, R, (long)(q - p));
+        __gmpz_add(R, R, &c);
+        p = q;
+    }
+}
+
+// Y ~ 2^k / sqrt(c) for a small integer c: Newton y' = y + y(1 - c y^2)/2.
+void inv_sqrt(Mul &M, unsigned long c, unsigned long k, gmp_mpz *Y) {
+    std::vector<unsigned long> ps = newton_chain(k);
+    unsigned long p = ps.back();
+    __gmpz_set_ui(Y, (unsigned long)std::ldexp(1.0L / std::sqrt((long double)c), (int)p));
+    Mpz s, e, corr;
+    for (size_t i = ps.size() - 1; i-- > 0;) {
+        const unsigned long q = ps[i];
+        M.mul(&s, Y, Y);                                // ~ 2^(2p) / c
+        __gmpz_mul_ui(&s, &s, c);
+        __gmpz_set_ui(&e, 1);
+        __gmpz_mul_2exp(&e, &e, 2 * p);
+        __gmpz_sub(&e, &e, &s);                         // ~ err 2^p, signed
+        const long eb = (long)bits_of(&e);
+        const long drop = std::max(0L, eb - (long)(q - p + 48));
+        shift(&e, &e, -drop);
+        M.mul(&corr, Y, &e);
+        shift(&corr, &corr, drop - (long)(3 * p + 1 - q));
+        shift(Y, Y, (long)(q - p));
+        __gmpz_add(Y, Y, &corr);
+        p = q;
+    }
+}
+
+// pi * 2^prec (within a few units), from the root's Q and |T|:
+// pi = 426880 sqrt(10005) Q / (A Q - |T|), every large product through M.
+void assemble_pi(Mul &M, const Mpz &Q, const Mpz &Tabs, unsigned long prec, gmp_mpz *pi_fix) {
+    const unsigned long w = prec + 64;  // working precision
+    Mpz den, q;
+    __gmpz_mul_ui(&den, &Q, CH_A);
+    __gmpz_sub(&den, &den, &Tabs);
+    __gmpz_set(&q, &Q);
+    const long s = (long)bits_of(&den) - (long)(w + 64);
+    if (s > 0) {  // Q / den needs w + 64 bits of each
+        shift(&den, &den, -s);
+        shift(&q, &q, -s);
+    }
+    const unsigned long n = bits_of(&den), k = w + 32;
+    Mpz R, Y, qf;
+    reciprocal(M, &den, k, &R);  // ~ 2^(n+k) / den
+    inv_sqrt(M, 10005, k, &Y);   // ~ 2^k / sqrt(10005)
+    M.mul(&qf, &q, &R);
+    shift(&qf, &qf, (long)w - (long)(n + k));  // ~ Q/den 2^w
+    __gmpz_mul_ui(&Y, &Y, 10005);              // ~ sqrt(10005) 2^k
+    M.mul(pi_fix, &Y, &qf);
+    __gmpz_mul_ui(pi_fix, pi_fix, 426880);
+    shift(pi_fix, pi_fix, -(long)(k + w - prec));
+}
+
+// Decimal digits of a binary fraction by a scaled remainder tree: the first h
+// of n digits of f come from f truncated, the last n - h from the fraction
+// part of f * 10^h; only multiplications, a tree level at a time (each level's
+// products batched on the GPU).  Every fraction carries an error bound in
+// units of its last place; a digit boundary within that bound stops the run
+// (the truncation would be undecided), as does the reference's exact
+// Round::Down (util.rs:48-56) never need to.
+struct Decimal {
+    Mul &M;
+    int threads;
+    size_t leaf_digits = 2048;
+    std::map<unsigned long, std::unique_ptr<Mpz>> pow10;
+
+    struct Frac {
+        Mpz f;               // value f / 2^F
+        unsigned long F = 0;
+        size_t n = 0;        // digits wanted
+        size_t at = 0;       // offset in the output
+        unsigned err = 0;    // error bound, units of 2^-F
+    };
+
+    explicit Decimal(Mul &m, int t) : M(m), threads(t) {}
+
+    const gmp_mpz *p10(unsigned long h) {
+        auto it = pow10.find(h);
+        if (it != pow10.end()) return &*it->second;
+        std::unique_ptr<Mpz> v(new Mpz);
+        if (h <= 19) {
+            __gmpz_ui_pow_ui(&*v, 10, h);
+        } else {
+            const gmp_mpz *half = p10(h / 2);
+            M.mul(&*v, half, half);
+            if (h & 1) __gmpz_mul_ui(&*v, &*v, 10);
+        }
+        const gmp_mpz *r = &*v;
+        pow10[h] = std::move(v);
+        return r;
+    }
+
+    // x in [E 10^m, 2^F - E 10^m]: the digits cannot move under the error E.
+    bool decided(const gmp_mpz *rem, unsigned long F, unsigned err, const gmp_mpz *p10m) {
+        Mpz b, top;
+        __gmpz_mul_ui(&b, p10m, err + 1);
+        if (__gmpz_cmp(rem, &b) < 0) return false;
+        __gmpz_set_ui(&top, 1);
+        __gmpz_mul_2exp(&top, &top, F);
+        __gmpz_sub(&top, &top, &b);
+        return __gmpz_cmp(rem, &top) <= 0;
+    }
+
+    void leaf(Frac &x, std::string &out) {
+        Mpz v, digits, rem;
+        const gmp_mpz *pw = p10(x.n);
+        __gmpz_mul(&v, &x.f, pw);
+        __gmpz_tdiv_q_2exp(&digits, &v, x.F);
+        __gmpz_tdiv_r_2exp(&rem, &v, x.F);
+        if (!decided(&rem, x.F, x.err, pw)) die("decimal conversion undecided at a leaf", "");
+        char *s = __gmpz_get_str(nullptr, 10, &digits);
+        const size_t len = std::strlen(s);
+        if (len > x.n) die("leaf overflow", "");
+        std::memset(&out[x.at], '0', x.n - len);
+        std::memcpy(&out[x.at + x.n - len], s, len);
+        std::free(s);
+    }
+
+    // n digits of f / 2^F into out[0..n)
+    void run(Frac root, std::string &out) {
+        std::vector<Frac> level;
+        level.push_back(std::move(root));
+        std::vector<Frac> leaves;
+        while (!level.empty()) {
+            std::vector<Frac> next;
+            std::vector<size_t> split;
+            for (size_t i = 0; i < level.size(); i++)
+                if (level[i].n <= leaf_digits) leaves.push_back(std::move(level[i]));
+                else split.push_back(i);
+            // powers first (sequential: they share squares), then the level's products
+            for (size_t i : split) p10(level[i].n / 2);
+            std::vector<Mpz> g(split.size());
+            std::vector<Mul::Job> jobs;
+            for (size_t k = 0; k < split.size(); k++)
+                jobs.push_back(Mul::Job{&g[k], &level[split[k]].f, p10(level[split[k]].n / 2)});
+            M.batch(jobs);
+            next.resize(2 * split.size());
+            parallel_for(split.size(), threads, [&](size_t k) {
+                Frac &x = level[split[k]];
+                const size_t h = x.n / 2, l = x.n - h;
+                const unsigned long bh = (unsigned long)std::ceil((double)h * std::log2(10.0));
+                const unsigned long bl = (unsigned long)std::ceil((double)l * std::log2(10.0));
+                Frac &hi = next[2 * k], &lo = next[2 * k + 1];
+                // high digits: f truncated by the bits the low digits need
+                hi.F = x.F - bl;
+                shift(&hi.f, &x.f, -(long)bl);
+                hi.n = h;
+                hi.at = x.at;
+                hi.err = x.err + 1;
+                // low digits: fraction part of f 10^h
+                Mpz rem;
+                __gmpz_tdiv_r_2exp(&rem, &g[k], x.F);
+                if (!decided(&rem, x.F, x.err, p10(h))) die("decimal conversion undecided", "");
+                lo.F = x.F - bh;
+                shift(&lo.f, &rem, -(long)bh);
+                lo.n = l;
+                lo.at = x.at + h;
+                lo.err = x.err + 1;
+            });
+            level = std::move(next);
+        }
+        parallel_for(leaves.size(), threads, [&](size_t i) { leaf(leaves[i], out); });
+    }
+};
+
+// floor(pi * 10^(D-1)) as D characters with every large product on the GPU.
+std::string pi_digits_gpu(Mul &M, const Mpz &Q, const Mpz &Tabs, uint64_t digits, uint64_t prec, int threads,
+                          double *t_fa, double *t_dec) {
+    auto t0 = Clock::now();
+    Mpz pi_fix;
+    assemble_pi(M, Q, Tabs, (unsigned long)prec, &pi_fix);
+    *t_fa = since(t0);
+    t0 = Clock::now();
+    Mpz ip;
+    __gmpz_tdiv_q_2exp(&ip, &pi_fix, prec);
+    if (ip.limbs() != 1 || ip.z.d[0] != 3) die("integer part is not 3", "");
+    std::string out(digits, '0');
+    out[0] = '3';
+    if (digits > 1) {
+        Decimal dec(M, threads);
+        Decimal::Frac root;
+        __gmpz_tdiv_r_2exp(&root.f, &pi_fix, prec);
+        root.F = prec;
+        root.n = digits - 1;
+        root.at = 1;
+        root.err = 4;
+        // pre-compute the powers of the tree's top levels while nothing else runs
+        dec.run(std::move(root), out);
+    }
+    *t_dec = since(t0);
+    return out;
+}
+
+// floor(pi * 10^(D-1)) with GMP's own square root, division and radix
+// conversion (comparison path).
+std::string final_assembly_gmp(const Mpz &Q, const Mpz &Tabs, uint64_t digits, uint64_t prec, double *t_fa,
                            double *t_dec) {
     auto t0 = Clock::now();
     // denom = A Q + T = A Q - |T|
@@ -329,7 +651,8 @@ std::string final_assembly(const Mpz &Q, const Mpz &Tabs, uint64_t digits, uint6
 
 void usage() {
     std::fprintf(stderr,
-                 "usage: pi_chudnovsky DIGITS OUT_FILE [--gpu-min-limbs L] [--threads T] [--host-merge] [--device D]\n");
+                 "usage: pi_chudnovsky DIGITS OUT_FILE [--gpu-min-limbs L] [--threads T] [--host-merge]\n"
+                 "                     [--fa-gmp] [--fa-min-limbs L] [--device D]\n");
     std::exit(2);
 }
 
@@ -345,6 +668,8 @@ int main(int argc, char **argv) {
         if (a == "--gpu-min-limbs" && i + 1 < argc) o.gpu_min_limbs = std::atof(argv[++i]);
         else if (a == "--threads" && i + 1 < argc) o.threads = std::atoi(argv[++i]);
         else if (a == "--host-merge") o.host_merge = true;
+        else if (a == "--fa-gmp") o.fa_gmp = true;
+        else if (a == "--fa-min-limbs" && i + 1 < argc) o.fa_min_limbs = std::atof(argv[++i]);
         else if (a == "--device" && i + 1 < argc) o.device = std::atoi(argv[++i]);
         else usage();
     }
@@ -371,12 +696,14 @@ int main(int argc, char **argv) {
     const bool any_merge = !d.nodes[0].host;
 
     double t_init = 0, t_upload = 0, t_download = 0;
-    if (any_merge && d.merges_on_device()) {
+    const bool gpu_merge = any_merge && d.merges_on_device();
+    const bool gpu_fa = !o.fa_gmp && (double)prec / 64.0 >= o.fa_min_limbs;
+    if (gpu_merge || gpu_fa) {
         auto t0 = Clock::now();
         gnt_config cfg{};
         cfg.device = o.device;
         check(gnt_init(&d.ctx, &cfg), "init");
-        check(gnt_serve_start(d.ctx), "serve start");
+        if (gpu_merge) check(gnt_serve_start(d.ctx), "serve start");
         t_init = since(t0);
     }
     auto t0 = Clock::now();
@@ -424,8 +751,18 @@ int main(int argc, char **argv) {
         t_download = since(t0);
     }
 
+    uint64_t sstats[6] = {0, 0, 0, 0, 0, 0};
+    if (gpu_merge) {
+        gnt_serve_stats(d.ctx, sstats, 6);
+        gnt_serve_stop(d.ctx);
+    }
     double t_fa = 0, t_dec = 0;
-    std::string s = final_assembly(Q, Tabs, o.digits, prec, &t_fa, &t_dec);
+    Mul M;
+    M.ctx = gpu_fa ? d.ctx : nullptr;
+    M.min_limbs = o.fa_min_limbs;
+    M.threads = o.threads;
+    std::string s = o.fa_gmp ? final_assembly_gmp(Q, Tabs, o.digits, prec, &t_fa, &t_dec)
+                             : pi_digits_gpu(M, Q, Tabs, o.digits, prec, o.threads, &t_fa, &t_dec);
     t0 = Clock::now();
     std::fwrite(s.data(), 1, 1, f);
     if (o.digits > 1) {
@@ -437,12 +774,7 @@ int main(int argc, char **argv) {
     const double t_write = since(t0);
     const double t_total = since(t_start);
 
-    uint64_t sstats[6] = {0, 0, 0, 0, 0, 0};
-    if (d.ctx) {
-        gnt_serve_stats(d.ctx, sstats, 6);
-        gnt_serve_stop(d.ctx);
-        gnt_shutdown(d.ctx);
-    }
+    if (d.ctx) gnt_shutdown(d.ctx);
     double t_levels = 0;
     std::string lv = "[";
     for (size_t i = 0; i < d.levels.size(); i++) {
@@ -458,12 +790,14 @@ int main(int argc, char **argv) {
                 "\"gpu_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
                 "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
                 "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_total\": %.3f, "
-                "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"levels\": %s}\n",
+                "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"fa\": \"%s\", \"fa_gpu_products\": %zu, "
+                "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"levels\": %s}\n",
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
                 !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"), o.threads, o.gpu_min_limbs,
                 d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
                 t_download, t_fa, t_dec, t_write, t_total, (unsigned long long)sstats[0],
-                (unsigned long long)sstats[1], lv.c_str());
+                (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
+                M.host_products, M.gpu_limbs, lv.c_str());
     (void)t_levels;
     return 0;
 }

@@ -36,7 +36,7 @@ EXPORTS = (
     "gnt_sync", "gnt_mul_batch", "gnt_mul_batch_serial", "gnt_plan_info", "gnt_set_timing", "gnt_get_timing",
     "gnt_batch_create", "gnt_batch_upload", "gnt_batch_run", "gnt_batch_download", "gnt_batch_info",
     "gnt_batch_destroy", "gnt_serve_start", "gnt_serve_stop", "gnt_submit", "gnt_poll", "gnt_wait",
-    "gnt_serve_stats",
+    "gnt_serve_stats", "gnt_set_plan", "gnt_load_plans", "gnt_tune",
 )
 
 
@@ -104,6 +104,9 @@ def load() -> ctypes.CDLL:
     L.gnt_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p),
                            ctypes.POINTER(ctypes.c_double)]
     L.gnt_serve_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
+    L.gnt_set_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.gnt_load_plans.argtypes = [ctypes.c_char_p]
+    L.gnt_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
     L.gnt_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.gnt_get_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_uint64)]
@@ -160,9 +163,10 @@ class DeviceBuf:
 class Context:
     """One device context (gnt_ctx)."""
 
-    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0):
+    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0, plan_cache: Optional[str] = None):
         L = load()
-        cfg = GntConfig(device=device, hbm_cap_bytes=hbm_cap_bytes, plan_cache_path=None)
+        cfg = GntConfig(device=device, hbm_cap_bytes=hbm_cap_bytes,
+                        plan_cache_path=plan_cache.encode() if plan_cache else None)
         h = ctypes.c_void_p()
         _check(L.gnt_init(ctypes.byref(h), ctypes.byref(cfg)))
         self.h = h
@@ -352,6 +356,29 @@ class Server:
 
     def stop(self):
         _check(load().gnt_serve_stop(self.ctx.h))
+
+
+def set_plan(log_n: int, log_r=None):
+    """Install a pass split for transforms of length 2^log_n (None: built-in rule)."""
+    arr = (ctypes.c_int * 4)(*(list(log_r or []) + [0] * (4 - len(log_r or []))))
+    _check(load().gnt_set_plan(log_n, len(log_r or []), arr))
+
+
+def load_plans(path: str):
+    _check(load().gnt_load_plans(path.encode()))
+
+
+def tune(ctx: "Context", min_log_n: int, max_log_n: int, reps: int = 3, path: Optional[str] = None):
+    """Time every split of each length on the device, install the fastest, write the cache."""
+    _check(load().gnt_tune(ctx.h, min_log_n, max_log_n, reps, path.encode() if path else None))
+
+
+def plan(log_n: int):
+    """(npass, radices) the library uses for a length-2^log_n transform."""
+    na = 1 << (log_n - 3) if log_n >= 3 else 1
+    ln, np_, lr = ctypes.c_int(), ctypes.c_int(), (ctypes.c_int * 4)()
+    _check(load().gnt_plan_info(na, na, ctypes.byref(ln), ctypes.byref(np_), lr))
+    return np_.value, list(lr)[:np_.value]
 
 
 _default: Optional[Context] = None

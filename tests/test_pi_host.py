@@ -34,7 +34,7 @@ def _run(tmp_path, digits, *extra):
 
 @pytest.mark.parametrize("digits", [0, 1, 2, 15, 100, 101, 1000])
 def test_small_digit_counts(tmp_path, digits):
-    data, _ = _run(tmp_path, digits)
+    data, _ = _run(tmp_path, digits, "--fa-min-limbs", "1e18")
     ref = open(os.path.join(GOLD, "pi_10000.txt"), "rb").read()
     if digits == 0:
         assert data == b"\n"
@@ -44,9 +44,17 @@ def test_small_digit_counts(tmp_path, digits):
         assert data == ref[:digits + 1] + b"\n"
 
 
-@pytest.mark.parametrize("digits,limbs", [(10000, "1"), (10000, "8"), (100000, "64")])
-def test_host_merged_tree(tmp_path, digits, limbs):
-    data, st = _run(tmp_path, digits, "--host-merge", "--gpu-min-limbs", limbs)
+# Final assembly: "newton" is the driver's own (Newton reciprocal and inverse
+# square root, scaled-remainder-tree decimal conversion) with host products;
+# "gmp" is GMP's square root, division and radix conversion.
+FA = {"newton": ("--fa-min-limbs", "1e18"), "gmp": ("--fa-gmp",)}
+
+
+@pytest.mark.parametrize("fa", sorted(FA))
+@pytest.mark.parametrize("digits,limbs", [(10000, "1"), (10000, "8"), (100000, "64"), (1000000, "512")])
+def test_host_merged_tree(tmp_path, digits, limbs, fa):
+    data, st = _run(tmp_path, digits, "--host-merge", "--gpu-min-limbs", limbs, *FA[fa])
     assert st["merge"] == "host" and st["merge_nodes"] > 0
+    assert st["fa"] == ("gmp" if fa == "gmp" else "host")
     gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
     assert hashlib.sha256(data).hexdigest() == gold[str(digits)]["sha256"]
