@@ -26,6 +26,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_PLANS = os.path.join(ROOT, "pi_amd", "plans_gfx950.json")
+
+
+def make_ctx(args, **kw):
+    """Context with the tuned plan cache when one is present."""
+    from pi_amd import gnt
+    return gnt.Context(0, plan_cache=args.plans or None, **kw)
 
 
 def splitmix64(seed: int, n: int) -> np.ndarray:
@@ -93,12 +101,18 @@ def main():
     ap.add_argument("--op", choices=["mul", "sqr"], default="mul")
     ap.add_argument("--cpu-log-size", type=int, default=22, help="log2 limbs of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batched", "serve"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky"], default="single",
                     help="single: the metric (one 2^24 product per step); batched: 1024 uneven pairs per step; "
-                         "serve: open-loop Poisson request stream through the serving loop")
+                         "serve: open-loop Poisson request stream through the serving loop; "
+                         "chudnovsky: pi to --digits digits end to end")
     ap.add_argument("--rates", default="100,200,400,800,1600", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--pairs", type=int, default=1024)
+    ap.add_argument("--plans", default=DEFAULT_PLANS if os.path.exists(DEFAULT_PLANS) else "",
+                    help="plan cache written by the autotuner ('' = built-in plans)")
+    ap.add_argument("--tune", default="", help="LO:HI: autotune transform lengths 2^LO..2^HI, write --tune-out, exit")
+    ap.add_argument("--tune-out", default=DEFAULT_PLANS)
+    ap.add_argument("--digits", type=int, default=100_000_000, help="chudnovsky: decimal digits")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -108,14 +122,24 @@ def main():
 
     from pi_amd import gnt
 
+    if args.tune:
+        lo, hi = (int(x) for x in args.tune.split(":"))
+        ctx = make_ctx(args)
+        t0 = time.perf_counter()
+        gnt.tune(ctx, lo, hi, reps=5, path=args.tune_out)
+        ctx.close()
+        print(json.dumps({"tuned": [lo, hi], "path": args.tune_out, "s": round(time.perf_counter() - t0, 2)}))
+        return
     if args.workload == "batched":
         return bench_batched(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "chudnovsky":
+        return bench_chudnovsky(args)
 
     n = 1 << args.log_size
     a, b = operand(3, n), operand(4, n)
-    ctx = gnt.Context(0)
+    ctx = make_ctx(args)
     log_n, radices = gnt.plan_info(n, n)
     da = ctx.upload(a)
     db = ctx.upload(b)
@@ -171,6 +195,7 @@ def main():
             "workload": f"single GPU {args.op} of 2^{args.log_size} x 2^{args.log_size} limbs",
             "limbs_a": n, "limbs_b": n, "log_n": log_n, "radices_log2": radices,
             "inputs": "resident in HBM", "check_low_limb_and_length": ok,
+            "plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in",
         },
         "roofline": {
             "bound": "hbm",
@@ -249,6 +274,7 @@ def bench_batched(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (SplitMix64 uniform limbs, sizes seed 7)",
         "config": {"workload": f"{len(pairs)} pairs, n = round(2^U[8,20]) limbs per operand",
+                   "plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in",
                    "total_operand_limbs": limbs, "modes": res},
     }
     ctx.close()
@@ -273,7 +299,7 @@ def bench_serve(args):
     def residue(x):
         return int.from_bytes(np.ascontiguousarray(x, dtype="<u8").tobytes(), "little") % q61
 
-    ctx = gnt.Context(0, hbm_cap_bytes=200 * 10**9)
+    ctx = make_ctx(args, hbm_cap_bytes=200 * 10**9)
     srv = gnt.Server(ctx)
     # warm-up: first-use allocations of every size class
     for e in range(8, 25, 2):
@@ -384,7 +410,8 @@ def bench_serve(args):
         "steps": args.requests, "warmup": 9, "ms_per_step": best["p50_ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (SplitMix64 limbs, Poisson arrivals, seed 8)",
-        "config": {"workload": "open-loop Poisson; n = round(2^U[8,24]); 70/30 mul/sqr; 20% reuse of device results; "
+        "config": {"plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in",
+                   "workload": "open-loop Poisson; n = round(2^U[8,24]); 70/30 mul/sqr; 20% reuse of device results; "
                                "HBM cap 200 GB", "sweep": points, "server": stats},
     }
     print(json.dumps(line), flush=True)
@@ -392,3 +419,56 @@ def bench_serve(args):
 
 if __name__ == "__main__":
     main()
+
+
+def bench_chudnovsky(args):
+    """pi to --digits decimal digits with pi_chudnovsky (binary-splitting merges
+    through the serving loop, final assembly and decimal conversion over GPU
+    products); digits checked against the pinned SHA-256 when one exists."""
+    import hashlib
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "pi_amd", "pi_chudnovsky")
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "pi_digits.json")))["digits"].get(str(args.digits))
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "pi.txt")
+        cmd = [exe, str(args.digits), out]
+        if args.plans:
+            cmd += ["--plans", args.plans]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            sys.exit(r.returncode)
+        st = json.loads(r.stdout.strip().splitlines()[-1])
+        h = hashlib.sha256()
+        with open(out, "rb") as f:
+            for chunk in iter(lambda: f.read(1 << 24), b""):
+                h.update(chunk)
+    merge_s = sum(L["s"] for L in st["levels"])
+    line = {
+        "metric": f"Chudnovsky pi to {args.digits} digits, wall time",
+        "value": round(st["t_total"], 3),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(st["t_total"] * 1e3, 1),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "the Chudnovsky series (no input data)",
+        "config": {"workload": f"pi to {args.digits} digits", "n_terms": st["n_terms"], "prec_bits": st["prec_bits"],
+                   "gpu_min_limbs": st["gpu_min_limbs"], "host_threads": st["threads"],
+                   "plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in"},
+        "digits_sha256": h.hexdigest(),
+        "byte_identical_to_pinned": (h.hexdigest() == gold["sha256"]) if gold else None,
+        "process_wall_s": round(wall, 3),
+        "phases_s": {k: st[k] for k in ("t_init", "t_host_bs", "t_upload", "t_merge", "t_download",
+                                         "t_final_assembly", "t_decimal", "t_write")},
+        "gpu_merge_share": round(merge_s / st["t_total"], 4),
+        "levels": st["levels"],
+    }
+    print(json.dumps(line), flush=True)

@@ -115,6 +115,7 @@ struct Options {
     bool fa_gmp = false;          // final assembly and radix conversion by GMP (comparison run)
     double fa_min_limbs = 1024;   // final-assembly / conversion products on the GPU from this size
     int device = 0;
+    std::string plans;            // plan cache from the autotuner
 };
 
 struct Driver {
@@ -652,7 +653,7 @@ std::string final_assembly_gmp(const Mpz &Q, const Mpz &Tabs, uint64_t digits, u
 void usage() {
     std::fprintf(stderr,
                  "usage: pi_chudnovsky DIGITS OUT_FILE [--gpu-min-limbs L] [--threads T] [--host-merge]\n"
-                 "                     [--fa-gmp] [--fa-min-limbs L] [--device D]\n");
+                 "                     [--fa-gmp] [--fa-min-limbs L] [--device D] [--plans CACHE.json]\n");
     std::exit(2);
 }
 
@@ -671,6 +672,7 @@ int main(int argc, char **argv) {
         else if (a == "--fa-gmp") o.fa_gmp = true;
         else if (a == "--fa-min-limbs" && i + 1 < argc) o.fa_min_limbs = std::atof(argv[++i]);
         else if (a == "--device" && i + 1 < argc) o.device = std::atoi(argv[++i]);
+        else if (a == "--plans" && i + 1 < argc) o.plans = argv[++i];
         else usage();
     }
     if (o.threads <= 0) {
@@ -702,6 +704,7 @@ int main(int argc, char **argv) {
         auto t0 = Clock::now();
         gnt_config cfg{};
         cfg.device = o.device;
+        cfg.plan_cache_path = o.plans.empty() ? nullptr : o.plans.c_str();
         check(gnt_init(&d.ctx, &cfg), "init");
         if (gpu_merge) check(gnt_serve_start(d.ctx), "serve start");
         t_init = since(t0);

@@ -110,6 +110,20 @@ u64 emu_root(int log_n) {
     return root_pow(tw(), (u32)e);
 }
 
+// Install a pass split for one length in this emulation (npass 0: built-in rule).
+int emu_set_plan(int log_n, int npass, const int *log_r) {
+    if (npass == 0) {
+        g_plan_choice[log_n] = PlanChoice{};
+        return 0;
+    }
+    if (!split_valid(log_n, npass, log_r)) return -1;
+    PlanChoice c{};
+    c.npass = npass;
+    for (int p = 0; p < npass; p++) c.log_r[p] = log_r[p];
+    g_plan_choice[log_n] = c;
+    return 0;
+}
+
 void emu_plan(int log_n, int *npass, int *log_r, int *log_c) {
     Plan pl = make_plan(log_n);
     *npass = pl.npass;

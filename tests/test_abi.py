@@ -58,3 +58,31 @@ def test_plan_info_and_errors_without_gpu():
     assert rc in (gnt.GNT_OK, gnt.GNT_E_DEVICE)
     if rc == gnt.GNT_OK:
         lib.gnt_shutdown(h)
+
+
+def test_plan_overrides_and_cache(tmp_path):
+    """gnt_set_plan / gnt_load_plans change the split gnt_plan_info reports
+    (host-side state, no device needed); uncovered splits are refused."""
+    from pi_amd import gnt
+    try:
+        assert gnt.plan(24) == (3, [8, 8, 8])          # built-in rule
+        gnt.set_plan(24, [7, 7, 10])
+        assert gnt.plan(24) == (3, [7, 7, 10])
+        with pytest.raises(gnt.GntError):
+            gnt.set_plan(24, [6, 8, 10])               # radix 2^6 has no strided tile shape
+        with pytest.raises(gnt.GntError):
+            gnt.set_plan(24, [8, 8])                   # does not sum to 24
+        gnt.set_plan(24)
+        assert gnt.plan(24) == (3, [8, 8, 8])
+        p = tmp_path / "plans.json"
+        p.write_text('{"device": "gfx950",\n "plans": {\n  "17": [9, 8],\n  "22": [8, 7, 7]\n },\n'
+                     ' "us": {\n  "17": {"9,8": 1.0}\n }\n}\n')
+        gnt.load_plans(str(p))
+        assert gnt.plan(17) == (2, [9, 8]) and gnt.plan(22) == (3, [8, 7, 7])
+        bad = tmp_path / "bad.json"
+        bad.write_text('{"plans": {"17": [5, 12]}}')
+        with pytest.raises(gnt.GntError):
+            gnt.load_plans(str(bad))
+    finally:
+        for n in (17, 22, 24):
+            gnt.set_plan(n)

@@ -141,3 +141,39 @@ def test_all_ones(emu, orc, n):
     want[n + 1:] = np.uint64(2**64 - 1)
     assert np.array_equal(emu_mul(emu, a, a), want)
     assert np.array_equal(emu_mul(emu, a), want)
+
+
+def _splits(log_n):
+    import itertools
+    out = []
+    for np_ in (2, 3, 4):
+        for c in itertools.product(range(7, 11), repeat=np_):
+            if sum(c) == log_n:
+                out.append(list(c))
+    return out
+
+
+@pytest.mark.parametrize("log_n", [14, 17, 21])
+def test_every_split_gives_the_same_product(emu, orc, log_n):
+    """The autotuner may pick any split of a length into radix-2^7..2^10 passes:
+    every one must give the exact product (forward and inverse both follow it)."""
+    E = emu
+    E.emu_set_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    na = (1 << (log_n - 3)) - 3
+    a = orc.random_vec(na, log_n)
+    b = orc.random_vec(na - 5, log_n + 1)
+    want = orc.gmp_mul(a, b)
+    try:
+        for sp in _splits(log_n):
+            arr = (ctypes.c_int * 4)(*(sp + [0] * (4 - len(sp))))
+            assert E.emu_set_plan(log_n, len(sp), arr) == 0
+            npass, lr, lc = ctypes.c_int(), (ctypes.c_int * 4)(), (ctypes.c_int * 4)()
+            E.emu_plan(log_n, ctypes.byref(npass), lr, lc)
+            assert list(lr)[:npass.value] == sp
+            assert np.array_equal(emu_mul(E, a, b), want), sp
+            assert np.array_equal(emu_mul(E, a), orc.gmp_sqr(a)), sp
+    finally:
+        E.emu_set_plan(log_n, 0, None)
+    # an uncovered split is refused
+    bad = (ctypes.c_int * 4)(6, log_n - 6, 0, 0)
+    assert E.emu_set_plan(log_n, 2, bad) != 0

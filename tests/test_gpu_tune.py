@@ -1,0 +1,63 @@
+"""Autotuned plans on the device: every split the tuner may choose gives the
+exact product, and a tuned plan cache round-trips through gnt_init."""
+import itertools
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _splits(log_n):
+    out = []
+    for np_ in (2, 3, 4):
+        for c in itertools.product(range(7, 11), repeat=np_):
+            if sum(c) == log_n:
+                out.append(list(c))
+    return out
+
+
+@pytest.mark.parametrize("log_n", [14, 19, 21, 24])
+def test_every_split_exact(ctx, orc, log_n):
+    from pi_amd import gnt
+    na = (1 << (log_n - 3)) - 1
+    a = orc.splitmix64(log_n, na)
+    b = orc.splitmix64(log_n + 100, na - 7)
+    want, want_sq = orc.gmp_mul(a, b), orc.gmp_sqr(a)
+    try:
+        for sp in _splits(log_n):
+            print(f"log_n {log_n} split {sp}", flush=True)
+            gnt.set_plan(log_n, sp)
+            assert gnt.plan(log_n) == (len(sp), sp)
+            assert np.array_equal(ctx.mul_mag_ntt(a, b), want), sp
+            assert np.array_equal(ctx.sqr(a), want_sq), sp
+    finally:
+        gnt.set_plan(log_n)
+
+
+def test_tune_writes_loadable_cache(ctx, orc, tmp_path):
+    from pi_amd import gnt
+    path = tmp_path / "plans.json"
+    try:
+        gnt.tune(ctx, 12, 18, reps=2, path=str(path))
+        cache = json.loads(path.read_text())
+        assert set(cache["plans"]) == {str(n) for n in range(12, 19)}
+        for n, sp in cache["plans"].items():
+            assert gnt.plan(int(n)) == (len(sp), sp)          # installed
+            assert ",".join(map(str, sp)) in cache["us"][n]   # the fastest timed candidate
+            timed = cache["us"][n]
+            assert timed[",".join(map(str, sp))] == min(timed.values())
+        for n in range(12, 19):
+            gnt.set_plan(n)
+        c2 = gnt.Context(0, plan_cache=str(path))             # gnt_init loads it
+        try:
+            for n, sp in cache["plans"].items():
+                assert gnt.plan(int(n)) == (len(sp), sp)
+            a = orc.splitmix64(7, 1 << 14)
+            assert np.array_equal(c2.mul_mag_ntt(a, a), orc.gmp_sqr(a))
+        finally:
+            c2.close()
+    finally:
+        for n in range(12, 19):
+            gnt.set_plan(n)
