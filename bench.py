@@ -417,18 +417,15 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
-if __name__ == "__main__":
-    main()
-
 
 def bench_chudnovsky(args):
-    """pi to --digits decimal digits with pi_chudnovsky (binary-splitting merges
+    """pi to --digits decimal digits with pi_compute (binary-splitting merges
     through the serving loop, final assembly and decimal conversion over GPU
     products); digits checked against the pinned SHA-256 when one exists."""
     import hashlib
     import subprocess
     import tempfile
-    exe = os.path.join(ROOT, "pi_amd", "pi_chudnovsky")
+    exe = os.path.join(ROOT, "pi_amd", "pi_compute")
     gold = json.load(open(os.path.join(ROOT, "tests", "golden", "pi_digits.json")))["digits"].get(str(args.digits))
     with tempfile.TemporaryDirectory() as tmp:
         out = os.path.join(tmp, "pi.txt")
@@ -472,3 +469,7 @@ def bench_chudnovsky(args):
         "levels": st["levels"],
     }
     print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

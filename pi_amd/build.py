@@ -63,12 +63,12 @@ def build(verbose: bool = False) -> str:
 
 
 GMP = "/usr/lib/x86_64-linux-gnu/libgmp.so.10"
-PI_BIN = os.path.join(HERE, "pi_chudnovsky")
+PI_BIN = os.path.join(HERE, "pi_compute")
 
 
 def build_pi_driver() -> str:
-    """Host driver of the Chudnovsky run (binary splitting over libgnt, GMP for host nodes)."""
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-pthread", os.path.join(CSRC, "pi_chudnovsky.cpp"),
+    """Host driver of the pi runs (Chudnovsky binary splitting and Gauss-Legendre over libgnt)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-pthread", os.path.join(CSRC, "pi_compute.cpp"),
            "-o", PI_BIN + ".tmp", "-L" + HERE, "-lgnt", GMP, "-Wl,-rpath,$ORIGIN",
            "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.run(cmd, check=True)

@@ -52,7 +52,7 @@ using Clock = std::chrono::steady_clock;
 double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
 [[noreturn]] void die(const char *what, const char *detail = "") {
-    std::fprintf(stderr, "pi_chudnovsky: %s %s\n", what, detail);
+    std::fprintf(stderr, "pi_compute: %s %s\n", what, detail);
     std::exit(1);
 }
 
@@ -116,6 +116,7 @@ struct Options {
     double fa_min_limbs = 1024;   // final-assembly / conversion products on the GPU from this size
     int device = 0;
     std::string plans;            // plan cache from the autotuner
+    std::string algorithm = "chudnovsky";
 };
 
 struct Driver {
@@ -573,13 +574,8 @@ struct Decimal {
 };
 
 // floor(pi * 10^(D-1)) as D characters with every large product on the GPU.
-std::string pi_digits_gpu(Mul &M, const Mpz &Q, const Mpz &Tabs, uint64_t digits, uint64_t prec, int threads,
-                          double *t_fa, double *t_dec) {
-    auto t0 = Clock::now();
-    Mpz pi_fix;
-    assemble_pi(M, Q, Tabs, (unsigned long)prec, &pi_fix);
-    *t_fa = since(t0);
-    t0 = Clock::now();
+// D characters of pi from pi_fix ~ pi 2^prec (error bound err units).
+std::string digits_of(Mul &M, const Mpz &pi_fix, uint64_t digits, uint64_t prec, int threads, unsigned err) {
     Mpz ip;
     __gmpz_tdiv_q_2exp(&ip, &pi_fix, prec);
     if (ip.limbs() != 1 || ip.z.d[0] != 3) die("integer part is not 3", "");
@@ -592,12 +588,103 @@ std::string pi_digits_gpu(Mul &M, const Mpz &Q, const Mpz &Tabs, uint64_t digits
         root.F = prec;
         root.n = digits - 1;
         root.at = 1;
-        root.err = 4;
-        // pre-compute the powers of the tree's top levels while nothing else runs
+        root.err = err;
         dec.run(std::move(root), out);
     }
+    return out;
+}
+
+std::string pi_digits_gpu(Mul &M, const Mpz &Q, const Mpz &Tabs, uint64_t digits, uint64_t prec, int threads,
+                          double *t_fa, double *t_dec) {
+    auto t0 = Clock::now();
+    Mpz pi_fix;
+    assemble_pi(M, Q, Tabs, (unsigned long)prec, &pi_fix);
+    *t_fa = since(t0);
+    t0 = Clock::now();
+    std::string out = digits_of(M, pi_fix, digits, prec, threads, 4);
     *t_dec = since(t0);
     return out;
+}
+
+// ---- Gauss-Legendre (gauss_legendre.rs:61-166) ----
+
+// ~ sqrt(x), within a small fraction of a unit: Newton for y ~ 1/sqrt(v),
+// v = x / 2^e in [1/4, 1) (e even), then sqrt(x) = x y 2^(-e/2).  The
+// reference's Float::sqrt_mut (float.rs:85-174) is the same precision-
+// doubling iteration on Floats.
+void sqrt_big(Mul &M, const gmp_mpz *x, unsigned long k, gmp_mpz *out) {
+    const unsigned long n = bits_of(x);
+    const unsigned long e = n + (n & 1);
+    std::vector<unsigned long> ps = newton_chain(k);
+    unsigned long p = ps.back();
+    Mpz y;
+    {
+        Mpz top;
+        shift(&top, x, 53 - (long)n);
+        const long double v = std::ldexp((long double)top.z.d[0], (int)n - 53 - (int)e);  // x / 2^e
+        __gmpz_set_ui(&y, (unsigned long)std::ldexp(1.0L / std::sqrt(v), (int)p));
+    }
+    const unsigned long g = 32;
+    Mpz vt, sq, t, er, corr;
+    for (size_t i = ps.size() - 1; i-- > 0;) {
+        const unsigned long q = ps[i];
+        shift(&vt, x, (long)(q + g) - (long)e);   // ~ v 2^(q+g)
+        M.mul(&sq, &y, &y);                        // ~ 2^(2p) / v
+        M.mul(&t, &vt, &sq);                       // ~ 2^(q+g+2p)
+        __gmpz_set_ui(&er, 1);
+        __gmpz_mul_2exp(&er, &er, q + g + 2 * p);
+        __gmpz_sub(&er, &er, &t);                  // ~ err 2^(q+g+p), signed
+        const long eb = (long)bits_of(&er);
+        const long drop = std::max(0L, eb - (long)(q - p + 48));
+        shift(&er, &er, -drop);
+        M.mul(&corr, &y, &er);
+        shift(&corr, &corr, drop - (long)(g + 3 * p + 1));
+        shift(&y, &y, (long)(q - p));
+        __gmpz_add(&y, &y, &corr);
+        p = q;
+    }
+    // sqrt(x) = x y_k / 2^(e/2 + k), with x cut to k + 64 bits
+    Mpz xt;
+    const long d = std::max(0L, (long)n - (long)(k + 64));
+    shift(&xt, x, -d);
+    M.mul(out, &xt, &y);
+    shift(out, out, d - (long)(e / 2 + k));
+}
+
+// pi 2^prec by the Brent-Salamin AGM in fixed point with 64 guard bits:
+//   a0 = 1, b0 = 1/sqrt(2), t0 = 1/4, p0 = 1
+//   a' = (a+b)/2, b' = sqrt(a b), t' = t - p (a - a')^2, p' = 2p
+//   pi = (a+b)^2 / (4t)
+// for ceil(log2 prec) + 2 iterations (gauss_legendre.rs:77-79, 119-153).
+void pi_gauss_legendre(Mul &M, unsigned long prec, gmp_mpz *pi_fix, int *iters_out) {
+    const unsigned long W = prec + 64, k = W + 32;
+    Mpz a, b, t, an, ab, dd;
+    __gmpz_set_ui(&a, 1);
+    __gmpz_mul_2exp(&a, &a, W);
+    inv_sqrt(M, 2, W, &b);                 // 2^W / sqrt(2)
+    __gmpz_set_ui(&t, 1);
+    __gmpz_mul_2exp(&t, &t, W - 2);
+    const int iters = (int)std::ceil(std::log2((double)prec)) + 2;
+    for (int i = 0; i < iters; i++) {
+        __gmpz_add(&an, &a, &b);
+        __gmpz_tdiv_q_2exp(&an, &an, 1);
+        M.mul(&ab, &a, &b);                // scale 2^(2W)
+        sqrt_big(M, &ab, k, &b);           // b' = sqrt(a b), scale 2^W
+        __gmpz_sub(&dd, &a, &an);
+        M.mul(&dd, &dd, &dd);
+        shift(&dd, &dd, (long)i - (long)W);  // p (a - a')^2 with p = 2^i
+        __gmpz_sub(&t, &t, &dd);
+        __gmpz_swap(&a, &an);
+    }
+    __gmpz_add(&an, &a, &b);
+    M.mul(&ab, &an, &an);                  // (a+b)^2, scale 2^(2W)
+    shift(&ab, &ab, -(long)W);
+    Mpz R;
+    const unsigned long nt = bits_of(&t);
+    reciprocal(M, &t, k, &R);              // ~ 2^(nt+k) / t
+    M.mul(pi_fix, &ab, &R);                // (a+b)^2 / (4t) at scale 2^W
+    shift(pi_fix, pi_fix, (long)W - (long)(nt + k + 2) - 64);
+    *iters_out = iters;
 }
 
 // floor(pi * 10^(D-1)) with GMP's own square root, division and radix
@@ -652,12 +739,60 @@ std::string final_assembly_gmp(const Mpz &Q, const Mpz &Tabs, uint64_t digits, u
 
 void usage() {
     std::fprintf(stderr,
-                 "usage: pi_chudnovsky DIGITS OUT_FILE [--gpu-min-limbs L] [--threads T] [--host-merge]\n"
+                 "usage: pi_compute DIGITS OUT_FILE [--algorithm chudnovsky|gauss-legendre] [--gpu-min-limbs L]\n"
+                 "                  [--threads T] [--host-merge]\n"
                  "                     [--fa-gmp] [--fa-min-limbs L] [--device D] [--plans CACHE.json]\n");
     std::exit(2);
 }
 
 }  // namespace
+
+void write_digits(FILE *f, const std::string &s, uint64_t digits, const std::string &path) {
+    std::fwrite(s.data(), 1, 1, f);
+    if (digits > 1) {
+        std::fputc('.', f);
+        std::fwrite(s.data() + 1, 1, s.size() - 1, f);
+    }
+    std::fputc('\n', f);
+    if (std::fclose(f) != 0) die("write failed", path.c_str());
+}
+
+int run_gauss_legendre(const Options &o, uint64_t prec, FILE *f, Clock::time_point t_start) {
+    gnt_ctx *ctx = nullptr;
+    auto t0 = Clock::now();
+    const bool gpu = (double)prec / 64.0 >= o.fa_min_limbs;
+    if (gpu) {
+        gnt_config cfg{};
+        cfg.device = o.device;
+        cfg.plan_cache_path = o.plans.empty() ? nullptr : o.plans.c_str();
+        check(gnt_init(&ctx, &cfg), "init");
+    }
+    const double t_init = since(t0);
+    Mul M;
+    M.ctx = ctx;
+    M.min_limbs = o.fa_min_limbs;
+    M.threads = o.threads;
+    t0 = Clock::now();
+    Mpz pi_fix;
+    int iters = 0;
+    pi_gauss_legendre(M, (unsigned long)prec, &pi_fix, &iters);
+    const double t_agm = since(t0);
+    t0 = Clock::now();
+    // AGM rounding: t loses up to ~2^iters units at 2^-W, W = prec + 64
+    const std::string s = digits_of(M, pi_fix, o.digits, prec, o.threads, 8);
+    const double t_dec = since(t0);
+    t0 = Clock::now();
+    write_digits(f, s, o.digits, o.out);
+    const double t_write = since(t0);
+    if (ctx) gnt_shutdown(ctx);
+    std::printf("{\"digits\": %llu, \"algorithm\": \"gauss-legendre\", \"prec_bits\": %llu, \"iterations\": %d, "
+                "\"threads\": %d, \"products\": \"%s\", \"t_init\": %.3f, \"t_agm\": %.3f, \"t_decimal\": %.3f, "
+                "\"t_write\": %.3f, \"t_total\": %.3f, \"gpu_products\": %zu, \"host_products\": %zu, "
+                "\"gpu_operand_limbs\": %.0f}\n",
+                (unsigned long long)o.digits, (unsigned long long)prec, iters, o.threads, gpu ? "gpu" : "host", t_init,
+                t_agm, t_dec, t_write, since(t_start), M.gpu_products, M.host_products, M.gpu_limbs);
+    return 0;
+}
 
 int main(int argc, char **argv) {
     if (argc < 3) usage();
@@ -673,6 +808,7 @@ int main(int argc, char **argv) {
         else if (a == "--fa-min-limbs" && i + 1 < argc) o.fa_min_limbs = std::atof(argv[++i]);
         else if (a == "--device" && i + 1 < argc) o.device = std::atoi(argv[++i]);
         else if (a == "--plans" && i + 1 < argc) o.plans = argv[++i];
+        else if (a == "--algorithm" && i + 1 < argc) o.algorithm = argv[++i];
         else usage();
     }
     if (o.threads <= 0) {
@@ -690,6 +826,8 @@ int main(int argc, char **argv) {
     }
     const uint64_t n_terms = (uint64_t)std::ceil((double)o.digits / DIGITS_PER_TERM) + SAFETY_TERMS;
     const uint64_t prec = (uint64_t)std::ceil((double)o.digits * std::log2(10.0)) + SAFETY_BITS;
+    if (o.algorithm == "gauss-legendre") return run_gauss_legendre(o, prec, f, t_start);
+    if (o.algorithm != "chudnovsky") usage();
 
     Driver d;
     d.o = o;
@@ -767,13 +905,7 @@ int main(int argc, char **argv) {
     std::string s = o.fa_gmp ? final_assembly_gmp(Q, Tabs, o.digits, prec, &t_fa, &t_dec)
                              : pi_digits_gpu(M, Q, Tabs, o.digits, prec, o.threads, &t_fa, &t_dec);
     t0 = Clock::now();
-    std::fwrite(s.data(), 1, 1, f);
-    if (o.digits > 1) {
-        std::fputc('.', f);
-        std::fwrite(s.data() + 1, 1, s.size() - 1, f);
-    }
-    std::fputc('\n', f);
-    if (std::fclose(f) != 0) die("write failed", o.out.c_str());
+    write_digits(f, s, o.digits, o.out);
     const double t_write = since(t0);
     const double t_total = since(t_start);
 

@@ -16,7 +16,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PI = os.path.join(ROOT, "pi_amd", "pi_chudnovsky")
+PI = os.path.join(ROOT, "pi_amd", "pi_compute")
 GOLD = os.path.join(ROOT, "tests", "golden")
 
 
@@ -27,7 +27,7 @@ def _digest(path):
 
 def _run(digits, tmp_path, *extra):
     out = tmp_path / f"pi_{digits}.txt"
-    print(f"pi_chudnovsky {digits} {' '.join(extra)}", flush=True)
+    print(f"pi_compute {digits} {' '.join(extra)}", flush=True)
     r = subprocess.run([PI, str(digits), str(out), *extra], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     stats = json.loads(r.stdout.strip().splitlines()[-1])
@@ -49,4 +49,13 @@ def test_pi_sha256(tmp_path, digits, gpu_min_limbs):
     out, st = _run(digits, tmp_path, "--gpu-min-limbs", gpu_min_limbs)
     assert st["merge"] == "gpu" and st["merge_nodes"] > 0
     assert os.path.getsize(out) == gold[str(digits)]["bytes"]
+    assert _digest(out) == gold[str(digits)]["sha256"]
+
+
+@pytest.mark.parametrize("digits,fa_min", [(100000, "16"), (1000000, "256"), (10000000, "1024")])
+def test_gauss_legendre_gpu(tmp_path, digits, fa_min):
+    """The AGM cross-check with its full-precision products and squares on the GPU."""
+    gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
+    out, st = _run(digits, tmp_path, "--algorithm", "gauss-legendre", "--fa-min-limbs", fa_min)
+    assert st["products"] == "gpu" and st["gpu_products"] > 0
     assert _digest(out) == gold[str(digits)]["sha256"]

@@ -10,7 +10,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PI = os.path.join(ROOT, "pi_amd", "pi_chudnovsky")
+PI = os.path.join(ROOT, "pi_amd", "pi_compute")
 GOLD = os.path.join(ROOT, "tests", "golden")
 FIRST_100 = ("3.141592653589793238462643383279502884197169399375105820974944592307816406286208998628034825342117067")
 
@@ -58,3 +58,16 @@ def test_host_merged_tree(tmp_path, digits, limbs, fa):
     assert st["fa"] == ("gmp" if fa == "gmp" else "host")
     gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
     assert hashlib.sha256(data).hexdigest() == gold[str(digits)]["sha256"]
+
+
+@pytest.mark.parametrize("digits", [2, 1000, 10000, 100000])
+def test_gauss_legendre_host(tmp_path, digits):
+    """The AGM cross-check (gauss_legendre.rs:61-166) with host products."""
+    data, st = _run(tmp_path, digits, "--algorithm", "gauss-legendre", "--fa-min-limbs", "1e18")
+    assert st["algorithm"] == "gauss-legendre" and st["products"] == "host"
+    ref = open(os.path.join(GOLD, "pi_10000.txt"), "rb").read()
+    if digits <= 10000:
+        assert data == (ref[:digits + 1] + b"\n" if digits < 10000 else ref)
+    else:
+        gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
+        assert hashlib.sha256(data).hexdigest() == gold[str(digits)]["sha256"]
