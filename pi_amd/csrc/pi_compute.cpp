@@ -907,9 +907,10 @@ int main(int argc, char **argv) {
     t0 = Clock::now();
     write_digits(f, s, o.digits, o.out);
     const double t_write = since(t0);
-    const double t_total = since(t_start);
-
+    t0 = Clock::now();
     if (d.ctx) gnt_shutdown(d.ctx);
+    const double t_shutdown = since(t0);
+    const double t_total = since(t_start);
     double t_levels = 0;
     std::string lv = "[";
     for (size_t i = 0; i < d.levels.size(); i++) {
@@ -924,13 +925,13 @@ int main(int argc, char **argv) {
     std::printf("{\"digits\": %llu, \"n_terms\": %llu, \"prec_bits\": %llu, \"merge\": \"%s\", \"threads\": %d, "
                 "\"gpu_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
                 "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
-                "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_total\": %.3f, "
+                "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_shutdown\": %.3f, \"t_total\": %.3f, "
                 "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"fa\": \"%s\", \"fa_gpu_products\": %zu, "
                 "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"levels\": %s}\n",
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
                 !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"), o.threads, o.gpu_min_limbs,
                 d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
-                t_download, t_fa, t_dec, t_write, t_total, (unsigned long long)sstats[0],
+                t_download, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
                 M.host_products, M.gpu_limbs, lv.c_str());
     (void)t_levels;
