@@ -484,6 +484,7 @@ struct Decimal {
 
     explicit Decimal(Mul &m, int t) : M(m), threads(t) {}
 
+    // 10^h, memoized.  Inserts: call only from one thread at a time.
     const gmp_mpz *p10(unsigned long h) {
         auto it = pow10.find(h);
         if (it != pow10.end()) return &*it->second;
@@ -569,6 +570,9 @@ struct Decimal {
             });
             level = std::move(next);
         }
+        // every power the leaves need exists before the threads start: p10()
+        // inserts into the map, which is not safe to do concurrently
+        for (const Frac &x : leaves) p10(x.n);
         parallel_for(leaves.size(), threads, [&](size_t i) { leaf(leaves[i], out); });
     }
 };
