@@ -75,8 +75,8 @@ size_t Pool::size_class(size_t bytes) {
 
 void *Pool::acquire(size_t bytes) {
     bytes = size_class(bytes);
-    {
-        std::lock_guard<std::mutex> g(mu);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        std::unique_lock<std::mutex> g(mu);
         auto it = free_.find(bytes);
         if (it != free_.end() && !it->second.empty()) {
             void *p = it->second.back();
@@ -84,7 +84,27 @@ void *Pool::acquire(size_t bytes) {
             pooled_bytes -= bytes;
             return p;
         }
+        if (bytes <= SLAB_PIECE_MAX) {
+            if (slab_left_ < bytes) {
+                void *s = nullptr;
+                if (hipMalloc(&s, SLAB_BYTES) != hipSuccess) {
+                    g.unlock();
+                    if (attempt == 0) trim();  // idle large buffers back to the device, then retry
+                    continue;
+                }
+                slabs_.push_back(s);
+                live_bytes += SLAB_BYTES;
+                slab_cur_ = (char *)s;
+                slab_left_ = SLAB_BYTES;
+            }
+            void *p = slab_cur_;  // classes are multiples of 256 bytes: pieces stay aligned
+            slab_cur_ += bytes;
+            slab_left_ -= bytes;
+            return p;
+        }
+        break;
     }
+    if (bytes <= SLAB_PIECE_MAX) return nullptr;
     void *p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) {
         // give cached buffers back to the device and try once more
@@ -101,8 +121,9 @@ void Pool::release(void *p, size_t bytes) {
     bytes = size_class(bytes);
     std::lock_guard<std::mutex> g(mu);
     auto &v = free_[bytes];
-    // idle buffers are kept while the idle total stays under POOL_IDLE_MAX
-    if (pooled_bytes + bytes <= POOL_IDLE_MAX && v.size() < 4096) {
+    // slab pieces are always kept; large buffers while the idle total stays
+    // under POOL_IDLE_MAX
+    if (bytes <= SLAB_PIECE_MAX || (pooled_bytes + bytes <= POOL_IDLE_MAX && v.size() < 4096)) {
         v.push_back(p);
         pooled_bytes += bytes;
     } else {
@@ -113,13 +134,30 @@ void Pool::release(void *p, size_t bytes) {
 
 void Pool::trim() {
     std::lock_guard<std::mutex> g(mu);
-    for (auto &kv : free_)
-        for (void *p : kv.second) {
-            (void)hipFree(p);
-            live_bytes -= kv.first;
-            pooled_bytes -= kv.first;
+    for (auto it = free_.begin(); it != free_.end();) {
+        if (it->first <= SLAB_PIECE_MAX) {
+            ++it;
+            continue;
         }
+        for (void *p : it->second) {
+            (void)hipFree(p);
+            live_bytes -= it->first;
+            pooled_bytes -= it->first;
+        }
+        it = free_.erase(it);
+    }
+}
+
+void Pool::release_all() {
+    trim();
+    std::lock_guard<std::mutex> g(mu);
+    for (void *s : slabs_) (void)hipFree(s);
+    live_bytes -= slabs_.size() * SLAB_BYTES;
+    slabs_.clear();
     free_.clear();
+    pooled_bytes = 0;
+    slab_cur_ = nullptr;
+    slab_left_ = 0;
 }
 
 int Ctx::init_tables() {
@@ -175,15 +213,18 @@ int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64
         (void)hipEventRecord(ev[0], stream);
     }
     const int P = pl.npass;
+    const bool ab = !sqr && P > 1;  // both last forward passes in the fused kernel
     int rc = 0;
     if (sqr) {
         if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, tw(), stream);
     } else {
-        rc = launch_forward(pl, P, xa, da, na, tw(), stream);
+        rc = launch_forward(pl, ab ? P - 1 : P, xa, da, na, tw(), stream);
         if (!rc && P > 1) rc = launch_forward(pl, P - 1, xb, db, nb, tw(), stream);
     }
     if (timing) (void)hipEventRecord(ev[1], stream);
-    if (!rc) {
+    if (!rc && ab) {
+        rc = launch_last_fused_ab(pl, xa, xb, tw(), stream);
+    } else if (!rc) {
         const u64 *bsrc = sqr ? xa : xb;
         const u64 *blimbs = sqr ? da : db;
         rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, tw(), Ls, Hs, n_out, n_limbs, stream);
@@ -282,8 +323,8 @@ int gnt_shutdown(gnt_ctx *ctx) {
     gnt_serve_stop(ctx);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    c->pool.trim();
-    c->upload_pool.trim();
+    c->pool.release_all();
+    c->upload_pool.release_all();
     (void)hipFree(c->wlo);
     (void)hipFree(c->whi);
     (void)hipFree(c->wsm);

@@ -21,14 +21,6 @@ using u64 = uint64_t;
 namespace gnt {
 namespace {
 
-struct Limbs {
-    std::vector<u64> v;
-    size_t len() const { return v.size(); }
-    void trim() {
-        while (!v.empty() && v.back() == 0) v.pop_back();
-    }
-};
-
 // integer.rs:492-518
 std::vector<u64> schoolbook(const u64 *a, size_t na, const u64 *b, size_t nb) {
     std::vector<u64> out(na + nb, 0);

@@ -43,9 +43,13 @@ __global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     extern __shared__ uint64_t tile[];
     for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
         const TItem it = items[find_item(items, n_items, t)];
-        const G g = make_geom<G>(it.d, t - it.tile0);
+        const G g = make_geom<G>(it.d, t - it.tile0);  // (FUSED_AB builds its half-width geometry)
         if (KIND == K_FWD) body_fwd<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, tw);
         else if (KIND == K_INV) body_inv<X, G>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs);
+        else if (KIND == K_FUSED_AB) {
+            using H = typename HalfGeom<G>::type;  // half-width tiles of the last-pass shape
+            if constexpr (HalfGeom<G>::ok) body_fused_ab<X, H>(tile, make_geom<H>(it.d, t - it.tile0), it.x, it.other, it.log_n, tw);
+        }
         else body_fused<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls,
                               it.Hs, it.n_coeffs, it.n_limbs);
         __syncthreads();  // the tile buffer is reused by the next iteration
@@ -236,7 +240,9 @@ bool prod_init(Prod &p) {
     p.N = (size_t)1 << log_n;
     p.nt = (p.nl + CARRY_TILE - 1) / CARRY_TILE;
     const int P = p.pl.npass;
-    p.nstages = p.sqr() ? 2 * P + 2 : 2 * P + 3;
+    // a multiply with two or more passes runs both last forward passes in the
+    // fused stage, as a square does its one
+    p.nstages = (p.sqr() || P > 1) ? 2 * P + 2 : 2 * P + 3;
     p.stage = 0;
     return true;
 }
@@ -259,7 +265,8 @@ void add_stage(Iteration &it, Prod &p) {
     const bool sq = p.sqr();
     uint64_t *Ls = p.lh, *Hs = p.lh + p.nl;
     // forward passes before the fused one
-    const int nfwd = sq ? P - 1 : P;  // stages that hold forward passes
+    const bool ab = !sq && P > 1;      // fused stage takes both last forward passes
+    const int nfwd = (sq || ab) ? P - 1 : P;  // stages that hold forward passes
     if (p.linear()) {
         CItem c{};
         c.Ls = p.da;
@@ -287,6 +294,13 @@ void add_stage(Iteration &it, Prod &p) {
             b.pad = K_FWD;
             it.titems.push_back(b);
         }
+        return;
+    }
+    if (s == nfwd && ab) {
+        TItem f = titem(p, P - 1, p.xa, nullptr, 0);
+        f.other = p.xb;
+        f.pad = K_FUSED_AB;
+        it.titems.push_back(f);
         return;
     }
     if (s == nfwd) {
@@ -337,7 +351,8 @@ void finalize_iteration(Iteration &it) {
                shape_key(it.titems[j].d) == shape_key(it.titems[i].d)) {
             const TItem &t = it.titems[j];
             it.titems[j].tile0 = tiles;
-            tiles += ((uint64_t)1 << t.log_n) >> (t.d.log_r + t.d.log_c);
+            // fused A+B items run on half-width tiles
+            tiles += ((uint64_t)1 << t.log_n) >> (t.d.log_r + t.d.log_c - (t.pad == K_FUSED_AB ? 1 : 0));
             lds_log = std::max(lds_log, t.d.log_r + t.d.log_c);
             j++;
         }
@@ -386,6 +401,7 @@ int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c
         if (g.kind == K_FWD) rc = launch_tgroup<K_FWD>(g, dev_t, tw, st);
         else if (g.kind == K_INV) rc = launch_tgroup<K_INV>(g, dev_t, tw, st);
         else if (g.kind == K_FUSED) rc = launch_tgroup<K_FUSED>(g, dev_t, tw, st);
+        else if (g.kind == K_FUSED_AB) rc = launch_tgroup<K_FUSED_AB>(g, dev_t, tw, st);
         else {
             const CItem *items = dev_c + g.first;
             const int n = (int)g.count;

@@ -22,14 +22,14 @@
 
 namespace gnt {
 
-enum ItemKind : int { K_FWD = 0, K_INV = 1, K_FUSED = 2 };
+enum ItemKind : int { K_FWD = 0, K_INV = 1, K_FUSED = 2, K_FUSED_AB = 3 };
 
 // One tile-pass over one array (device descriptor).
 struct TItem {
-    uint64_t *x;            // FWD/INV: array in place; FUSED: B's array (source)
+    uint64_t *x;            // FWD/INV: array in place; FUSED: B's array (source); FUSED_AB: A's array
     const uint64_t *limbs;  // pack source for a first pass, else null
     uint64_t *xa;           // FUSED: A's transform and output
-    const uint64_t *other;  // FUSED: A (null squares)
+    const uint64_t *other;  // FUSED: A (null squares); FUSED_AB: B's array
     uint64_t *Ls, *Hs;      // limb sums (last inverse pass), else null
     uint64_t n_limbs_in;
     uint64_t n_coeffs;
@@ -87,7 +87,7 @@ bool prod_init(Prod &p);
 
 // Launch groups of one iteration.
 struct Group {
-    int kind;   // K_FWD/K_INV/K_FUSED, or 10+k for carry step k
+    int kind;   // K_FWD/K_INV/K_FUSED/K_FUSED_AB, or 10+k for carry step k
     int shape;  // shape_key of the group's tiles
     size_t first, count;  // range in the iteration's item table
     uint64_t tiles;       // total tiles of the launch

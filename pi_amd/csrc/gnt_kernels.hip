@@ -2,10 +2,12 @@
 // 64 KiB LDS tiles).  Integer-only: no MFMA, no floating point.
 //
 // One product (reference: crates/bignum/src/ntt.rs:806-865):
-//   k_fwd_pass       forward passes of A (pack fused into the first) and of
-//                    B except B's last pass
-//   k_last_fused     B's last forward pass, the pointwise product with A and
-//                    1/N, and the first inverse pass, in one tile round trip
+//   k_fwd_pass       forward passes of A and B (pack fused into the first)
+//                    except their last passes
+//   k_last_fused_ab  A's and B's last forward passes, the pointwise product
+//                    and 1/N, and the first inverse pass, in one tile round
+//                    trip (k_last_fused: the same for a square, or for a
+//                    single-pass plan after A's whole transform)
 //   k_inv_pass       the remaining inverse passes; the last one emits limb
 //                    sums L_j + 2^64 H_j instead of coefficients
 //   k_carry_*        exact carry: tile scan -> tile-carry scan -> sparse ripple
@@ -55,6 +57,16 @@ __global__ GNT_TBOUNDS void k_last_fused(const u64 *__restrict__ xb, const u64 *
     extern __shared__ u64 tile[];
     body_fused<X, G>(tile, make_geom<G>(d, blockIdx.x), xb, limbs, n_limbs_in, xa, a_other, log_n, tw, Ls, Hs,
                      n_coeffs, n_limbs);
+}
+
+// Last forward passes of A and B, pointwise, 1/N, first inverse pass; result
+// to xa (multiplies with plans of two or more passes).
+// G is the plan's last-pass shape; the kernel works on half-width tiles of it.
+template <class G>
+__global__ GNT_TBOUNDS void k_last_fused_ab(u64 *xa, const u64 *__restrict__ xb, PassDesc d, int log_n, Tw tw) {
+    extern __shared__ u64 tile[];
+    using H = typename HalfGeom<G>::type;
+    if constexpr (HalfGeom<G>::ok) body_fused_ab<X, H>(tile, make_geom<H>(d, blockIdx.x), xa, xb, log_n, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,6 +248,23 @@ int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t 
         break;
         GNT_FOR_EACH_SHAPE(FUSED_CASE)
 #undef FUSED_CASE
+        default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_last_fused_ab(const Plan &pl, u64 *xa, const u64 *xb, const Tw &tw, hipStream_t st) {
+    if (pl.npass < 2) return -1;
+    const PassDesc d = pl.pass[pl.npass - 1];
+    switch (shape_key(d)) {
+#define FAB_CASE(LR, LC, S)                                                                                        \
+    case shape_key(LR, LC, S):                                                                                     \
+        if (!HalfGeom<Geom<LR, LC, S>>::ok) return -1;                                                            \
+        hipLaunchKernelGGL((k_last_fused_ab<Geom<LR, LC, S>>), dim3(2 * pass_grid(d, pl.log_n).x), dim3(TBLOCK),  \
+                           pass_lds(d), st, xa, xb, d, pl.log_n, tw);                                              \
+        break;
+        GNT_FOR_EACH_SHAPE(FAB_CASE)
+#undef FAB_CASE
         default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -21,6 +21,9 @@ int launch_forward(const Plan &pl, int npasses, uint64_t *x, const uint64_t *lim
 int launch_last_fused(const Plan &pl, const uint64_t *xb, const uint64_t *limbs_b, size_t nb, uint64_t *xa,
                       bool square, const Tw &tw, uint64_t *Ls, uint64_t *Hs, size_t n_coeffs, size_t n_limbs,
                       hipStream_t st);
+// Multiply with a plan of two or more passes: last forward pass of A (xa) and
+// of B (xb), pointwise, 1/N, first inverse pass; result in xa.
+int launch_last_fused_ab(const Plan &pl, uint64_t *xa, const uint64_t *xb, const Tw &tw, hipStream_t st);
 // Inverse passes npass-2 .. 0; pass 0 writes limb sums Ls/Hs (n_limbs each).
 int launch_inverse_rest(const Plan &pl, uint64_t *x, const Tw &tw, uint64_t *Ls, uint64_t *Hs, size_t n_coeffs,
                         size_t n_limbs, hipStream_t st);

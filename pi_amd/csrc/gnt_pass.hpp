@@ -334,6 +334,58 @@ GL_HD void body_fused(u64 *tile, const G &g, const u64 *xb, const u64 *limbs, si
     else tile_store<X, G>(tile, g, xa);
 }
 
+// Last forward pass of A and of B (both read after their earlier passes), the
+// pointwise product, 1/N and the first inverse pass, result to xa: one tile
+// round trip instead of a separate pass over A.  G is a half-width tile
+// (2^(TILE_LOG-1) elements) of the plan's contiguous last pass, so A's and
+// B's tiles share the LDS of one full tile: tile[0, n) holds A's, tile[n, 2n)
+// B's.  Plans of two or more passes (no pass twiddle on the last pass).
+template <class X, class G, int I>
+GL_HD void tile_fwd_pair(u64 *tile, const G &g, const Tw &tw) {
+    if constexpr (I < Sub<G::LOGR>::count) {
+        const int ntask = G::tile_n >> Sub<G::LOGR>::log(I);
+        GNT_FOR(X, L, 2 * ntask) {
+            const bool b = L >= ntask;
+            fwd_task<G, I>(tile + (b ? G::tile_n : 0), g, b ? L - ntask : L, tw);
+        }
+        X::sync();
+        tile_fwd_pair<X, G, I + 1>(tile, g, tw);
+    }
+}
+
+template <class X, class G>
+GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_n, const Tw &tw) {
+    u64 *ta = tile, *tb = tile + G::tile_n;
+    GNT_FOR(X, idx, 2 * G::tile_n) {
+        const bool b = idx >= G::tile_n;
+        const int k = b ? idx - G::tile_n : idx;
+        int n, c;
+        tile_rc<G>(k, n, c);
+        const u32 a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
+        (b ? tb : ta)[tix(g, n, c)] = b ? xb[a] : xa[a];
+    }
+    X::sync();
+    tile_fwd_pair<X, G, 0>(tile, g, tw);
+    const int e = 192 - log_n;
+    GNT_FOR(X, idx, G::tile_n) {
+        int n, c;
+        tile_rc<G>(idx, n, c);
+        const int t = tix(g, n, c);
+        ta[t] = gl::mul_pow2(gl::mul(ta[t], tb[t]), e);
+    }
+    X::sync();
+    tile_inv<X, G, Sub<G::LOGR>::count - 1>(ta, g, tw);
+    tile_store<X, G>(ta, g, xa);
+}
+
+// Half-width geometry of a full-tile contiguous pass shape (the fused A+B
+// tiles); FullG itself when the shape cannot be halved (never launched then).
+template <class FullG>
+struct HalfGeom {
+    static constexpr bool ok = !FullG::strided && FullG::log_c >= 1 && FullG::LOGR + FullG::log_c == TILE_LOG;
+    using type = Geom<FullG::LOGR, ok ? FullG::log_c - 1 : FullG::log_c, false>;
+};
+
 // Carry-lookahead pairs: bit 0 = generate, bit 1 = propagate.  `lo` covers
 // the lower limbs, `hi` the limbs just above; identity is 2 (g=0, p=1).
 GL_HD u32 gp_combine(u32 lo, u32 hi) {

@@ -16,20 +16,30 @@ namespace gnt {
 
 int set_error(int code, const char *msg);
 
-// Idle device memory a pool keeps for reuse.
+// Idle device memory a pool keeps for reuse (buffers above SLAB_PIECE_MAX).
 constexpr size_t POOL_IDLE_MAX = (size_t)32 << 30;
+// Buffers up to SLAB_PIECE_MAX are carved from SLAB_BYTES slabs and always
+// kept for reuse: a level of binary splitting holds tens of thousands of small
+// buffers (lengths, carry flags, small products), which would otherwise cost
+// one hipMalloc and one hipFree each.
+constexpr size_t SLAB_PIECE_MAX = (size_t)4 << 20;
+constexpr size_t SLAB_BYTES = (size_t)64 << 20;
 
 class Pool {
 public:
     static size_t size_class(size_t bytes);
     void *acquire(size_t bytes);
     void release(void *p, size_t bytes);
-    void trim();
+    void trim();         // give idle large buffers back to the device
+    void release_all();  // everything, slabs included (nothing may be in use)
     size_t live_bytes = 0;    // allocated from the device (pooled or in use)
     size_t pooled_bytes = 0;  // of which idle in the free lists
 private:
     std::mutex mu;
     std::map<size_t, std::vector<void *>> free_;
+    std::vector<void *> slabs_;
+    char *slab_cur_ = nullptr;
+    size_t slab_left_ = 0;
 };
 
 struct Ctx {

@@ -51,6 +51,17 @@ void k_last_fused(const PassDesc &d, int log_n, const u64 *xb, const u64 *limbs,
                           n_coeffs, n_limbs);
 }
 
+template <class G>
+void k_last_fused_ab(const PassDesc &d, int log_n, u64 *xa, const u64 *xb) {
+    using H = typename HalfGeom<G>::type;
+    if constexpr (HalfGeom<G>::ok) {
+        std::vector<u64> tile(2 * H::tile_n);
+        const size_t grid = ((size_t)1 << log_n) / H::tile_n;
+        for (size_t t = 0; t < grid; t++)
+            body_fused_ab<HX, H>(tile.data(), make_geom<H>(d, t), xa, xb, log_n, tw());
+    }
+}
+
 #define DISPATCH(D, CALL)                                            \
     switch (shape_key(D)) {                                          \
         GNT_FOR_EACH_SHAPE(CALL)                                     \
@@ -149,13 +160,20 @@ size_t emu_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
     std::vector<u64> Ls(nl), Hs(nl);
     const int P = pl.npass;
     // the same launch sequence as Ctx::enqueue_product
+    const bool ab = !sqr && P > 1;
     if (sqr) {
         if (P > 1) forward(pl, P - 1, xa.data(), a, na);
     } else {
-        forward(pl, P, xa.data(), a, na);
+        forward(pl, ab ? P - 1 : P, xa.data(), a, na);
         if (P > 1) forward(pl, P - 1, xb.data(), b, nb);
     }
-    {
+    if (ab) {
+        const PassDesc &d = pl.pass[P - 1];
+#define F(LR, LC, S) \
+    case shape_key(LR, LC, S): k_last_fused_ab<Geom<LR, LC, S>>(d, log_n, xa.data(), xb.data()); break;
+        DISPATCH(d, F)
+#undef F
+    } else {
         const PassDesc &d = pl.pass[P - 1];
         const bool only = P == 1;
         const u64 *bsrc = sqr ? xa.data() : xb.data();
