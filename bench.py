@@ -178,7 +178,7 @@ def main():
     value = limbs / (ms_per_step * 1e-3) / 1e6
     bytes_alg = b_sqr(n, log_n) if args.op == "sqr" else b_mul(n, n, log_n)
     achieved = bytes_alg / (t_prod_ms * 1e-3) / 1e9
-    # the dominant single launch: k_last_fused reads B's and A's arrays and
+    # the dominant single launch: the fused kernel reads A's and B's arrays and
     # writes A's (a square reads and writes one array): 24N (16N) bytes
     fused_ms = phases[1] / prods
     fused_bytes = (16 if args.op == "sqr" else 24) * (1 << log_n)
@@ -215,7 +215,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_last_fused (B's last forward pass + pointwise + A's first inverse pass)",
+            "kernel": ("k_last_fused_ab (A's and B's last forward passes + pointwise + first inverse pass)"
+                       if args.op == "mul" else "k_last_fused (last forward pass + pointwise square + first inverse pass)"),
             "achieved": round(fused_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -25,8 +25,28 @@ HIP_FLAGS = [
 ]
 
 
+RESOURCES = os.path.join(HERE, "kernel_resources.json")
+
+
+def parse_resource_remarks(text: str) -> dict:
+    """Kernel -> resource figures from clang's kernel-resource-usage remarks."""
+    import re
+    out, cur = {}, None
+    for line in text.splitlines():
+        m = re.search(r"remark: Function Name: (\S+)", line)
+        if m:
+            cur = out.setdefault(m.group(1), {})
+            continue
+        m = re.search(r"remark:\s+([A-Za-z ]+?)(?: \[[^\]]*\])?: (\w+) \[-Rpass-analysis", line)
+        if m and cur is not None:
+            v = m.group(2)
+            cur[m.group(1).strip()] = int(v) if v.isdigit() else v
+    return out
+
+
 def build(verbose: bool = False) -> str:
     objs = []
+    resources = {}
     os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
     procs = []
     for src in SOURCES:
@@ -34,7 +54,7 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         obj = os.path.join(HERE, "_obj", src.replace(".hip", ".o"))
-        cmd = [HIPCC, *HIP_FLAGS, "-c", path, "-o", obj]
+        cmd = [HIPCC, *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), src))
@@ -46,10 +66,12 @@ def build(verbose: bool = False) -> str:
         if p.returncode != 0:
             failed.append(src)
             sys.stderr.write(text)
-        elif verbose and text.strip():
-            print(text)
+        resources.update(parse_resource_remarks(text))
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
+    import json
+    with open(RESOURCES, "w") as f:
+        json.dump(resources, f, indent=0, sort_keys=True)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     subprocess.run(cmd, check=True)

@@ -217,9 +217,10 @@ int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64
     int rc = 0;
     if (sqr) {
         if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, tw(), stream);
+    } else if (ab) {
+        rc = launch_forward_pair(pl, P - 1, xa, da, na, xb, db, nb, tw(), stream);
     } else {
-        rc = launch_forward(pl, ab ? P - 1 : P, xa, da, na, tw(), stream);
-        if (!rc && P > 1) rc = launch_forward(pl, P - 1, xb, db, nb, tw(), stream);
+        rc = launch_forward(pl, P, xa, da, na, tw(), stream);
     }
     if (timing) (void)hipEventRecord(ev[1], stream);
     if (!rc && ab) {

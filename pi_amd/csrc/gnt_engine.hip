@@ -16,8 +16,6 @@ using X = DevExec<TBLOCK>;
 constexpr int CARRY_LPT = 4;                  // limbs per thread
 constexpr int CARRY_TILE = BLOCK * CARRY_LPT;  // limbs per carry tile
 constexpr int SCAN_BLOCK = 1024;
-// resident tiles per launch: every CU holds two 64 KiB tiles
-constexpr uint64_t MAX_GRID = 256 * 2 * 4;
 
 size_t ceil_log2(size_t x) {
     size_t l = 0;
@@ -37,23 +35,42 @@ __device__ __forceinline__ int find_item(const T *__restrict__ items, int n, uin
     return lo;
 }
 
+// A descriptor every lane of the wave holds the same copy of: read word by word
+// through readfirstlane so the compiler keeps it in scalar registers (the
+// vector registers are all needed by the tile body).
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "descriptor of 32-bit words");
+    T r;
+    const int *src = reinterpret_cast<const int *>(&v);
+    int *dst = reinterpret_cast<int *>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return r;
+}
+
 template <class G, int KIND>
 __global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tpass(
     const TItem *__restrict__ items, int n_items, uint64_t total, Tw tw) {
+    // One tile per workgroup, as the single-product kernels run: a loop over
+    // tiles inside the workgroup lets the compiler keep per-thread address
+    // arithmetic alive across iterations, which spills the tile bodies.
     extern __shared__ uint64_t tile[];
-    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-        const TItem it = items[find_item(items, n_items, t)];
-        const G g = make_geom<G>(it.d, t - it.tile0);  // (FUSED_AB builds its half-width geometry)
-        if (KIND == K_FWD) body_fwd<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, tw);
-        else if (KIND == K_INV) body_inv<X, G>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs);
-        else if (KIND == K_FUSED_AB) {
-            using H = typename HalfGeom<G>::type;  // half-width tiles of the last-pass shape
-            if constexpr (HalfGeom<G>::ok) body_fused_ab<X, H>(tile, make_geom<H>(it.d, t - it.tile0), it.x, it.other, it.log_n, tw);
-        }
-        else body_fused<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls,
-                              it.Hs, it.n_coeffs, it.n_limbs);
-        __syncthreads();  // the tile buffer is reused by the next iteration
+    const uint64_t t = blockIdx.x;
+    if (t >= total) return;
+    const TItem it = wave_uniform(items[find_item(items, n_items, t)]);
+    if constexpr (KIND == K_FUSED_AB) {
+        using H = typename HalfGeom<G>::type;  // half-width tiles of the last-pass shape
+        if constexpr (HalfGeom<G>::ok)
+            body_fused_ab<X, H>(tile, make_geom<H>(it.d, t - it.tile0), it.x, it.other, it.log_n, tw);
+        return;
     }
+    const G g = make_geom<G>(it.d, t - it.tile0);
+    if (KIND == K_FWD) body_fwd<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, tw);
+    else if (KIND == K_INV) body_inv<X, G, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0);
+    else if (KIND == K_INV_SUMS) body_inv<X, G, true>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs);
+    else body_fused<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls, it.Hs,
+                          it.n_coeffs, it.n_limbs);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ctiles(const CItem *__restrict__ items, int n_items, uint64_t total) {
@@ -317,7 +334,7 @@ void add_stage(Iteration &it, Prod &p) {
         const int pass = P - 2 - inv_stage;
         TItem v = titem(p, pass, p.xa, nullptr, 0);
         if (pass == 0) { v.Ls = Ls; v.Hs = Hs; }
-        v.pad = K_INV;
+        v.pad = pass == 0 ? K_INV_SUMS : K_INV;
         it.titems.push_back(v);
         return;
     }
@@ -379,7 +396,8 @@ void finalize_iteration(Iteration &it) {
 template <int KIND>
 static int launch_tgroup(const Group &g, const TItem *dev, const Tw &tw, hipStream_t st) {
     const size_t lds = ((size_t)1 << g.lds_log) * 8;
-    dim3 grid((unsigned)std::min<uint64_t>(g.tiles, MAX_GRID));
+    if (g.tiles >= ((uint64_t)1 << 31)) return -1;
+    dim3 grid((unsigned)g.tiles);
     const TItem *items = dev + g.first;
     const int n = (int)g.count;
     switch (g.shape) {
@@ -400,6 +418,7 @@ int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c
         int rc = 0;
         if (g.kind == K_FWD) rc = launch_tgroup<K_FWD>(g, dev_t, tw, st);
         else if (g.kind == K_INV) rc = launch_tgroup<K_INV>(g, dev_t, tw, st);
+        else if (g.kind == K_INV_SUMS) rc = launch_tgroup<K_INV_SUMS>(g, dev_t, tw, st);
         else if (g.kind == K_FUSED) rc = launch_tgroup<K_FUSED>(g, dev_t, tw, st);
         else if (g.kind == K_FUSED_AB) rc = launch_tgroup<K_FUSED_AB>(g, dev_t, tw, st);
         else {

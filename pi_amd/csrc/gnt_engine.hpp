@@ -22,7 +22,7 @@
 
 namespace gnt {
 
-enum ItemKind : int { K_FWD = 0, K_INV = 1, K_FUSED = 2, K_FUSED_AB = 3 };
+enum ItemKind : int { K_FWD = 0, K_INV = 1, K_FUSED = 2, K_FUSED_AB = 3, K_INV_SUMS = 4 };
 
 // One tile-pass over one array (device descriptor).
 struct TItem {
@@ -87,7 +87,7 @@ bool prod_init(Prod &p);
 
 // Launch groups of one iteration.
 struct Group {
-    int kind;   // K_FWD/K_INV/K_FUSED/K_FUSED_AB, or 10+k for carry step k
+    int kind;   // K_FWD/K_INV/K_INV_SUMS/K_FUSED/K_FUSED_AB, or 10+k for carry step k
     int shape;  // shape_key of the group's tiles
     size_t first, count;  // range in the iteration's item table
     uint64_t tiles;       // total tiles of the launch

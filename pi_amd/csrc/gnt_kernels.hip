@@ -38,12 +38,25 @@ __global__ GNT_TBOUNDS void k_fwd_pass(u64 *__restrict__ x, const u64 *__restric
     body_fwd<X, G>(tile, make_geom<G>(d, blockIdx.x), x, limbs, n_limbs, tw);
 }
 
-// Inverse pass; with Ls set it is the last one and writes limb sums.
+// The same forward pass over two operands in one launch: blocks [0, tiles)
+// work on A, [tiles, 2 tiles) on B.
 template <class G>
+__global__ GNT_TBOUNDS void k_fwd_pass_pair(u64 *__restrict__ xa, const u64 *__restrict__ la, size_t na,
+                                            u64 *__restrict__ xb, const u64 *__restrict__ lb, size_t nb,
+                                            unsigned tiles, PassDesc d, Tw tw) {
+    extern __shared__ u64 tile[];
+    const bool b = blockIdx.x >= tiles;
+    const unsigned t = b ? blockIdx.x - tiles : blockIdx.x;
+    if (b) body_fwd<X, G>(tile, make_geom<G>(d, t), xb, lb, nb, tw);
+    else body_fwd<X, G>(tile, make_geom<G>(d, t), xa, la, na, tw);
+}
+
+// Inverse pass; SUMS: the last one, writing limb sums.
+template <class G, bool SUMS>
 __global__ GNT_TBOUNDS void k_inv_pass(u64 *__restrict__ x, PassDesc d, Tw tw, u64 *__restrict__ Ls,
                                        u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
     extern __shared__ u64 tile[];
-    body_inv<X, G>(tile, make_geom<G>(d, blockIdx.x), x, tw, Ls, Hs, n_coeffs, n_limbs);
+    body_inv<X, G, SUMS>(tile, make_geom<G>(d, blockIdx.x), x, tw, Ls, Hs, n_coeffs, n_limbs);
 }
 
 // Last forward pass of B (read from xb, or packed from limbs), pointwise with
@@ -232,6 +245,26 @@ int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+int launch_forward_pair(const Plan &pl, int npasses, u64 *xa, const u64 *la, size_t na, u64 *xb, const u64 *lb,
+                        size_t nb, const Tw &tw, hipStream_t st) {
+    for (int p = 0; p < npasses; p++) {
+        const PassDesc d = pl.pass[p];
+        const unsigned tiles = pass_grid(d, pl.log_n).x;
+        const u64 *sa = p == 0 ? la : nullptr, *sb = p == 0 ? lb : nullptr;
+        switch (shape_key(d)) {
+#define FWD2_CASE(LR, LC, S)                                                                                     \
+    case shape_key(LR, LC, S):                                                                                   \
+        hipLaunchKernelGGL((k_fwd_pass_pair<Geom<LR, LC, S>>), dim3(2 * tiles), dim3(TBLOCK), pass_lds(d), st, xa, \
+                           sa, na, xb, sb, nb, tiles, d, tw);                                                    \
+        break;
+            GNT_FOR_EACH_SHAPE(FWD2_CASE)
+#undef FWD2_CASE
+            default: return -1;
+        }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t nb, u64 *xa, bool square,
                       const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs, hipStream_t st) {
     const PassDesc d = pl.pass[pl.npass - 1];
@@ -279,8 +312,12 @@ int launch_inverse_rest(const Plan &pl, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, 
         switch (shape_key(d)) {
 #define INV_CASE(LR, LC, S)                                                                                     \
     case shape_key(LR, LC, S):                                                                                  \
-        hipLaunchKernelGGL((k_inv_pass<Geom<LR, LC, S>>), pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, \
-                           d, tw, ls, hs, n_coeffs, n_limbs);                                                   \
+        if (ls)                                                                                                 \
+            hipLaunchKernelGGL((k_inv_pass<Geom<LR, LC, S>, true>), pass_grid(d, pl.log_n), dim3(TBLOCK),       \
+                               pass_lds(d), st, x, d, tw, ls, hs, n_coeffs, n_limbs);                           \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_inv_pass<Geom<LR, LC, S>, false>), pass_grid(d, pl.log_n), dim3(TBLOCK),      \
+                               pass_lds(d), st, x, d, tw, ls, hs, n_coeffs, n_limbs);                           \
         break;
             GNT_FOR_EACH_SHAPE(INV_CASE)
 #undef INV_CASE

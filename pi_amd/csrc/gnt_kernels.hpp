@@ -15,6 +15,9 @@ namespace gnt {
 // zero tail) instead of reading x.
 int launch_forward(const Plan &pl, int npasses, uint64_t *x, const uint64_t *limbs, size_t n_limbs, const Tw &tw,
                    hipStream_t st);
+// Forward passes 0 .. npasses-1 of two operands (same plan), one launch per pass.
+int launch_forward_pair(const Plan &pl, int npasses, uint64_t *xa, const uint64_t *la, size_t na, uint64_t *xb,
+                        const uint64_t *lb, size_t nb, const Tw &tw, hipStream_t st);
 // Last forward pass of B + pointwise with A + 1/N + first inverse pass, result
 // in xa.  square: B is A (xb == xa) and the pointwise product is A*A.  For a
 // single-pass plan B is packed from limbs_b and the output is limb sums.

@@ -307,14 +307,15 @@ GL_HD void body_fwd(u64 *tile, const G &g, u64 *x, const u64 *limbs, size_t n_li
     tile_store<X, G>(tile, g, x);
 }
 
-// Inverse pass; with Ls set it is the last one and writes limb sums.
-template <class X, class G>
+// Inverse pass; SUMS: it is the last one and writes limb sums (a compile-time
+// choice, so each kernel carries only one of the two tails).
+template <class X, class G, bool SUMS>
 GL_HD void body_inv(u64 *tile, const G &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
                     size_t n_limbs) {
     tile_load<X, G>(tile, g, x, nullptr, 0);
     if constexpr (G::strided) tile_twiddle<X, G, true>(tile, g, tw);
     tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
-    if (Ls) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    if constexpr (SUMS) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
     else tile_store<X, G>(tile, g, x);
 }
 

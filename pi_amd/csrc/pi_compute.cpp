@@ -528,23 +528,31 @@ struct Decimal {
     }
 
     // n digits of f / 2^F into out[0..n)
+    std::string stats;  // per level: nodes, products on the GPU, seconds (JSON)
+
     void run(Frac root, std::string &out) {
         std::vector<Frac> level;
         level.push_back(std::move(root));
         std::vector<Frac> leaves;
+        stats = "[";
         while (!level.empty()) {
+            auto t0 = Clock::now();
+            const size_t gpu0 = M.gpu_products;
             std::vector<Frac> next;
             std::vector<size_t> split;
             for (size_t i = 0; i < level.size(); i++)
                 if (level[i].n <= leaf_digits) leaves.push_back(std::move(level[i]));
                 else split.push_back(i);
+            if (split.empty()) break;
             // powers first (sequential: they share squares), then the level's products
             for (size_t i : split) p10(level[i].n / 2);
+            const double t_pow = since(t0);
             std::vector<Mpz> g(split.size());
             std::vector<Mul::Job> jobs;
             for (size_t k = 0; k < split.size(); k++)
                 jobs.push_back(Mul::Job{&g[k], &level[split[k]].f, p10(level[split[k]].n / 2)});
             M.batch(jobs);
+            const double t_mul = since(t0) - t_pow;
             next.resize(2 * split.size());
             parallel_for(split.size(), threads, [&](size_t k) {
                 Frac &x = level[split[k]];
@@ -568,16 +576,27 @@ struct Decimal {
                 lo.at = x.at + h;
                 lo.err = x.err + 1;
             });
+            char buf[200];
+            std::snprintf(buf, sizeof buf, "%s{\"nodes\": %zu, \"gpu_products\": %zu, \"s_powers\": %.4f, \"s_products\": %.4f, \"s\": %.4f}",
+                          stats.size() > 1 ? ", " : "", split.size(), M.gpu_products - gpu0, t_pow, t_mul, since(t0));
+            stats += buf;
             level = std::move(next);
         }
         // every power the leaves need exists before the threads start: p10()
         // inserts into the map, which is not safe to do concurrently
+        auto t0 = Clock::now();
         for (const Frac &x : leaves) p10(x.n);
         parallel_for(leaves.size(), threads, [&](size_t i) { leaf(leaves[i], out); });
+        char buf[120];
+        std::snprintf(buf, sizeof buf, "%s{\"leaves\": %zu, \"s\": %.4f}]", stats.size() > 1 ? ", " : "", leaves.size(),
+                      since(t0));
+        stats += buf;
     }
 };
 
 // floor(pi * 10^(D-1)) as D characters with every large product on the GPU.
+std::string g_decimal_stats = "[]";  // last conversion's per-level timings (JSON)
+
 // D characters of pi from pi_fix ~ pi 2^prec (error bound err units).
 std::string digits_of(Mul &M, const Mpz &pi_fix, uint64_t digits, uint64_t prec, int threads, unsigned err) {
     Mpz ip;
@@ -594,6 +613,7 @@ std::string digits_of(Mul &M, const Mpz &pi_fix, uint64_t digits, uint64_t prec,
         root.at = 1;
         root.err = err;
         dec.run(std::move(root), out);
+        g_decimal_stats = dec.stats;
     }
     return out;
 }
@@ -931,13 +951,13 @@ int main(int argc, char **argv) {
                 "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
                 "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_shutdown\": %.3f, \"t_total\": %.3f, "
                 "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"fa\": \"%s\", \"fa_gpu_products\": %zu, "
-                "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"levels\": %s}\n",
+                "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"decimal_levels\": %s, \"levels\": %s}\n",
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
                 !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"), o.threads, o.gpu_min_limbs,
                 d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
                 t_download, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
-                M.host_products, M.gpu_limbs, lv.c_str());
+                M.host_products, M.gpu_limbs, g_decimal_stats.c_str(), lv.c_str());
     (void)t_levels;
     return 0;
 }

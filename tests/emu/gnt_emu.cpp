@@ -38,7 +38,8 @@ void k_inv_pass(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_t n
     std::vector<u64> tile(G::tile_n);
     const size_t grid = ((size_t)1 << log_n) / G::tile_n;
     for (size_t t = 0; t < grid; t++)
-        body_inv<HX, G>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
+        if (Ls) body_inv<HX, G, true>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
+        else body_inv<HX, G, false>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
 }
 
 template <class G>

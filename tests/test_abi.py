@@ -86,3 +86,23 @@ def test_plan_overrides_and_cache(tmp_path):
     finally:
         for n in (17, 22, 24):
             gnt.set_plan(n)
+
+
+def test_transform_kernels_do_not_spill():
+    """Every transform kernel keeps its tile within the register budget that
+    two 64 KiB tiles per CU allow (8 waves per SIMD, <= 64 VGPRs) and spills
+    nothing to scratch.  Figures: the compiler's resource remarks, recorded by
+    the build (pi_amd/kernel_resources.json)."""
+    import json
+    path = os.path.join(ROOT, "pi_amd", "kernel_resources.json")
+    if not os.path.exists(path):
+        pytest.skip("library built without resource remarks")
+    res = json.load(open(path))
+    tk = {k: v for k, v in res.items()
+          if any(s in k for s in ("k_fwd_pass", "k_inv_pass", "k_last_fused", "k_tpass"))}
+    assert len(tk) > 50
+    bad = {k: v for k, v in tk.items() if v.get("VGPRs Spill", 0) or v.get("ScratchSize", 0)}
+    assert not bad, f"spilling kernels: {sorted(bad)[:5]}"
+    # every kernel that runs a full tile holds 8 waves per SIMD
+    low = {k: v["Occupancy"] for k, v in tk.items() if v.get("VGPRs", 0) and v.get("Occupancy", 8) < 8}
+    assert not low, low
