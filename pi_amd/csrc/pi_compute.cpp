@@ -113,10 +113,11 @@ struct Options {
     int threads = 0;
     bool host_merge = false;      // merge the upper tree on the host too (comparison run)
     bool fa_gmp = false;          // final assembly and radix conversion by GMP (comparison run)
-    double fa_min_limbs = 1024;   // final-assembly / conversion products on the GPU from this size
+    double fa_min_limbs = 8192;   // final-assembly / conversion products on the GPU from this size
     int device = 0;
     std::string plans;            // plan cache from the autotuner
     std::string algorithm = "chudnovsky";
+    bool shutdown = false;        // free the context's device memory before exiting
 };
 
 struct Driver {
@@ -150,7 +151,10 @@ struct Driver {
         return id;
     }
 
-    void run_host_nodes(int threads) {
+    // Host nodes on `threads` threads; with upload set, each node's P, |T|, Q
+    // go to the device as soon as it is done (overlapping the other nodes'
+    // compute) and the host copies are dropped.
+    void run_host_nodes(int threads, bool upload_done = false) {
         std::vector<int> order = host_nodes;
         std::sort(order.begin(), order.end(), [&](int x, int y) { return nodes[x].b - nodes[x].a > nodes[y].b - nodes[y].a; });
         std::atomic<size_t> next{0};
@@ -163,6 +167,15 @@ struct Driver {
                 n.Q.reset(new Mpz);
                 n.T.reset(new Mpz);
                 bs_host(n.a, n.b, &*n.P, &*n.Q, &*n.T, order[i] != 0);
+                if (upload_done && order[i] != 0) {
+                    if (n.T->sign() >= 0) die("T(a,b) expected negative", "");
+                    n.bp = upload(*n.P);
+                    n.bq = upload(*n.Q);
+                    n.bt = upload(*n.T);
+                    n.P.reset();
+                    n.Q.reset();
+                    n.T.reset();
+                }
             }
         };
         std::vector<std::thread> pool;
@@ -764,12 +777,20 @@ std::string final_assembly_gmp(const Mpz &Q, const Mpz &Tabs, uint64_t digits, u
 void usage() {
     std::fprintf(stderr,
                  "usage: pi_compute DIGITS OUT_FILE [--algorithm chudnovsky|gauss-legendre] [--gpu-min-limbs L]\n"
-                 "                  [--threads T] [--host-merge]\n"
+                 "                  [--threads T] [--host-merge] [--shutdown]\n"
                  "                     [--fa-gmp] [--fa-min-limbs L] [--device D] [--plans CACHE.json]\n");
     std::exit(2);
 }
 
 }  // namespace
+
+// End of a run: output flushed; the runtime's exit releases what the
+// context still holds.
+int finish(const Options &) {
+    std::fflush(stdout);
+    std::fflush(stderr);
+    return 0;
+}
 
 void write_digits(FILE *f, const std::string &s, uint64_t digits, const std::string &path) {
     std::fwrite(s.data(), 1, 1, f);
@@ -808,14 +829,15 @@ int run_gauss_legendre(const Options &o, uint64_t prec, FILE *f, Clock::time_poi
     t0 = Clock::now();
     write_digits(f, s, o.digits, o.out);
     const double t_write = since(t0);
-    if (ctx) gnt_shutdown(ctx);
+    if (ctx && o.shutdown) gnt_shutdown(ctx);  // else the process exit returns the device memory
+    else if (ctx) gnt_sync(ctx);
     std::printf("{\"digits\": %llu, \"algorithm\": \"gauss-legendre\", \"prec_bits\": %llu, \"iterations\": %d, "
                 "\"threads\": %d, \"products\": \"%s\", \"t_init\": %.3f, \"t_agm\": %.3f, \"t_decimal\": %.3f, "
                 "\"t_write\": %.3f, \"t_total\": %.3f, \"gpu_products\": %zu, \"host_products\": %zu, "
                 "\"gpu_operand_limbs\": %.0f}\n",
                 (unsigned long long)o.digits, (unsigned long long)prec, iters, o.threads, gpu ? "gpu" : "host", t_init,
                 t_agm, t_dec, t_write, since(t_start), M.gpu_products, M.host_products, M.gpu_limbs);
-    return 0;
+    return finish(o);
 }
 
 int main(int argc, char **argv) {
@@ -833,6 +855,7 @@ int main(int argc, char **argv) {
         else if (a == "--device" && i + 1 < argc) o.device = std::atoi(argv[++i]);
         else if (a == "--plans" && i + 1 < argc) o.plans = argv[++i];
         else if (a == "--algorithm" && i + 1 < argc) o.algorithm = argv[++i];
+        else if (a == "--shutdown") o.shutdown = true;
         else usage();
     }
     if (o.threads <= 0) {
@@ -872,7 +895,7 @@ int main(int argc, char **argv) {
         t_init = since(t0);
     }
     auto t0 = Clock::now();
-    d.run_host_nodes(o.threads);
+    d.run_host_nodes(o.threads, gpu_merge);  // merged on the device: uploads overlap the host work
     const double t_host = since(t0);
 
     Mpz Q, Tabs;
@@ -888,29 +911,18 @@ int main(int argc, char **argv) {
         __gmpz_neg(&Tabs, &*d.nodes[0].T);
     } else {
         t0 = Clock::now();
-        for (int id : d.host_nodes) {
-            Node &n = d.nodes[id];
-            if (n.T->sign() >= 0) die("T(a,b) expected negative", "");
-            n.bp = d.upload(*n.P);
-            n.bq = d.upload(*n.Q);
-            n.bt = d.upload(*n.T);
-            n.P.reset();
-            n.Q.reset();
-            n.T.reset();
-        }
-        t_upload = since(t0);
-        t0 = Clock::now();
         for (int depth = d.max_depth; depth >= 0; depth--) d.merge_level_device(depth);
         t_merge = since(t0);
         t0 = Clock::now();
         Node &root = d.nodes[0];
         size_t nq = Driver::buf_len(d.ctx, root.bq), nt = Driver::buf_len(d.ctx, root.bt);
-        std::vector<uint64_t> hq(nq), ht(nt);
         size_t got = 0;
-        check(gnt_buf_download(d.ctx, root.bq, hq.data(), nq, &got), "download");
-        check(gnt_buf_download(d.ctx, root.bt, ht.data(), nt, &got), "download");
-        Q.set_limbs(hq.data(), nq);
-        Tabs.set_limbs(ht.data(), nt);
+        __gmpz_realloc2(&Q, (unsigned long)(nq ? nq : 1) * 64);
+        __gmpz_realloc2(&Tabs, (unsigned long)(nt ? nt : 1) * 64);
+        check(gnt_buf_download(d.ctx, root.bq, Q.z.d, nq, &got), "download");
+        Q.z.size = (int)got;
+        check(gnt_buf_download(d.ctx, root.bt, Tabs.z.d, nt, &got), "download");
+        Tabs.z.size = (int)got;
         gnt_buf_release(d.ctx, root.bq);
         gnt_buf_release(d.ctx, root.bt);
         t_download = since(t0);
@@ -931,8 +943,12 @@ int main(int argc, char **argv) {
     t0 = Clock::now();
     write_digits(f, s, o.digits, o.out);
     const double t_write = since(t0);
+    // The process ends here and the runtime's exit releases the device
+    // memory; gnt_shutdown would first free the pools buffer by buffer
+    // (~0.3 s more at 10^8 digits).  Only the stream is drained.
     t0 = Clock::now();
-    if (d.ctx) gnt_shutdown(d.ctx);
+    if (d.ctx && o.shutdown) gnt_shutdown(d.ctx);
+    else if (d.ctx) gnt_sync(d.ctx);
     const double t_shutdown = since(t0);
     const double t_total = since(t_start);
     double t_levels = 0;
@@ -959,5 +975,5 @@ int main(int argc, char **argv) {
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
                 M.host_products, M.gpu_limbs, g_decimal_stats.c_str(), lv.c_str());
     (void)t_levels;
-    return 0;
+    return finish(o);
 }
