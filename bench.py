@@ -53,6 +53,21 @@ def operand(seed: int, n: int) -> np.ndarray:
     return x
 
 
+def operand_dist(dist: str, seed: int, n: int) -> np.ndarray:
+    """uniform limbs, all-ones limbs (the longest carry chains), or sparse:
+    each limb nonzero with probability 1/100 (decided by one stream, values
+    from another), top limb nonzero."""
+    if dist == "uniform":
+        return operand(seed, n)
+    if dist == "ones":
+        return np.full(n, np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+    pick = splitmix64(seed, n) % np.uint64(100) == 0
+    x = np.where(pick, splitmix64(seed + 1000, n), np.uint64(0)).astype(np.uint64)
+    if x[-1] == 0:
+        x[-1] = 1
+    return x
+
+
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (KiB per dispatch; FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
 # factors a plain 1 GiB device copy reads), keyed by (op, log2 N, radices).
@@ -90,6 +105,20 @@ def cpu_baseline(log_size: int, op: str, threads: int) -> dict:
         times.append(time.perf_counter() - t0)
         reps += 1
     t = float(np.median(times[1:] if len(times) > 1 else times))
+    # one thread on a quarter-size sample, and GMP's mpn_mul (one thread) beside it
+    n1 = n >> 2
+    a1, b1 = a[:n1].copy(), (a if op == "sqr" else b)[:n1].copy()
+    orc.set_threads(1)
+    orc.apply_host_knobs(1)
+    t0 = time.perf_counter()
+    orc.mul_mag_ntt(a1, b1)
+    t1 = time.perf_counter() - t0
+    orc.set_threads(threads)
+    gmp = None
+    if orc.gmp() is not None:
+        t0 = time.perf_counter()
+        orc.gmp_mul(a, b) if op == "mul" else orc.gmp_sqr(a)
+        gmp = round(2 * n / (time.perf_counter() - t0) / 1e6, 3)
     return {
         "value": round(2 * n / t / 1e6, 3),
         "unit": "Mlimbs/s",
@@ -97,7 +126,20 @@ def cpu_baseline(log_size: int, op: str, threads: int) -> dict:
         "kind": "port",
         "sample": f"{reps} x {op} 2^{log_size} x 2^{log_size} limbs (median of runs after the first), "
                   f"four-step threshold {orc.get_four_step_threshold()}",
+        "cpu_model": cpu_model(),
+        "one_thread": {"value": round(2 * n1 / t1 / 1e6, 3), "sample": f"1 x {op} 2^{log_size - 2} limbs"},
+        "gmp_mpn_1thread": {"value": gmp, "sample": f"1 x {op} 2^{log_size} limbs"} if gmp else None,
     }
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -109,10 +151,11 @@ def main():
     ap.add_argument("--op", choices=["mul", "sqr"], default="mul")
     ap.add_argument("--cpu-log-size", type=int, default=22, help="log2 limbs of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky", "c2"], default="single",
                     help="single: the metric (one 2^24 product per step); batched: 1024 uneven pairs per step; "
                          "serve: open-loop Poisson request stream through the serving loop; "
-                         "chudnovsky: pi to --digits digits end to end")
+                         "chudnovsky: pi to --digits digits end to end; "
+                         "c2: mul and sqr at 2^20 and 2^24 limbs over three operand distributions")
     ap.add_argument("--rates", default="100,200,400,800,1600", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--pairs", type=int, default=1024)
@@ -132,7 +175,7 @@ def main():
 
     if args.tune:
         lo, hi = (int(x) for x in args.tune.split(":"))
-        ctx = make_ctx(args)
+        ctx = gnt.Context(0)  # tuned from the built-in plans
         t0 = time.perf_counter()
         gnt.tune(ctx, lo, hi, reps=5, path=args.tune_out)
         ctx.close()
@@ -144,6 +187,8 @@ def main():
         return bench_serve(args)
     if args.workload == "chudnovsky":
         return bench_chudnovsky(args)
+    if args.workload == "c2":
+        return bench_c2(args)
 
     n = 1 << args.log_size
     a, b = operand(3, n), operand(4, n)
@@ -236,9 +281,7 @@ def main():
     }
     ctx.close()
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
-        line["cpu_baseline"] = cpu_baseline(min(args.cpu_log_size, args.log_size), args.op, threads)
+        line["cpu_baseline"] = cpu_baseline(min(args.cpu_log_size, args.log_size), args.op, host_threads())
     print(json.dumps(line), flush=True)
 
 
@@ -260,7 +303,7 @@ def bench_batched(args):
 
     pairs = batched_pairs(args.pairs)
     limbs = sum(a.size + b.size for a, b in pairs)
-    ctx = gnt.Context(0)
+    ctx = make_ctx(args)
     res = {}
     for label, graph, serial in (("batched_graph", True, False), ("batched_launches", False, False),
                                  ("per_product", False, True)):
@@ -301,7 +344,36 @@ def bench_batched(args):
                    "total_operand_limbs": limbs, "modes": res},
     }
     ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_batched(pairs, host_threads())
     print(json.dumps(line), flush=True)
+
+
+def host_threads() -> int:
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+
+
+def cpu_baseline_batched(pairs, threads: int, sample: int = 128) -> dict:
+    """The oracle's mul_mag routing (schoolbook / Karatsuba / NTT as
+    integer.rs:460-486 picks them) on the first `sample` pairs, run
+    concurrently on a pool of host threads as binary splitting's sibling
+    subtrees run under rayon."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as orc
+
+    orc.set_threads(1)
+    orc.apply_host_knobs(threads)
+    sub = pairs[:sample]
+    limbs = sum(a.size + b.size for a, b in sub)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda p: orc.mul_mag(p[0], p[1]), sub))
+    t = time.perf_counter() - t0
+    orc.set_threads(threads)
+    return {"value": round(limbs / t / 1e6, 3), "unit": "Mlimbs/s", "cores": threads, "kind": "port",
+            "sample": f"first {len(sub)} of the {len(pairs)} pairs ({limbs} operand limbs), one pair per thread",
+            "cpu_model": cpu_model()}
 
 
 def bench_serve(args):
@@ -492,6 +564,56 @@ def bench_chudnovsky(args):
         "levels": st["levels"],
     }
     print(json.dumps(line), flush=True)
+
+
+
+def bench_c2(args):
+    """Single products at 2^20 and 2^24 limbs, mul and sqr, uniform / all-ones /
+    sparse operands: device time per product (HIP events), Mlimbs/s and the
+    byte-model GB/s; every result checked (low limb and length)."""
+    from pi_amd import gnt
+    ctx = make_ctx(args)
+    rows = []
+    for log_size in (20, 24):
+        n = 1 << log_size
+        for dist in ("uniform", "ones", "sparse"):
+            a, b = operand_dist(dist, 3 if dist != "sparse" else 5, n), operand_dist(dist, 4 if dist != "sparse" else 6, n)
+            da, db = ctx.upload(a), ctx.upload(b)
+            for op in ("mul", "sqr"):
+                log_n, _ = gnt.plan_info(n, n)
+                step = (lambda: ctx.sqr_buf(da).release()) if op == "sqr" else (lambda: ctx.mul_buf(da, db).release())
+                for _ in range(2):
+                    step()
+                ctx.sync()
+                reps = 20 if log_size == 20 else 6
+                ctx.set_timing(True)
+                for _ in range(reps):
+                    step()
+                ctx.sync()
+                phases, prods = ctx.get_timing()
+                ctx.set_timing(False)
+                t_ms = phases[4] / prods
+                r = ctx.sqr_buf(da) if op == "sqr" else ctx.mul_buf(da, db)
+                out = r.download()
+                r.release()
+                bb = a if op == "sqr" else b
+                ok = out.size <= 2 * n and int(out[0]) == (int(a[0]) * int(bb[0])) % (1 << 64)
+                byt = b_sqr(n, log_n) if op == "sqr" else b_mul(n, n, log_n)
+                rows.append({"limbs": n, "op": op, "dist": dist, "device_ms": round(t_ms, 4),
+                             "Mlimbs_per_s": round(2 * n / (t_ms * 1e-3) / 1e6, 1),
+                             "model_GBps": round(byt / (t_ms * 1e-3) / 1e9, 1), "check": ok})
+            da.release()
+            db.release()
+    ctx.close()
+    top = [r for r in rows if r["limbs"] == 1 << 24 and r["op"] == "mul" and r["dist"] == "uniform"][0]
+    print(json.dumps({
+        "metric": "Goldilocks-NTT single products at 2^20 and 2^24 limbs (three distributions)",
+        "value": top["Mlimbs_per_s"], "unit": "Mlimbs/s", "n_gpus": 1, "steps": 6, "warmup": 2,
+        "ms_per_step": top["device_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic (SplitMix64: uniform seeds 3/4, sparse seeds 5/6, all-ones)",
+        "config": {"workload": "mul and sqr, 2^20 and 2^24 limbs per operand, inputs resident in HBM",
+                   "plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in", "rows": rows},
+    }), flush=True)
 
 
 if __name__ == "__main__":

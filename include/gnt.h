@@ -52,7 +52,9 @@ int gnt_shutdown(gnt_ctx *ctx);
  * trimmed of most-significant zero limbs (ntt.rs:712-715, 781-783);
  * out_cap >= na+nb.  na == 0 or nb == 0 gives *out_len = 0 (ntt.rs:807-809).
  * GNT_E_TOO_LARGE when 4*(na+nb) > 2^30 coefficients (the reference's cap is
- * 2^32, integer.rs:476-478; the caller splits with Karatsuba either way). */
+ * 2^32, integer.rs:476-478; the caller splits with Karatsuba either way).
+ * Thread-safe; while the serving loop runs (gnt_serve_start), calls from any
+ * thread are submitted to it, so concurrent callers are batched together. */
 int gnt_mul(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out,
             size_t out_cap, size_t *out_len);
 /* gnt_sqr replaces the reference's squares, which all go through mul_mag(a, a)

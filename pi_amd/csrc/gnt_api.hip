@@ -342,6 +342,30 @@ static int check_ctx(Ctx *c) {
 }
 
 // One product of host operands (b == nullptr squares a).
+// While the serving loop runs, host-buffer products from any thread go through
+// it, so concurrent callers share launches (continuous batching) instead of
+// taking turns on the context.
+static int served_product(Ctx *c, const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, size_t out_cap,
+                          size_t *out_len) {
+    gnt_ctx *ctx = reinterpret_cast<gnt_ctx *>(c);
+    gnt_req r{};
+    r.op = b ? GNT_OP_MUL : GNT_OP_SQR;
+    r.a = a;
+    r.na = na;
+    r.b = b;
+    r.nb = nb;
+    uint64_t ticket = 0;
+    if (int rc = gnt_submit(ctx, &r, &ticket)) return rc;
+    gnt_buf *res = nullptr;
+    if (int rc = gnt_wait(ctx, ticket, -1, &res, nullptr)) return rc;
+    size_t len = 0;
+    int rc = gnt_buf_download(ctx, res, out, out_cap, &len);
+    gnt_buf_release(ctx, res);
+    if (rc) return rc;
+    *out_len = len;
+    return GNT_OK;
+}
+
 static int host_product(Ctx *c, const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, size_t out_cap,
                         size_t *out_len) {
     const bool sqr = b == nullptr;
@@ -355,6 +379,7 @@ static int host_product(Ctx *c, const u64 *a, size_t na, const u64 *b, size_t nb
     if (out_cap < na + nb) return fail(GNT_E_ARG, "out_cap %zu < %zu", out_cap, na + nb);
     gnt::Plan pl;
     if (!gnt::plan_for(na, nb, pl)) return fail(GNT_E_TOO_LARGE, "4*(na+nb) exceeds 2^%d", gnt::MAX_LOG_N);
+    if (c->server) return served_product(c, a, na, sqr ? nullptr : b, nb, out, out_cap, out_len);
     std::lock_guard<std::mutex> g(c->mu);
     const size_t nl = na + nb;
     u64 *da = (u64 *)c->pool.acquire(na * 8);

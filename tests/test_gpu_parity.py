@@ -419,3 +419,36 @@ def test_serving_add_sub(ctx, orc):
             b_.release()
     finally:
         srv.stop()
+
+
+def test_concurrent_host_products_join_the_serving_loop(orc):
+    """gnt_mul from several host threads while the serving loop runs: every
+    call goes through the loop (continuous batching) and is exact."""
+    import threading
+    from pi_amd import gnt
+    c = gnt.Context(0)
+    srv = gnt.Server(c)
+    try:
+        rng = np.random.default_rng(21)
+        jobs = [(orc.splitmix64(700 + i, int(2 ** rng.uniform(10, 17))), orc.splitmix64(800 + i, int(2 ** rng.uniform(10, 17))))
+                for i in range(24)]
+        out = [None] * len(jobs)
+
+        def work(k):
+            for i in range(k, len(jobs), 6):
+                a, b = jobs[i]
+                out[i] = c.mul_mag_ntt(a, b) if i % 4 else c.sqr(a)
+
+        ts = [threading.Thread(target=work, args=(k,)) for k in range(6)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        st = srv.stats()
+        assert st["completed"] >= len(jobs)            # all of them went through the loop
+        for i, (a, b) in enumerate(jobs):
+            want = orc.gmp_mul(a, b) if i % 4 else orc.gmp_sqr(a)
+            assert np.array_equal(out[i], want), i
+    finally:
+        srv.stop()
+        c.close()
