@@ -3,6 +3,7 @@
 
 #include <algorithm>
 
+#include "gnt_carry.hpp"
 #include "gnt_engine.hpp"
 #include "gnt_pass.hpp"
 
@@ -10,11 +11,8 @@ namespace gnt {
 
 namespace {
 
-constexpr int BLOCK = 256;    // carry kernels
 constexpr int TBLOCK = 1024;  // transform kernels: 64 KiB tile, 8 waves per SIMD at two tiles per CU
 using X = DevExec<TBLOCK>;
-constexpr int CARRY_LPT = 4;                  // limbs per thread
-constexpr int CARRY_TILE = BLOCK * CARRY_LPT;  // limbs per carry tile
 constexpr int SCAN_BLOCK = 1024;
 
 size_t ceil_log2(size_t x) {
@@ -73,85 +71,37 @@ __global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                           it.n_coeffs, it.n_limbs);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_ctiles(const CItem *__restrict__ items, int n_items, uint64_t total) {
-    __shared__ uint32_t scan[BLOCK];
-    __shared__ uint32_t tile_hi;
+// Limb values of a sum or difference: a + b, or a + ~b + 1 over n_limbs = na
+// limbs (b zero-extended before the complement).
+struct LinearSrc {
+    const uint64_t *a, *b;
+    uint64_t na, nb;
+    bool sub;
+    __device__ __forceinline__ void operator()(uint64_t j, uint64_t &y, uint32_t &g) const {
+        const uint64_t x = j < na ? a[j] : 0;
+        uint64_t v = j < nb ? b[j] : 0;
+        if (sub) v = ~v;
+        y = x + v;
+        g = (uint32_t)(y < x);
+        if (sub && j == 0) {  // the +1 of the two's complement
+            y += 1;
+            g |= (uint32_t)(y == 0);
+        }
+    }
+};
+
+__global__ __launch_bounds__(CARRY_BLOCK) void k_ctiles(const CItem *__restrict__ items, int n_items, uint64_t total) {
+    __shared__ uint32_t sh[2 * CARRY_WAVES];
     for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-        const CItem it = items[find_item(items, n_items, t)];
+        const CItem it = wave_uniform(items[find_item(items, n_items, t)]);
         const uint64_t k = t - it.tile0;
-        const size_t j0 = (size_t)k * CARRY_TILE + (size_t)threadIdx.x * CARRY_LPT;
-        uint64_t y[CARRY_LPT];
-        uint32_t gbits = 0, pbits = 0;
-        if (it.mode == C_SUMS) {
-            uint64_t Hprev = (j0 > 0 && j0 - 1 < it.n_limbs) ? it.Hs[j0 - 1] : 0;
-#pragma unroll
-            for (int q = 0; q < CARRY_LPT; q++) {
-                size_t j = j0 + q;
-                uint64_t L = 0, H = 0;
-                if (j < it.n_limbs) { L = it.Ls[j]; H = it.Hs[j]; }
-                uint64_t s = L + Hprev;
-                gbits |= (uint32_t)(s < L) << q;
-                // limbs past the end propagate, so a tile is all-ones exactly
-                // when its limbs inside the result are
-                pbits |= (uint32_t)(s == ~0ull || j >= it.n_limbs) << q;
-                y[q] = s;
-                Hprev = H;
-            }
-        } else {
-            const bool sub = it.mode == C_SUB;
-#pragma unroll
-            for (int q = 0; q < CARRY_LPT; q++) {
-                size_t j = j0 + q;
-                uint64_t s = 0;
-                uint32_t g = 0;
-                if (j < it.n_limbs) {
-                    uint64_t a = j < it.na ? it.Ls[j] : 0;
-                    uint64_t b = j < it.nb ? it.Hs[j] : 0;
-                    if (sub) b = ~b;  // zero-extended, then complemented
-                    s = a + b;
-                    g = s < a;
-                    if (sub && j == 0) {  // the +1 of the two's complement
-                        s += 1;
-                        g |= s == 0;
-                    }
-                }
-                gbits |= g << q;
-                pbits |= (uint32_t)(s == ~0ull || j >= it.n_limbs) << q;
-                y[q] = s;
-            }
-        }
-        uint32_t agg = 2;
-#pragma unroll
-        for (int q = 0; q < CARRY_LPT; q++) agg = gp_combine(agg, ((gbits >> q) & 1) | (((pbits >> q) & 1) << 1));
-        scan[threadIdx.x] = agg;
-        if (threadIdx.x == 0) tile_hi = 0;
-        __syncthreads();
-        for (int off = 1; off < BLOCK; off <<= 1) {
-            uint32_t v = scan[threadIdx.x];
-            uint32_t w = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 2u;
-            __syncthreads();
-            scan[threadIdx.x] = gp_combine(w, v);
-            __syncthreads();
-        }
-        uint32_t carry = threadIdx.x > 0 ? (scan[threadIdx.x - 1] & 1) : 0;
-        uint32_t hi_nz = 0;
-#pragma unroll
-        for (int q = 0; q < CARRY_LPT; q++) {
-            size_t j = j0 + q;
-            uint64_t z = y[q] + carry;
-            carry = ((gbits >> q) & 1) | (((pbits >> q) & 1) & carry);
-            if (j < it.n_limbs) {
-                it.out[j] = z;
-                if (z != 0) hi_nz = (uint32_t)(j + 1);
-            }
-        }
-        if (hi_nz) atomicMax(&tile_hi, hi_nz);
-        __syncthreads();
+        uint32_t gp, hi;
+        if (it.mode == C_SUMS) carry_tile(SumsSrc{it.Ls, it.Hs}, it.n_limbs, k, it.out, sh, gp, hi);
+        else carry_tile(LinearSrc{it.Ls, it.Hs, it.na, it.nb, it.mode == C_SUB}, it.n_limbs, k, it.out, sh, gp, hi);
         if (threadIdx.x == 0) {
-            it.hi[k] = tile_hi;
-            it.gp[k] = (unsigned char)scan[BLOCK - 1];
+            it.hi[k] = hi;
+            it.gp[k] = (unsigned char)gp;
         }
-        __syncthreads();
     }
 }
 
@@ -189,44 +139,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_cscan(const CItem *__restrict__ 
     if (threadIdx.x == 0) *it.out_len = maxhi;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cripple(const CItem *__restrict__ items, int n_items, uint64_t total) {
-    __shared__ uint32_t allmax[BLOCK];
+// Tiles whose carry-in is 1 add it (one wave per tile).
+__global__ __launch_bounds__(CARRY_RIPPLE_BLOCK) void k_cripple(const CItem *__restrict__ items, int n_items,
+                                                                uint64_t total) {
     for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-        const CItem it = items[find_item(items, n_items, t)];
+        const CItem it = wave_uniform(items[find_item(items, n_items, t)]);
         const uint64_t k = t - it.tile0;
         if (it.cin[k] == 0) continue;  // uniform per workgroup
-        const size_t j0 = (size_t)k * CARRY_TILE + (size_t)threadIdx.x * CARRY_LPT;
-        uint64_t z[CARRY_LPT];
-        uint32_t am = 1;
-#pragma unroll
-        for (int q = 0; q < CARRY_LPT; q++) {
-            size_t j = j0 + q;
-            z[q] = j < it.n_limbs ? it.out[j] : 0;
-            am &= (j < it.n_limbs && z[q] == ~0ull) ? 1u : 0u;
-        }
-        allmax[threadIdx.x] = am;
-        __syncthreads();
-        for (int off = 1; off < BLOCK; off <<= 1) {
-            uint32_t v = allmax[threadIdx.x];
-            uint32_t w = threadIdx.x >= (unsigned)off ? allmax[threadIdx.x - off] : 1u;
-            __syncthreads();
-            allmax[threadIdx.x] = v & w;
-            __syncthreads();
-        }
-        uint32_t carry = threadIdx.x > 0 ? allmax[threadIdx.x - 1] : 1u;
-        uint32_t hi_nz = 0;
-#pragma unroll
-        for (int q = 0; q < CARRY_LPT; q++) {
-            size_t j = j0 + q;
-            if (carry && j < it.n_limbs) {
-                z[q] += 1;
-                carry = z[q] == 0;
-                it.out[j] = z[q];
-                if (z[q] != 0) hi_nz = (uint32_t)(j + 1);
-            }
-        }
-        if (hi_nz) atomicMax(it.out_len, hi_nz);
-        __syncthreads();
+        const uint32_t hi = carry_ripple_tile(it.out, it.n_limbs, k);
+        if (threadIdx.x == 0 && hi) atomicMax(it.out_len, hi);
     }
 }
 
@@ -425,9 +346,9 @@ int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c
             const CItem *items = dev_c + g.first;
             const int n = (int)g.count;
             dim3 grid((unsigned)std::min<uint64_t>(g.tiles, 65536));
-            if (g.kind == 10) hipLaunchKernelGGL(k_ctiles, grid, dim3(BLOCK), 0, st, items, n, g.tiles);
+            if (g.kind == 10) hipLaunchKernelGGL(k_ctiles, grid, dim3(CARRY_BLOCK), 0, st, items, n, g.tiles);
             else if (g.kind == 11) hipLaunchKernelGGL(k_cscan, dim3((unsigned)n), dim3(SCAN_BLOCK), 0, st, items);
-            else hipLaunchKernelGGL(k_cripple, grid, dim3(BLOCK), 0, st, items, n, g.tiles);
+            else hipLaunchKernelGGL(k_cripple, grid, dim3(CARRY_RIPPLE_BLOCK), 0, st, items, n, g.tiles);
         }
         if (rc) return rc;
     }

@@ -14,12 +14,12 @@
 // A square runs A through k_last_fused with itself.
 #include <hip/hip_runtime.h>
 
+#include "gnt_carry.hpp"
 #include "gnt_kernels.hpp"
 #include "gnt_pass.hpp"
 
 namespace gnt {
 
-constexpr int BLOCK = 256;        // carry kernels
 #ifndef GNT_TBLOCK
 #define GNT_TBLOCK 1024
 #endif
@@ -86,64 +86,21 @@ __global__ GNT_TBOUNDS void k_last_fused_ab(u64 *xa, const u64 *__restrict__ xb,
 // Carry: limb sums -> limbs (ntt.rs:669-791).  Limb j's value is
 // y_j = L_j + H_{j-1} with a 1-bit overflow g_j; what is left is a 1-bit
 // carry-lookahead scan over (generate g_j, propagate y_j == 2^64-1), done per
-// tile, then across tiles, then applied to the (rare) tiles that receive one.
+// tile (gnt_carry.hpp), then across tiles, then applied to the tiles that
+// receive one.
 // ---------------------------------------------------------------------------
 
-constexpr int CARRY_LIMBS_PER_THREAD = 4;
-constexpr int CARRY_TILE = BLOCK * CARRY_LIMBS_PER_THREAD;  // limbs per workgroup
-
-__global__ __launch_bounds__(BLOCK) void k_carry_tiles(const u64 *__restrict__ Ls, const u64 *__restrict__ Hs,
-                                                       u64 *__restrict__ out, size_t n_limbs,
-                                                       unsigned char *__restrict__ tile_gp, u32 *__restrict__ out_len) {
-    __shared__ u32 scan[BLOCK];
-    const size_t j0 = (size_t)blockIdx.x * CARRY_TILE + (size_t)threadIdx.x * CARRY_LIMBS_PER_THREAD;
-    u64 y[CARRY_LIMBS_PER_THREAD];
-    u32 gbits = 0, pbits = 0;
-    u64 Hprev = (j0 > 0 && j0 - 1 < n_limbs) ? Hs[j0 - 1] : 0;
-#pragma unroll
-    for (int k = 0; k < CARRY_LIMBS_PER_THREAD; k++) {
-        size_t j = j0 + k;
-        u64 L = 0, H = 0;
-        if (j < n_limbs) { L = Ls[j]; H = Hs[j]; }
-        u64 s = L + Hprev;
-        gbits |= (u32)(s < L) << k;
-        pbits |= (u32)(s == ~0ull) << k;
-        y[k] = s;
-        Hprev = H;
+__global__ __launch_bounds__(CARRY_BLOCK) void k_carry_tiles(const u64 *__restrict__ Ls, const u64 *__restrict__ Hs,
+                                                             u64 *__restrict__ out, size_t n_limbs,
+                                                             unsigned char *__restrict__ tile_gp,
+                                                             u32 *__restrict__ out_len) {
+    __shared__ u32 sh[2 * CARRY_WAVES];
+    u32 gp, hi;
+    carry_tile(SumsSrc{Ls, Hs}, n_limbs, blockIdx.x, out, sh, gp, hi);
+    if (threadIdx.x == 0) {
+        tile_gp[blockIdx.x] = (unsigned char)gp;
+        if (hi) atomicMax(out_len, hi);
     }
-    u32 agg = 2;
-#pragma unroll
-    for (int k = 0; k < CARRY_LIMBS_PER_THREAD; k++)
-        agg = gp_combine(agg, ((gbits >> k) & 1) | (((pbits >> k) & 1) << 1));
-    scan[threadIdx.x] = agg;
-    __syncthreads();
-    for (int off = 1; off < BLOCK; off <<= 1) {
-        u32 v = scan[threadIdx.x];
-        u32 w = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 2u;
-        __syncthreads();
-        scan[threadIdx.x] = gp_combine(w, v);
-        __syncthreads();
-    }
-    u32 carry = threadIdx.x > 0 ? (scan[threadIdx.x - 1] & 1) : 0;
-    u32 hi_nz = 0;
-#pragma unroll
-    for (int k = 0; k < CARRY_LIMBS_PER_THREAD; k++) {
-        size_t j = j0 + k;
-        u64 z = y[k] + carry;
-        carry = ((gbits >> k) & 1) | (((pbits >> k) & 1) & carry);
-        if (j < n_limbs) {
-            out[j] = z;
-            if (z != 0) hi_nz = (u32)(j + 1);
-        }
-    }
-    // one atomic per workgroup for the trimmed length
-    __shared__ u32 tile_hi;
-    if (threadIdx.x == 0) tile_hi = 0;
-    __syncthreads();
-    if (hi_nz) atomicMax(&tile_hi, hi_nz);
-    __syncthreads();
-    if (threadIdx.x == 0 && tile_hi) atomicMax(out_len, tile_hi);
-    if (threadIdx.x == BLOCK - 1) tile_gp[blockIdx.x] = (unsigned char)scan[BLOCK - 1];
 }
 
 // One workgroup: exclusive carry-in of every tile.
@@ -172,44 +129,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_carry_scan(const unsigned char *
     }
 }
 
-// Tiles whose carry-in is 1 add it: the +1 ripples through the run of
-// all-ones limbs at the bottom of the tile.
-__global__ __launch_bounds__(BLOCK) void k_carry_ripple(u64 *__restrict__ out, size_t n_limbs,
-                                                        const unsigned char *__restrict__ tile_cin,
-                                                        u32 *__restrict__ out_len) {
+// Tiles whose carry-in is 1 add it (one wave per tile).
+__global__ __launch_bounds__(CARRY_RIPPLE_BLOCK) void k_carry_ripple(u64 *__restrict__ out, size_t n_limbs,
+                                                                     const unsigned char *__restrict__ tile_cin,
+                                                                     u32 *__restrict__ out_len) {
     if (tile_cin[blockIdx.x] == 0) return;  // uniform per workgroup
-    __shared__ u32 allmax[BLOCK];
-    const size_t j0 = (size_t)blockIdx.x * CARRY_TILE + (size_t)threadIdx.x * CARRY_LIMBS_PER_THREAD;
-    u64 z[CARRY_LIMBS_PER_THREAD];
-    u32 am = 1;
-#pragma unroll
-    for (int k = 0; k < CARRY_LIMBS_PER_THREAD; k++) {
-        size_t j = j0 + k;
-        z[k] = j < n_limbs ? out[j] : 0;
-        am &= (j < n_limbs && z[k] == ~0ull) ? 1u : 0u;
-    }
-    allmax[threadIdx.x] = am;
-    __syncthreads();
-    for (int off = 1; off < BLOCK; off <<= 1) {
-        u32 v = allmax[threadIdx.x];
-        u32 w = threadIdx.x >= (unsigned)off ? allmax[threadIdx.x - off] : 1u;
-        __syncthreads();
-        allmax[threadIdx.x] = v & w;
-        __syncthreads();
-    }
-    u32 carry = threadIdx.x > 0 ? allmax[threadIdx.x - 1] : 1u;
-    u32 hi_nz = 0;
-#pragma unroll
-    for (int k = 0; k < CARRY_LIMBS_PER_THREAD; k++) {
-        size_t j = j0 + k;
-        if (carry && j < n_limbs) {
-            z[k] += 1;
-            carry = z[k] == 0;
-            out[j] = z[k];
-            if (z[k] != 0) hi_nz = (u32)(j + 1);
-        }
-    }
-    if (hi_nz) atomicMax(out_len, hi_nz);
+    const u32 hi = carry_ripple_tile(out, n_limbs, blockIdx.x);
+    if (threadIdx.x == 0 && hi) atomicMax(out_len, hi);
 }
 
 __global__ void k_zero_u32(u32 *p, int n) {
@@ -333,10 +259,11 @@ int launch_carry(const u64 *Ls, const u64 *Hs, u64 *out, size_t n_limbs, unsigne
                  unsigned char *tile_cin, u32 *out_len, hipStream_t st) {
     size_t nt = carry_tiles(n_limbs);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, out_len, 1);
-    hipLaunchKernelGGL(k_carry_tiles, dim3((unsigned)nt), dim3(BLOCK), 0, st, Ls, Hs, out, n_limbs, tile_gp,
+    hipLaunchKernelGGL(k_carry_tiles, dim3((unsigned)nt), dim3(CARRY_BLOCK), 0, st, Ls, Hs, out, n_limbs, tile_gp,
                        out_len);
     hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, tile_gp, tile_cin, nt);
-    hipLaunchKernelGGL(k_carry_ripple, dim3((unsigned)nt), dim3(BLOCK), 0, st, out, n_limbs, tile_cin, out_len);
+    hipLaunchKernelGGL(k_carry_ripple, dim3((unsigned)nt), dim3(CARRY_RIPPLE_BLOCK), 0, st, out, n_limbs, tile_cin,
+                       out_len);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -1,21 +1,25 @@
-"""CPU restatement of the engine's three carry steps (k_ctiles, k_cscan,
-k_cripple in pi_amd/csrc/gnt_engine.hip), tile for tile, checked against
-CPython int for products' limb sums, sums and differences.  It pins the
-tile-level rules the kernels rely on: (generate, propagate) aggregation, the
-carry-in scan over tiles, the sparse ripple, and the trimmed length when a
-difference's top limbs cancel (a carry that leaves the top clears every
-all-ones tile it crosses)."""
+"""CPU restatement of the carry kernels' tile routines (carry_tile and
+carry_ripple_tile in pi_amd/csrc/gnt_carry.hpp, with the tile scan of
+k_cscan in pi_amd/csrc/gnt_engine.hip), wave for wave and row for row,
+checked against CPython int for products' limb sums, sums and differences.
+It pins the rules the kernels rely on: the 64 carries of a row from
+((G|P) + G + cin) ^ P, the per-lane (m0, pre) bits that re-apply a wave's
+carry-in, the (generate, propagate) aggregation over rows, waves and tiles,
+the ripple of a tile's carry-in, and the trimmed length when a difference's
+top limbs cancel (a carry that leaves the top clears every all-ones tile it
+crosses)."""
 import random
 
 import pytest
 
-TILE = 1024
+WAVES, ROWS = 4, 8
+TILE = WAVES * ROWS * 64
 MASK = (1 << 64) - 1
 SUMS, ADD, SUB = 0, 1, 2
 
 
 def _limb_values(mode, n_limbs, A, B, j):
-    """(s, generate) of limb j as k_ctiles forms it, before in-tile carries."""
+    """(y, generate, propagate) of limb j before carries (the Src functors)."""
     if j >= n_limbs:
         return 0, 0, True            # padding: propagates (never written)
     if mode == SUMS:
@@ -34,23 +38,79 @@ def _limb_values(mode, n_limbs, A, B, j):
     return s, g, s == MASK
 
 
+def row_carries(G, P, cin):
+    """Carries into the 64 lanes of a row and the carry out (row_carries)."""
+    A = G | P
+    t = (A + G) & MASK
+    S = (t + cin) & MASK
+    return S ^ P, int(t < A) | int(S < t)
+
+
+def _bits(flags):
+    return sum(1 << i for i, f in enumerate(flags) if f)
+
+
+def carry_tile(mode, n_limbs, A, B, tile, out):
+    """One tile as the kernel works it: returns ((generate, propagate), hi)."""
+    waves = []
+    for w in range(WAVES):
+        jw = tile * TILE + w * ROWS * 64
+        rows = []
+        for r in range(ROWS):
+            vals = [_limb_values(mode, n_limbs, A, B, jw + r * 64 + lane) for lane in range(64)]
+            G = _bits(v[1] for v in vals)
+            P = _bits(v[2] for v in vals)
+            M, co = row_carries(G, P, 0)
+            m0 = [(M >> lane) & 1 for lane in range(64)]
+            pre = [int((~P & ((1 << lane) - 1)) == 0) for lane in range(64)]
+            rows.append(([v[0] for v in vals], m0, pre, co, int(P == MASK)))
+        c = 0
+        for _, _, _, co, rp in rows:
+            c = co | (rp & c)
+        waves.append((rows, c, int(all(rw[4] for rw in rows))))
+    # second pass: every wave with its carry-in
+    cw, agg_g, agg_p, hi = 0, 0, 1, 0
+    for w, (rows, g, p) in enumerate(waves):
+        jw = tile * TILE + w * ROWS * 64
+        c = cw
+        for r, (y, m0, pre, co, rp) in enumerate(rows):
+            for lane in range(64):
+                j = jw + r * 64 + lane
+                z = (y[lane] + (m0[lane] | (c & pre[lane]))) & MASK
+                if j < n_limbs:
+                    out[j] = z
+                    if z:
+                        hi = max(hi, j + 1)
+            c = co | (rp & c)
+        agg_g, agg_p = g | (p & agg_g), agg_p & p
+        cw = g | (p & cw)
+    return (agg_g, agg_p), hi
+
+
+def carry_ripple_tile(out, n_limbs, tile):
+    """+1 into a tile, a row of 64 limbs at a time; 1 + the incremented limb."""
+    j1 = min(n_limbs, (tile + 1) * TILE)
+    for j0 in range(tile * TILE, j1, 64):
+        lanes = [j for j in range(j0, j0 + 64) if j < j1]
+        stop = [j for j in lanes if out[j] != MASK]
+        first = stop[0] if stop else j0 + 64
+        for j in lanes:
+            if j < first:
+                out[j] = 0
+            elif j == first:
+                out[j] += 1
+        if stop:
+            return first + 1
+    return 0
+
+
 def carry_tiles(mode, n_limbs, A, B):
     nt = (n_limbs + TILE - 1) // TILE
     out = [0] * n_limbs
     gp, hi = [], []
     for k in range(nt):
-        carry, h = 0, 0
-        G, P = 0, 1
-        for j in range(k * TILE, (k + 1) * TILE):
-            s, g, p = _limb_values(mode, n_limbs, A, B, j)
-            z = (s + carry) & MASK
-            carry = g | (int(p) & carry)
-            G, P = g | (int(p) & G), P & int(p)   # aggregate of the limbs so far
-            if j < n_limbs:
-                out[j] = z
-                if z:
-                    h = j + 1
-        gp.append((G, P))
+        g, h = carry_tile(mode, n_limbs, A, B, k, out)
+        gp.append(g)
         hi.append(h)
     # k_cscan: carry into each tile, length excluding all-ones tiles that a carry clears
     cin, carry, maxhi = [], 0, 0
@@ -62,14 +122,33 @@ def carry_tiles(mode, n_limbs, A, B):
         carry = G | (P & carry)
     # k_cripple
     for k in range(nt):
-        if not cin[k]:
-            continue
-        for j in range(k * TILE, min((k + 1) * TILE, n_limbs)):
-            out[j] = (out[j] + 1) & MASK
-            if out[j]:
-                maxhi = max(maxhi, j + 1)
-                break
+        if cin[k]:
+            maxhi = max(maxhi, carry_ripple_tile(out, n_limbs, k))
     return out, maxhi
+
+
+def test_row_carries_is_the_carry_chain():
+    """Brute force: the closed form equals c_i = g_(i-1) | (p_(i-1) & c_(i-1))."""
+    rnd = random.Random(5)
+    for _ in range(3000):
+        kind = rnd.random()
+        g = [0] * 64
+        p = [0] * 64
+        for i in range(64):
+            u = rnd.random()
+            if kind < 0.3:           # mostly propagate: long runs
+                p[i] = int(u < 0.9)
+                g[i] = int(not p[i] and u < 0.95)
+            else:
+                g[i] = int(u < 0.3)
+                p[i] = int(not g[i] and u < 0.7)
+        cin = rnd.getrandbits(1)
+        M, cout = row_carries(_bits(g), _bits(p), cin)
+        c = cin
+        for i in range(64):
+            assert (M >> i) & 1 == c
+            c = g[i] | (p[i] & c)
+        assert cout == c
 
 
 def to_limbs(x, n=None):
@@ -122,10 +201,10 @@ def test_add_sub_tiles(i):
 def test_product_limb_sums():
     """SUMS mode: limb j = L_j + H_(j-1), as the last inverse pass leaves them."""
     rnd = random.Random(9)
-    for n in (5, 1023, 1024, 1025, 3000):
+    for n in (5, 1023, 2047, 2048, 2049, 4100):
         L = [rnd.getrandbits(64) for _ in range(n)]
         H = [rnd.getrandbits(20) for _ in range(n - 1)] + [0]
-        run = range(n // 2, min(n - 2, n // 2 + 1500))
+        run = range(n // 2, min(n - 2, n // 2 + 2100))
         for j in run:
             L[j] = MASK                   # long propagate run
         L[n - 1] = rnd.getrandbits(32)    # the value fits n limbs, as a product's does
