@@ -164,6 +164,12 @@ int gnt_set_plan(int log_n, int npass, const int *log_r);
 /* Load a plan cache written by gnt_tune (also done by gnt_init when
  * gnt_config.plan_cache_path is set). */
 int gnt_load_plans(const char *path);
+/* A strided pass whose block size is at most 2^max_log_m multiplies its
+ * twiddles from a precomputed table (one per pass shape and direction, built
+ * on first use, 8 * 2^log_m bytes); larger blocks step them.  -1 disables
+ * tables; the default is 22.  *previous (if not null) receives the old
+ * setting.  Process-wide, like the plans. */
+int gnt_set_pass_tables(int max_log_m, int *previous);
 /* Time every valid split of each length in [min_log_n, max_log_n] (4..28) on
  * the device (median of reps products), install the fastest, and write the
  * choices and timings to cache_path (JSON) when it is not NULL. */

@@ -173,6 +173,40 @@ int Ctx::init_tables() {
     return GNT_OK;
 }
 
+void Ctx::pass_tables(const Plan &pl, PassTables &out) {
+    out = PassTables{};
+    std::lock_guard<std::mutex> lk(tab_mu);
+    for (int p = 0; p < pl.npass; p++) {
+        const PassDesc &d = pl.pass[p];
+        if (!pass_has_table(d)) continue;
+        for (int inv = 0; inv < 2; inv++) {
+            const int key = (d.log_m << 8) | (d.log_r << 1) | inv;
+            auto it = ptab.find(key);
+            uint64_t *T = nullptr;
+            if (it != ptab.end()) {
+                T = it->second;
+            } else {
+                if (hipMalloc(&T, sizeof(uint64_t) << d.log_m) != hipSuccess) {
+                    (void)hipGetLastError();
+                    T = nullptr;
+                } else if (launch_pass_table(T, d.log_m, d.log_r, inv != 0, tw(), stream) != 0) {
+                    (void)hipFree(T);
+                    T = nullptr;
+                }
+                if (!T) continue;
+                ptab[key] = T;
+            }
+            (inv ? out.inv : out.fwd)[p] = T;
+        }
+    }
+}
+
+void Ctx::free_pass_tables() {
+    std::lock_guard<std::mutex> lk(tab_mu);
+    for (auto &kv : ptab) (void)hipFree(kv.second);
+    ptab.clear();
+}
+
 bool plan_for(size_t na, size_t nb, Plan &pl) {
     size_t n_out = 4 * (na + nb) - 1;
     int log_n = ceil_log2(n_out);
@@ -188,6 +222,8 @@ int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64
     if (sqr) nb = na;
     Plan pl;
     if (!plan_for(na, nb, pl)) return set_error(GNT_E_TOO_LARGE, "product exceeds the transform cap");
+    PassTables tabs;
+    pass_tables(pl, tabs);
     const size_t N = (size_t)1 << pl.log_n;
     const size_t n_out = 4 * (na + nb) - 1;
     const size_t n_limbs = na + nb;
@@ -216,22 +252,23 @@ int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64
     const bool ab = !sqr && P > 1;  // both last forward passes in the fused kernel
     int rc = 0;
     if (sqr) {
-        if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, tw(), stream);
+        if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, tw(), tabs, stream);
     } else if (ab) {
-        rc = launch_forward_pair(pl, P - 1, xa, da, na, xb, db, nb, tw(), stream);
+        rc = launch_forward_pair(pl, P - 1, xa, da, na, xb, db, nb, tw(), tabs, stream);
     } else {
-        rc = launch_forward(pl, P, xa, da, na, tw(), stream);
+        rc = launch_forward(pl, P, xa, da, na, tw(), tabs, stream);
     }
     if (timing) (void)hipEventRecord(ev[1], stream);
     if (!rc && ab) {
-        rc = launch_last_fused_ab(pl, xa, xb, tw(), stream);
+        rc = launch_last_fused_ab(pl, xa, xb, tw(), tabs, stream);
     } else if (!rc) {
         const u64 *bsrc = sqr ? xa : xb;
         const u64 *blimbs = sqr ? da : db;
-        rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, tw(), Ls, Hs, n_out, n_limbs, stream);
+        rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, tw(), tabs, Ls, Hs, n_out, n_limbs,
+                               stream);
     }
     if (timing) (void)hipEventRecord(ev[2], stream);
-    if (!rc) rc = launch_inverse_rest(pl, xa, tw(), Ls, Hs, n_out, n_limbs, stream);
+    if (!rc) rc = launch_inverse_rest(pl, xa, tw(), tabs, Ls, Hs, n_out, n_limbs, stream);
     if (timing) (void)hipEventRecord(ev[3], stream);
     if (!rc) rc = launch_carry(Ls, Hs, dout, n_limbs, gp, gp + nt, dlen, stream);
     if (timing) (void)hipEventRecord(ev[4], stream);
@@ -326,6 +363,7 @@ int gnt_shutdown(gnt_ctx *ctx) {
     (void)hipStreamSynchronize(c->stream);
     c->pool.release_all();
     c->upload_pool.release_all();
+    c->free_pass_tables();
     (void)hipFree(c->wlo);
     (void)hipFree(c->whi);
     (void)hipFree(c->wsm);

@@ -68,6 +68,7 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
         p.da = b->in_a[i];
         p.db = sq ? nullptr : b->in_b[i];
         if (!prod_init(p)) { release_batch(b); return set_error(GNT_E_TOO_LARGE, "product exceeds the transform cap"); }
+        c->pass_tables(p.pl, p.tabs);
         p.dout = (u64 *)c->pool.acquire(p.nl * 8);
         p.dlen = (uint32_t *)c->pool.acquire(4);
         p.xa = (u64 *)c->pool.acquire(p.N * 8);

@@ -60,15 +60,24 @@ __global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     if constexpr (KIND == K_FUSED_AB) {
         using H = typename HalfGeom<G>::type;  // half-width tiles of the last-pass shape
         if constexpr (HalfGeom<G>::ok)
-            body_fused_ab<X, H>(tile, make_geom<H>(it.d, t - it.tile0), it.x, it.other, it.log_n, tw);
+            body_fused_ab<X, H>(tile, make_geom<H>(it.d, t - it.tile0), it.x, it.other, it.log_n, tw,
+                                StoreTw{it.T, it.log_t});
         return;
     }
     const G g = make_geom<G>(it.d, t - it.tile0);
-    if (KIND == K_FWD) body_fwd<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, tw);
-    else if (KIND == K_INV) body_inv<X, G, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0);
-    else if (KIND == K_INV_SUMS) body_inv<X, G, true>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs);
+    const StoreTw st{it.T, it.log_t};
+    if constexpr (KIND == K_FWD) body_fwd<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, tw, it.T);
+    else if constexpr (KIND == K_INV) body_inv<X, G, false, true, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, st);
+    else if constexpr (KIND == K_INV_PLAIN)
+        body_inv<X, G, false, false, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, st);
+    else if constexpr (KIND == K_INV_TSTORE)
+        body_inv<X, G, false, false, true>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, st);
+    else if constexpr (KIND == K_INV_SUMS)
+        body_inv<X, G, true, true, false>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs, st);
+    else if constexpr (KIND == K_INV_SUMS_PLAIN)
+        body_inv<X, G, true, false, false>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs, st);
     else body_fused<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls, it.Hs,
-                          it.n_coeffs, it.n_limbs);
+                          it.n_coeffs, it.n_limbs, st);
 }
 
 // Limb values of a sum or difference: a + b, or a + ~b + 1 over n_limbs = na
@@ -185,6 +194,15 @@ bool prod_init(Prod &p) {
     return true;
 }
 
+// The table the store of inverse pass `pass` (or of the fused stage) applies:
+// that of inverse pass pass-1, which runs next.
+static void inv_store_table(const Prod &p, int pass, TItem &t) {
+    if (pass >= 1 && p.tabs.inv[pass - 1]) {
+        t.T = p.tabs.inv[pass - 1];
+        t.log_t = p.pl.pass[pass - 1].log_m;
+    }
+}
+
 static TItem titem(const Prod &p, int pass, uint64_t *x, const uint64_t *limbs, size_t n_in) {
     TItem t{};
     t.x = x;
@@ -226,10 +244,12 @@ void add_stage(Iteration &it, Prod &p) {
     if (s < nfwd) {
         TItem a = titem(p, s, p.xa, s == 0 ? p.da : nullptr, p.na);
         a.pad = K_FWD;
+        a.T = p.tabs.fwd[s];
         it.titems.push_back(a);
         if (!sq && s < P - 1) {
             TItem b = titem(p, s, p.xb, s == 0 ? p.db : nullptr, p.nb);
             b.pad = K_FWD;
+            b.T = p.tabs.fwd[s];
             it.titems.push_back(b);
         }
         return;
@@ -238,6 +258,7 @@ void add_stage(Iteration &it, Prod &p) {
         TItem f = titem(p, P - 1, p.xa, nullptr, 0);
         f.other = p.xb;
         f.pad = K_FUSED_AB;
+        inv_store_table(p, P - 1, f);
         it.titems.push_back(f);
         return;
     }
@@ -247,6 +268,7 @@ void add_stage(Iteration &it, Prod &p) {
         f.other = sq ? nullptr : p.xa;
         if (P == 1) { f.Ls = Ls; f.Hs = Hs; }
         f.pad = K_FUSED;
+        inv_store_table(p, P - 1, f);
         it.titems.push_back(f);
         return;
     }
@@ -255,7 +277,11 @@ void add_stage(Iteration &it, Prod &p) {
         const int pass = P - 2 - inv_stage;
         TItem v = titem(p, pass, p.xa, nullptr, 0);
         if (pass == 0) { v.Ls = Ls; v.Hs = Hs; }
-        v.pad = pass == 0 ? K_INV_SUMS : K_INV;
+        if (pass > 0) inv_store_table(p, pass, v);
+        // a pass with a table had its twiddles applied by the stage before
+        const bool tab = p.tabs.inv[pass] != nullptr;
+        if (pass == 0) v.pad = tab ? K_INV_SUMS_PLAIN : K_INV_SUMS;
+        else v.pad = !tab ? K_INV : v.T ? K_INV_TSTORE : K_INV_PLAIN;
         it.titems.push_back(v);
         return;
     }
@@ -340,6 +366,9 @@ int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c
         if (g.kind == K_FWD) rc = launch_tgroup<K_FWD>(g, dev_t, tw, st);
         else if (g.kind == K_INV) rc = launch_tgroup<K_INV>(g, dev_t, tw, st);
         else if (g.kind == K_INV_SUMS) rc = launch_tgroup<K_INV_SUMS>(g, dev_t, tw, st);
+        else if (g.kind == K_INV_PLAIN) rc = launch_tgroup<K_INV_PLAIN>(g, dev_t, tw, st);
+        else if (g.kind == K_INV_TSTORE) rc = launch_tgroup<K_INV_TSTORE>(g, dev_t, tw, st);
+        else if (g.kind == K_INV_SUMS_PLAIN) rc = launch_tgroup<K_INV_SUMS_PLAIN>(g, dev_t, tw, st);
         else if (g.kind == K_FUSED) rc = launch_tgroup<K_FUSED>(g, dev_t, tw, st);
         else if (g.kind == K_FUSED_AB) rc = launch_tgroup<K_FUSED_AB>(g, dev_t, tw, st);
         else {

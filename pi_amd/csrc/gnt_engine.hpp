@@ -22,7 +22,21 @@
 
 namespace gnt {
 
-enum ItemKind : int { K_FWD = 0, K_INV = 1, K_FUSED = 2, K_FUSED_AB = 3, K_INV_SUMS = 4 };
+// Kinds of tile-pass items; each is its own kernel.  Inverse passes: K_INV
+// steps its twiddles after the load (a pass without a table), K_INV_PLAIN
+// found them applied by the stage before, K_INV_TSTORE in addition applies
+// the next inverse pass's table on the way out; the _SUMS kinds are the last
+// inverse pass, writing limb sums.
+enum ItemKind : int {
+    K_FWD = 0,
+    K_INV = 1,
+    K_FUSED = 2,
+    K_FUSED_AB = 3,
+    K_INV_SUMS = 4,
+    K_INV_PLAIN = 5,
+    K_INV_TSTORE = 6,
+    K_INV_SUMS_PLAIN = 7
+};
 
 // One tile-pass over one array (device descriptor).
 struct TItem {
@@ -35,8 +49,10 @@ struct TItem {
     uint64_t n_coeffs;
     uint64_t n_limbs;
     uint64_t tile0;         // first launch-wide tile index of this item
+    const uint64_t *T;      // FWD: the pass's twiddle table; INV/FUSED*: the next inverse pass's (null: none)
     PassDesc d;
     int log_n;
+    int log_t;              // log2 of T's length
     int pad;
 };
 
@@ -70,6 +86,7 @@ struct Prod {
     uint64_t *dout = nullptr;
     uint32_t *dlen = nullptr;
     Plan pl;
+    PassTables tabs;  // twiddle tables of pl's passes (filled by the runtime)
     size_t N = 0, n_out = 0, nl = 0, nt = 0;
     uint64_t *xa = nullptr, *xb = nullptr, *lh = nullptr;
     unsigned char *cs = nullptr;  // carry scratch: gp[nt] cin[nt] pad, hi[nt] u32
@@ -87,7 +104,7 @@ bool prod_init(Prod &p);
 
 // Launch groups of one iteration.
 struct Group {
-    int kind;   // K_FWD/K_INV/K_INV_SUMS/K_FUSED/K_FUSED_AB, or 10+k for carry step k
+    int kind;   // an ItemKind, or 10+k for carry step k
     int shape;  // shape_key of the group's tiles
     size_t first, count;  // range in the iteration's item table
     uint64_t tiles;       // total tiles of the launch

@@ -33,9 +33,9 @@ using X = DevExec<TBLOCK>;
 
 template <class G>
 __global__ GNT_TBOUNDS void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs, size_t n_limbs,
-                                       PassDesc d, Tw tw) {
+                                       PassDesc d, Tw tw, const u64 *__restrict__ T) {
     extern __shared__ u64 tile[];
-    body_fwd<X, G>(tile, make_geom<G>(d, blockIdx.x), x, limbs, n_limbs, tw);
+    body_fwd<X, G>(tile, make_geom<G>(d, blockIdx.x), x, limbs, n_limbs, tw, T);
 }
 
 // The same forward pass over two operands in one launch: blocks [0, tiles)
@@ -43,20 +43,21 @@ __global__ GNT_TBOUNDS void k_fwd_pass(u64 *__restrict__ x, const u64 *__restric
 template <class G>
 __global__ GNT_TBOUNDS void k_fwd_pass_pair(u64 *__restrict__ xa, const u64 *__restrict__ la, size_t na,
                                             u64 *__restrict__ xb, const u64 *__restrict__ lb, size_t nb,
-                                            unsigned tiles, PassDesc d, Tw tw) {
+                                            unsigned tiles, PassDesc d, Tw tw, const u64 *__restrict__ T) {
     extern __shared__ u64 tile[];
     const bool b = blockIdx.x >= tiles;
     const unsigned t = b ? blockIdx.x - tiles : blockIdx.x;
-    if (b) body_fwd<X, G>(tile, make_geom<G>(d, t), xb, lb, nb, tw);
-    else body_fwd<X, G>(tile, make_geom<G>(d, t), xa, la, na, tw);
+    if (b) body_fwd<X, G>(tile, make_geom<G>(d, t), xb, lb, nb, tw, T);
+    else body_fwd<X, G>(tile, make_geom<G>(d, t), xa, la, na, tw, T);
 }
 
 // Inverse pass; SUMS: the last one, writing limb sums.
-template <class G, bool SUMS>
+template <class G, bool SUMS, int FORM>
 __global__ GNT_TBOUNDS void k_inv_pass(u64 *__restrict__ x, PassDesc d, Tw tw, u64 *__restrict__ Ls,
-                                       u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs) {
+                                       u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs, StoreTw st) {
     extern __shared__ u64 tile[];
-    body_inv<X, G, SUMS>(tile, make_geom<G>(d, blockIdx.x), x, tw, Ls, Hs, n_coeffs, n_limbs);
+    body_inv<X, G, SUMS, FORM == INV_STEP, FORM == INV_TSTORE>(tile, make_geom<G>(d, blockIdx.x), x, tw, Ls, Hs,
+                                                               n_coeffs, n_limbs, st);
 }
 
 // Last forward pass of B (read from xb, or packed from limbs), pointwise with
@@ -66,20 +67,21 @@ template <class G>
 __global__ GNT_TBOUNDS void k_last_fused(const u64 *__restrict__ xb, const u64 *__restrict__ limbs,
                                          size_t n_limbs_in, u64 *xa, const u64 *a_other, PassDesc d, int log_n,
                                          Tw tw, u64 *__restrict__ Ls, u64 *__restrict__ Hs, size_t n_coeffs,
-                                         size_t n_limbs) {
+                                         size_t n_limbs, StoreTw st) {
     extern __shared__ u64 tile[];
     body_fused<X, G>(tile, make_geom<G>(d, blockIdx.x), xb, limbs, n_limbs_in, xa, a_other, log_n, tw, Ls, Hs,
-                     n_coeffs, n_limbs);
+                     n_coeffs, n_limbs, st);
 }
 
 // Last forward passes of A and B, pointwise, 1/N, first inverse pass; result
 // to xa (multiplies with plans of two or more passes).
 // G is the plan's last-pass shape; the kernel works on half-width tiles of it.
 template <class G>
-__global__ GNT_TBOUNDS void k_last_fused_ab(u64 *xa, const u64 *__restrict__ xb, PassDesc d, int log_n, Tw tw) {
+__global__ GNT_TBOUNDS void k_last_fused_ab(u64 *xa, const u64 *__restrict__ xb, PassDesc d, int log_n, Tw tw,
+                                            StoreTw st) {
     extern __shared__ u64 tile[];
     using H = typename HalfGeom<G>::type;
-    if constexpr (HalfGeom<G>::ok) body_fused_ab<X, H>(tile, make_geom<H>(d, blockIdx.x), xa, xb, log_n, tw);
+    if constexpr (HalfGeom<G>::ok) body_fused_ab<X, H>(tile, make_geom<H>(d, blockIdx.x), xa, xb, log_n, tw, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -138,6 +140,11 @@ __global__ __launch_bounds__(CARRY_RIPPLE_BLOCK) void k_carry_ripple(u64 *__rest
     if (threadIdx.x == 0 && hi) atomicMax(out_len, hi);
 }
 
+__global__ void k_pass_table(u64 *__restrict__ T, int log_m, int log_r, int inv, Tw tw) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ((size_t)1 << log_m)) T[i] = pass_table_entry(tw, (u32)i, log_m, log_r, inv != 0);
+}
+
 __global__ void k_zero_u32(u32 *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -152,8 +159,15 @@ static inline dim3 pass_grid(const PassDesc &d, int log_n) {
 }
 static inline size_t pass_lds(const PassDesc &d) { return ((size_t)1 << (d.log_r + d.log_c)) * sizeof(u64); }
 
+// The table the store of inverse pass p (or of the fused stage, p = npass-1)
+// applies: that of inverse pass p-1, which runs next.
+static StoreTw inv_store_tw(const Plan &pl, const PassTables &tabs, int p) {
+    if (p < 1 || !tabs.inv[p - 1]) return StoreTw{nullptr, 0};
+    return StoreTw{tabs.inv[p - 1], pl.pass[p - 1].log_m};
+}
+
 int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw,
-                   hipStream_t st) {
+                   const PassTables &tabs, hipStream_t st) {
     for (int p = 0; p < npasses; p++) {
         const PassDesc d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
@@ -161,7 +175,7 @@ int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t
 #define FWD_CASE(LR, LC, S)                                                                                     \
     case shape_key(LR, LC, S):                                                                                  \
         hipLaunchKernelGGL((k_fwd_pass<Geom<LR, LC, S>>), pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, x, \
-                           src, n_limbs, d, tw);                                                                \
+                           src, n_limbs, d, tw, tabs.fwd[p]);                                                                \
         break;
             GNT_FOR_EACH_SHAPE(FWD_CASE)
 #undef FWD_CASE
@@ -172,7 +186,7 @@ int launch_forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t
 }
 
 int launch_forward_pair(const Plan &pl, int npasses, u64 *xa, const u64 *la, size_t na, u64 *xb, const u64 *lb,
-                        size_t nb, const Tw &tw, hipStream_t st) {
+                        size_t nb, const Tw &tw, const PassTables &tabs, hipStream_t st) {
     for (int p = 0; p < npasses; p++) {
         const PassDesc d = pl.pass[p];
         const unsigned tiles = pass_grid(d, pl.log_n).x;
@@ -181,7 +195,7 @@ int launch_forward_pair(const Plan &pl, int npasses, u64 *xa, const u64 *la, siz
 #define FWD2_CASE(LR, LC, S)                                                                                     \
     case shape_key(LR, LC, S):                                                                                   \
         hipLaunchKernelGGL((k_fwd_pass_pair<Geom<LR, LC, S>>), dim3(2 * tiles), dim3(TBLOCK), pass_lds(d), st, xa, \
-                           sa, na, xb, sb, nb, tiles, d, tw);                                                    \
+                           sa, na, xb, sb, nb, tiles, d, tw, tabs.fwd[p]);                                                    \
         break;
             GNT_FOR_EACH_SHAPE(FWD2_CASE)
 #undef FWD2_CASE
@@ -192,7 +206,9 @@ int launch_forward_pair(const Plan &pl, int npasses, u64 *xa, const u64 *la, siz
 }
 
 int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t nb, u64 *xa, bool square,
-                      const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs, hipStream_t st) {
+                      const Tw &tw, const PassTables &tabs, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
+                      hipStream_t st) {
+    const StoreTw stw = inv_store_tw(pl, tabs, pl.npass - 1);
     const PassDesc d = pl.pass[pl.npass - 1];
     const bool only = pl.npass == 1;
     const u64 *src = only ? limbs_b : nullptr;
@@ -203,7 +219,7 @@ int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t 
 #define FUSED_CASE(LR, LC, S)                                                                                   \
     case shape_key(LR, LC, S):                                                                                  \
         hipLaunchKernelGGL((k_last_fused<Geom<LR, LC, S>>), pass_grid(d, pl.log_n), dim3(TBLOCK), pass_lds(d), st, \
-                           xb, src, nb, xa, other, d, pl.log_n, tw, ls, hs, n_coeffs, n_limbs);                 \
+                           xb, src, nb, xa, other, d, pl.log_n, tw, ls, hs, n_coeffs, n_limbs, stw);                 \
         break;
         GNT_FOR_EACH_SHAPE(FUSED_CASE)
 #undef FUSED_CASE
@@ -212,15 +228,17 @@ int launch_last_fused(const Plan &pl, const u64 *xb, const u64 *limbs_b, size_t 
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_last_fused_ab(const Plan &pl, u64 *xa, const u64 *xb, const Tw &tw, hipStream_t st) {
+int launch_last_fused_ab(const Plan &pl, u64 *xa, const u64 *xb, const Tw &tw, const PassTables &tabs,
+                         hipStream_t st) {
     if (pl.npass < 2) return -1;
+    const StoreTw stw = inv_store_tw(pl, tabs, pl.npass - 1);
     const PassDesc d = pl.pass[pl.npass - 1];
     switch (shape_key(d)) {
 #define FAB_CASE(LR, LC, S)                                                                                        \
     case shape_key(LR, LC, S):                                                                                     \
         if (!HalfGeom<Geom<LR, LC, S>>::ok) return -1;                                                            \
         hipLaunchKernelGGL((k_last_fused_ab<Geom<LR, LC, S>>), dim3(2 * pass_grid(d, pl.log_n).x), dim3(TBLOCK),  \
-                           pass_lds(d), st, xa, xb, d, pl.log_n, tw);                                              \
+                           pass_lds(d), st, xa, xb, d, pl.log_n, tw, stw);                                              \
         break;
         GNT_FOR_EACH_SHAPE(FAB_CASE)
 #undef FAB_CASE
@@ -229,27 +247,61 @@ int launch_last_fused_ab(const Plan &pl, u64 *xa, const u64 *xb, const Tw &tw, h
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_inverse_rest(const Plan &pl, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
-                        hipStream_t st) {
+// One inverse pass of a strided shape in the variant `form` (the last pass's
+// inverse runs in the fused stage, so contiguous shapes never come here).
+template <class G>
+static int launch_inv_shape(const Plan &pl, const PassDesc &d, u64 *x, const Tw &tw, u64 *ls, u64 *hs,
+                            size_t n_coeffs, size_t n_limbs, int form, const StoreTw &stw, hipStream_t st) {
+    if constexpr (!G::strided) {
+        return -1;
+    } else {
+        const dim3 grid = pass_grid(d, pl.log_n);
+        const size_t lds = pass_lds(d);
+        if (ls && form == INV_STEP)
+            hipLaunchKernelGGL((k_inv_pass<G, true, INV_STEP>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
+                               n_coeffs, n_limbs, stw);
+        else if (ls)
+            hipLaunchKernelGGL((k_inv_pass<G, true, INV_PLAIN>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
+                               n_coeffs, n_limbs, stw);
+        else if (form == INV_STEP)
+            hipLaunchKernelGGL((k_inv_pass<G, false, INV_STEP>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
+                               n_coeffs, n_limbs, stw);
+        else if (form == INV_TSTORE)
+            hipLaunchKernelGGL((k_inv_pass<G, false, INV_TSTORE>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
+                               n_coeffs, n_limbs, stw);
+        else
+            hipLaunchKernelGGL((k_inv_pass<G, false, INV_PLAIN>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
+                               n_coeffs, n_limbs, stw);
+        return 0;
+    }
+}
+
+int launch_inverse_rest(const Plan &pl, u64 *x, const Tw &tw, const PassTables &tabs, u64 *Ls, u64 *Hs,
+                        size_t n_coeffs, size_t n_limbs, hipStream_t st) {
     for (int p = pl.npass - 2; p >= 0; p--) {
         const PassDesc d = pl.pass[p];
         u64 *ls = p == 0 ? Ls : nullptr;
         u64 *hs = p == 0 ? Hs : nullptr;
+        const StoreTw stw = inv_store_tw(pl, tabs, p);
+        // a pass with a table had its twiddles applied by the stage before
+        const int form = !tabs.inv[p] ? INV_STEP : stw.T ? INV_TSTORE : INV_PLAIN;
         switch (shape_key(d)) {
-#define INV_CASE(LR, LC, S)                                                                                     \
-    case shape_key(LR, LC, S):                                                                                  \
-        if (ls)                                                                                                 \
-            hipLaunchKernelGGL((k_inv_pass<Geom<LR, LC, S>, true>), pass_grid(d, pl.log_n), dim3(TBLOCK),       \
-                               pass_lds(d), st, x, d, tw, ls, hs, n_coeffs, n_limbs);                           \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_inv_pass<Geom<LR, LC, S>, false>), pass_grid(d, pl.log_n), dim3(TBLOCK),      \
-                               pass_lds(d), st, x, d, tw, ls, hs, n_coeffs, n_limbs);                           \
+#define INV_CASE(LR, LC, S)                                                                                  \
+    case shape_key(LR, LC, S):                                                                               \
+        if (launch_inv_shape<Geom<LR, LC, S>>(pl, d, x, tw, ls, hs, n_coeffs, n_limbs, form, stw, st)) return -1; \
         break;
             GNT_FOR_EACH_SHAPE(INV_CASE)
 #undef INV_CASE
             default: return -1;
         }
     }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_pass_table(u64 *T, int log_m, int log_r, bool inverse, const Tw &tw, hipStream_t st) {
+    const size_t n = (size_t)1 << log_m;
+    hipLaunchKernelGGL(k_pass_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, log_m, log_r,
+                       inverse ? 1 : 0, tw);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -47,6 +47,18 @@ constexpr int WSM_SIZE = 2 << TILE_LOG;
 // omega_{2^32}^e from the two-level table.
 GL_HD u64 root_pow(const Tw &tw, u32 e) { return gl::mul(tw.wlo[e & ((1u << TW_SPLIT) - 1)], tw.whi[e >> TW_SPLIT]); }
 
+// Entry i = n*s + j of the twiddle table of a pass with block size
+// M = 2^log_m and radix 2^log_r (row n, column j of a block, s = M / R):
+// omega_M^(j * bitrev_R(n)), the twiddle tile_twiddle steps to, or its
+// inverse.  The table is indexed by an element's global position mod M.
+GL_HD u64 pass_table_entry(const Tw &tw, u32 i, int log_m, int log_r, bool inv) {
+    const int log_s = log_m - log_r;
+    const u32 n = i >> log_s, j = i & ((1u << log_s) - 1);
+    u32 e = (u32)(((u64)j * gl::bitrev(n, log_r)) << (ROOT_LOG - log_m));
+    if (inv) e = 0u - e;
+    return root_pow(tw, e);
+}
+
 // Sub-steps of an R-point column DFT: radix <= 8 each (8 values in registers
 // keeps a thread within 64 VGPRs), as even as possible, larger ones first.
 template <int LOGR>
@@ -196,6 +208,34 @@ GL_HD void tile_store(const u64 *tile, const G &g, u64 *__restrict__ x) {
     }
 }
 
+// Store a tile with every element multiplied by T[a mod 2^log_t], a its
+// global position: a forward pass's own table, or the table of the inverse
+// pass that reads this output next (applied here so that pass loads plain
+// data).  One multiply per element instead of tile_twiddle's two, and one
+// LDS round trip less.
+template <class X, class G>
+GL_HD void tile_store_tw(const u64 *tile, const G &g, u64 *__restrict__ x, const u64 *__restrict__ T, int log_t) {
+    const u32 mt = (1u << log_t) - 1;
+    GNT_FOR(X, idx, G::tile_n) {
+        int n, c;
+        tile_rc<G>(idx, n, c);
+        const u32 a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
+        x[a] = gl::mul(tile[tix(g, n, c)], T[a & mt]);
+    }
+}
+
+// The table a tile's store applies (T null: a plain store).
+struct StoreTw {
+    const u64 *T;
+    int log_t;
+};
+
+template <class X, class G>
+GL_HD void tile_store_any(const u64 *tile, const G &g, u64 *__restrict__ x, const StoreTw &st) {
+    if (st.T) tile_store_tw<X, G>(tile, g, x, st.T, st.log_t);
+    else tile_store<X, G>(tile, g, x);
+}
+
 // Pass twiddle omega_M^(j*kc) on the tile in LDS, kc = bitrev_R(p) for row
 // p, j = j0 + c for column c (a tile's columns are consecutive, j0 = cg0 mod
 // s).  Each task owns a run of up to 16 consecutive columns of one row and
@@ -299,25 +339,41 @@ GL_HD void tile_limb_sums(const u64 *tile, const G &g, size_t n_coeffs, size_t n
 // ---- pass bodies: what each kernel runs on one tile ----
 
 // Forward pass.  With `limbs` set the tile is packed from the operand's limbs.
+// A strided pass multiplies its twiddles from its table T on the way out, or
+// steps them (T null).
 template <class X, class G>
-GL_HD void body_fwd(u64 *tile, const G &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw) {
+GL_HD void body_fwd(u64 *tile, const G &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw, const u64 *T) {
     tile_load<X, G>(tile, g, x, limbs, n_limbs);
     tile_fwd<X, G, 0>(tile, g, tw);
-    if constexpr (G::strided) tile_twiddle<X, G, false>(tile, g, tw);
+    if constexpr (G::strided) {
+        if (T) {
+            tile_store_tw<X, G>(tile, g, x, T, g.log_m);
+            return;
+        }
+        tile_twiddle<X, G, false>(tile, g, tw);
+    }
     tile_store<X, G>(tile, g, x);
 }
 
-// Inverse pass; SUMS: it is the last one and writes limb sums (a compile-time
-// choice, so each kernel carries only one of the two tails).
-template <class X, class G, bool SUMS>
+// Inverse pass; SUMS: it is the last one and writes limb sums.  STEP: a
+// strided pass whose inverse twiddles were not applied by the producer of its
+// input (it has no table) steps them after the load.  TSTORE: the store
+// applies the table of the next inverse pass (st).  Compile-time choices, so
+// each kernel carries only the phases it runs (a kernel holding both forms
+// of a phase needs registers it then spills).
+template <class X, class G, bool SUMS, bool STEP, bool TSTORE>
 GL_HD void body_inv(u64 *tile, const G &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
-                    size_t n_limbs) {
+                    size_t n_limbs, const StoreTw &st) {
     tile_load<X, G>(tile, g, x, nullptr, 0);
-    if constexpr (G::strided) tile_twiddle<X, G, true>(tile, g, tw);
+    if constexpr (G::strided && STEP) tile_twiddle<X, G, true>(tile, g, tw);
     tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
     if constexpr (SUMS) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
+    else if constexpr (TSTORE) tile_store_tw<X, G>(tile, g, x, st.T, st.log_t);
     else tile_store<X, G>(tile, g, x);
 }
+
+// Which inverse-pass variant a pass runs (InvForm::of).
+enum InvForm : int { INV_STEP = 0, INV_PLAIN = 1, INV_TSTORE = 2 };
 
 // Last forward pass of B (from xb, or packed from limbs), pointwise with A
 // (a_other at the same positions; null squares), 1/N, first inverse pass;
@@ -326,13 +382,13 @@ GL_HD void body_inv(u64 *tile, const G &g, u64 *x, const Tw &tw, u64 *Ls, u64 *H
 template <class X, class G>
 GL_HD void body_fused(u64 *tile, const G &g, const u64 *xb, const u64 *limbs, size_t n_limbs_in, u64 *xa,
                       const u64 *a_other, int log_n, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
-                      size_t n_limbs) {
+                      size_t n_limbs, const StoreTw &st) {
     tile_load<X, G>(tile, g, xb, limbs, n_limbs_in);
     tile_fwd<X, G, 0>(tile, g, tw);
     tile_pointwise<X, G>(tile, g, a_other, log_n);
     tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
     if (Ls) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
-    else tile_store<X, G>(tile, g, xa);
+    else tile_store_any<X, G>(tile, g, xa, st);
 }
 
 // Last forward pass of A and of B (both read after their earlier passes), the
@@ -355,7 +411,8 @@ GL_HD void tile_fwd_pair(u64 *tile, const G &g, const Tw &tw) {
 }
 
 template <class X, class G>
-GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_n, const Tw &tw) {
+GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_n, const Tw &tw,
+                         const StoreTw &st) {
     u64 *ta = tile, *tb = tile + G::tile_n;
     GNT_FOR(X, idx, 2 * G::tile_n) {
         const bool b = idx >= G::tile_n;
@@ -376,7 +433,7 @@ GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_
     }
     X::sync();
     tile_inv<X, G, Sub<G::LOGR>::count - 1>(ta, g, tw);
-    tile_store<X, G>(ta, g, xa);
+    tile_store_any<X, G>(ta, g, xa, st);
 }
 
 // Half-width geometry of a full-tile contiguous pass shape (the fused A+B

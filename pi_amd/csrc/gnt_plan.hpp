@@ -103,6 +103,22 @@ struct PlanChoice {
 };
 inline PlanChoice g_plan_choice[MAX_LOG_N + 1] = {};
 
+// A strided pass whose block size M is at most 2^g_table_max_log takes its
+// twiddles from a precomputed table of M entries per direction (one multiply
+// per element); larger blocks step their twiddles (two multiplies), since
+// their table would be as large as the data.  Host side; -1 disables tables.
+inline int g_table_max_log = 22;
+static inline bool pass_has_table(const PassDesc &d) {
+    return d.log_m > d.log_r && d.log_m <= g_table_max_log;
+}
+
+// Twiddle tables of a plan's passes (device pointers; null where a pass steps
+// its twiddles), filled by the runtime.
+struct PassTables {
+    const uint64_t *fwd[MAX_PASSES] = {};
+    const uint64_t *inv[MAX_PASSES] = {};
+};
+
 static inline Plan make_plan(int log_n) {
     int lr[MAX_PASSES] = {0, 0, 0, 0};
     int np = 0;

@@ -58,6 +58,13 @@ struct Ctx {
     void *server = nullptr;  // serving loop (gnt_serve.hip) while running
 
     int init_tables();
+    // Twiddle tables of pl's strided passes that have one (pass_has_table),
+    // built on first use on the compute stream and kept until shutdown; a
+    // pass whose table cannot be allocated steps its twiddles instead.
+    void pass_tables(const Plan &pl, PassTables &out);
+    void free_pass_tables();
+    std::mutex tab_mu;
+    std::map<int, uint64_t *> ptab;  // key (log_m, log_r, inverse)
     int enqueue_product(const uint64_t *da, size_t na, const uint64_t *db, size_t nb, uint64_t *dout,
                         uint32_t *dlen);
 };

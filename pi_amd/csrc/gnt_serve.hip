@@ -342,6 +342,7 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
         delete q;
         return gnt::set_error(GNT_E_TOO_LARGE, "product exceeds the transform cap");
     }
+    if (!p.linear()) c->pass_tables(p.pl, p.tabs);
     q->own_a = req->a_buf ? nullptr : (u64 *)1;
     q->own_b = (sq || req->b_buf) ? nullptr : (u64 *)1;
     q->bytes = s->footprint(q);

@@ -78,6 +78,14 @@ using gnt::Ctx;
 
 extern "C" {
 
+int gnt_set_pass_tables(int max_log_m, int *previous) {
+    if (max_log_m < -1 || max_log_m > gnt::MAX_LOG_N) return gnt::set_error(GNT_E_ARG, "max_log_m out of range");
+    std::lock_guard<std::mutex> g(gnt::g_plan_mu);
+    if (previous) *previous = gnt::g_table_max_log;
+    gnt::g_table_max_log = max_log_m;
+    return GNT_OK;
+}
+
 int gnt_set_plan(int log_n, int npass, const int *log_r) {
     if (log_n < 1 || log_n > gnt::MAX_LOG_N) return gnt::set_error(GNT_E_ARG, "log_n out of range");
     if (npass == 0) {  // back to the built-in rule

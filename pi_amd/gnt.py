@@ -36,7 +36,7 @@ EXPORTS = (
     "gnt_sync", "gnt_mul_batch", "gnt_mul_batch_serial", "gnt_plan_info", "gnt_set_timing", "gnt_get_timing",
     "gnt_batch_create", "gnt_batch_upload", "gnt_batch_run", "gnt_batch_download", "gnt_batch_info",
     "gnt_batch_destroy", "gnt_serve_start", "gnt_serve_stop", "gnt_submit", "gnt_poll", "gnt_wait",
-    "gnt_serve_stats", "gnt_set_plan", "gnt_load_plans", "gnt_tune",
+    "gnt_serve_stats", "gnt_set_plan", "gnt_load_plans", "gnt_set_pass_tables", "gnt_tune",
 )
 
 
@@ -105,6 +105,7 @@ def load() -> ctypes.CDLL:
                            ctypes.POINTER(ctypes.c_double)]
     L.gnt_serve_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
     L.gnt_set_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.gnt_set_pass_tables.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.gnt_load_plans.argtypes = [ctypes.c_char_p]
     L.gnt_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
     L.gnt_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -356,6 +357,14 @@ class Server:
 
     def stop(self):
         _check(load().gnt_serve_stop(self.ctx.h))
+
+
+def set_pass_tables(max_log_m: int) -> int:
+    """Largest pass block size (log2) whose twiddles come from a table; -1
+    disables tables.  Returns the previous setting (process-wide)."""
+    prev = ctypes.c_int()
+    _check(load().gnt_set_pass_tables(max_log_m, ctypes.byref(prev)))
+    return prev.value
 
 
 def set_plan(log_n: int, log_r=None):

@@ -6,6 +6,7 @@
 // twiddle logic of the GPU path on the CPU against the oracle and against a
 // direct DFT; the GPU tests then check the device against the same oracle.
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "../../pi_amd/csrc/gnt_pass.hpp"
@@ -23,43 +24,80 @@ void tables() {
 
 Tw tw() { return Tw{g_lo.data(), g_hi.data(), g_sm.data()}; }
 
+// Pass twiddle tables, as the runtime builds them on the device.
+std::map<int, std::vector<u64>> g_ptab;
+
+PassTables tables_for(const Plan &pl) {
+    PassTables t;
+    for (int p = 0; p < pl.npass; p++) {
+        const PassDesc &d = pl.pass[p];
+        if (!pass_has_table(d)) continue;
+        for (int inv = 0; inv < 2; inv++) {
+            std::vector<u64> &T = g_ptab[(d.log_m << 8) | (d.log_r << 1) | inv];
+            if (T.empty()) {
+                T.resize((size_t)1 << d.log_m);
+                for (size_t i = 0; i < T.size(); i++) T[i] = pass_table_entry(tw(), (u32)i, d.log_m, d.log_r, inv != 0);
+            }
+            (inv ? t.inv : t.fwd)[p] = T.data();
+        }
+    }
+    return t;
+}
+
+StoreTw inv_store_tw(const Plan &pl, const PassTables &tabs, int p) {
+    if (p < 1 || !tabs.inv[p - 1]) return StoreTw{nullptr, 0};
+    return StoreTw{tabs.inv[p - 1], pl.pass[p - 1].log_m};
+}
+
 using HX = HostExec;
 
 // The three kernel bodies of gnt_kernels.hip, run tile after tile.
 template <class G>
-void k_fwd_pass(const PassDesc &d, int log_n, u64 *x, const u64 *limbs, size_t n_limbs) {
+void k_fwd_pass(const PassDesc &d, int log_n, u64 *x, const u64 *limbs, size_t n_limbs, const u64 *T) {
     std::vector<u64> tile(G::tile_n);
     const size_t grid = ((size_t)1 << log_n) / G::tile_n;
-    for (size_t t = 0; t < grid; t++) body_fwd<HX, G>(tile.data(), make_geom<G>(d, t), x, limbs, n_limbs, tw());
+    for (size_t t = 0; t < grid; t++) body_fwd<HX, G>(tile.data(), make_geom<G>(d, t), x, limbs, n_limbs, tw(), T);
 }
 
-template <class G>
-void k_inv_pass(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
+template <class G, bool SUMS, bool STEP, bool TSTORE>
+void k_inv_variant(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
+                   const StoreTw &st) {
     std::vector<u64> tile(G::tile_n);
     const size_t grid = ((size_t)1 << log_n) / G::tile_n;
     for (size_t t = 0; t < grid; t++)
-        if (Ls) body_inv<HX, G, true>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
-        else body_inv<HX, G, false>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs);
+        body_inv<HX, G, SUMS, STEP, TSTORE>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs, st);
+}
+
+// The variant the library launches for this pass (launch_inverse_rest).
+template <class G>
+void k_inv_pass(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
+                bool step_tw, const StoreTw &st) {
+    step_tw = step_tw && G::strided;  // a contiguous pass has no twiddles to step
+    if (Ls && step_tw) k_inv_variant<G, true, true, false>(d, log_n, x, Ls, Hs, n_coeffs, n_limbs, st);
+    else if (Ls) k_inv_variant<G, true, false, false>(d, log_n, x, Ls, Hs, n_coeffs, n_limbs, st);
+    else if (step_tw) k_inv_variant<G, false, true, false>(d, log_n, x, Ls, Hs, n_coeffs, n_limbs, st);
+    else if (st.T) k_inv_variant<G, false, false, true>(d, log_n, x, Ls, Hs, n_coeffs, n_limbs, st);
+    else k_inv_variant<G, false, false, false>(d, log_n, x, Ls, Hs, n_coeffs, n_limbs, st);
 }
 
 template <class G>
 void k_last_fused(const PassDesc &d, int log_n, const u64 *xb, const u64 *limbs, size_t n_in, u64 *xa,
-                  const u64 *other, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
+                  const u64 *other, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs, const StoreTw &st) {
     std::vector<u64> tile(G::tile_n);
     const size_t grid = ((size_t)1 << log_n) / G::tile_n;
     for (size_t t = 0; t < grid; t++)
         body_fused<HX, G>(tile.data(), make_geom<G>(d, t), xb, limbs, n_in, xa, other, log_n, tw(), Ls, Hs,
-                          n_coeffs, n_limbs);
+                          n_coeffs, n_limbs, st);
 }
 
 template <class G>
-void k_last_fused_ab(const PassDesc &d, int log_n, u64 *xa, const u64 *xb) {
+void k_last_fused_ab(const PassDesc &d, int log_n, u64 *xa, const u64 *xb, const StoreTw &st) {
     using H = typename HalfGeom<G>::type;
     if constexpr (HalfGeom<G>::ok) {
         std::vector<u64> tile(2 * H::tile_n);
         const size_t grid = ((size_t)1 << log_n) / H::tile_n;
         for (size_t t = 0; t < grid; t++)
-            body_fused_ab<HX, H>(tile.data(), make_geom<H>(d, t), xa, xb, log_n, tw());
+            body_fused_ab<HX, H>(tile.data(), make_geom<H>(d, t), xa, xb, log_n, tw(), st);
     }
 }
 
@@ -69,24 +107,29 @@ void k_last_fused_ab(const PassDesc &d, int log_n, u64 *xa, const u64 *xb) {
         default: break;                                              \
     }
 
-void forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs) {
+void forward(const Plan &pl, int npasses, u64 *x, const u64 *limbs, size_t n_limbs, const PassTables &tabs) {
     for (int p = 0; p < npasses; p++) {
         const PassDesc &d = pl.pass[p];
         const u64 *src = p == 0 ? limbs : nullptr;
 #define F(LR, LC, S) \
-    case shape_key(LR, LC, S): k_fwd_pass<Geom<LR, LC, S>>(d, pl.log_n, x, src, n_limbs); break;
+    case shape_key(LR, LC, S): k_fwd_pass<Geom<LR, LC, S>>(d, pl.log_n, x, src, n_limbs, tabs.fwd[p]); break;
         DISPATCH(d, F)
 #undef F
     }
 }
 
-void inverse(const Plan &pl, int from, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs) {
+void inverse(const Plan &pl, int from, u64 *x, u64 *Ls, u64 *Hs, size_t n_coeffs, size_t n_limbs,
+             const PassTables &tabs) {
     for (int p = from; p >= 0; p--) {
         const PassDesc &d = pl.pass[p];
         u64 *ls = p == 0 ? Ls : nullptr;
         u64 *hs = p == 0 ? Hs : nullptr;
-#define F(LR, LC, S) \
-    case shape_key(LR, LC, S): k_inv_pass<Geom<LR, LC, S>>(d, pl.log_n, x, ls, hs, n_coeffs, n_limbs); break;
+        const bool step_tw = tabs.inv[p] == nullptr;
+        const StoreTw st = inv_store_tw(pl, tabs, p);
+#define F(LR, LC, S)                                                                                          \
+    case shape_key(LR, LC, S):                                                                                \
+        k_inv_pass<Geom<LR, LC, S>>(d, pl.log_n, x, ls, hs, n_coeffs, n_limbs, step_tw, st);                 \
+        break;
         DISPATCH(d, F)
 #undef F
     }
@@ -106,13 +149,13 @@ extern "C" {
 void emu_forward(u64 *x, int log_n) {
     tables();
     Plan pl = make_plan(log_n);
-    forward(pl, pl.npass, x, nullptr, 0);
+    forward(pl, pl.npass, x, nullptr, 0, tables_for(pl));
 }
 
 void emu_inverse(u64 *x, int log_n) {
     tables();
     Plan pl = make_plan(log_n);
-    inverse(pl, pl.npass - 1, x, nullptr, nullptr, 0, 0);
+    inverse(pl, pl.npass - 1, x, nullptr, nullptr, 0, 0, tables_for(pl));
 }
 
 // omega_N of the library (primitive N-th root used by the transforms).
@@ -135,6 +178,15 @@ int emu_set_plan(int log_n, int npass, const int *log_r) {
     g_plan_choice[log_n] = c;
     return 0;
 }
+
+// Entry i of the twiddle table of a pass (block 2^log_m, radix 2^log_r).
+u64 emu_pass_table_entry(u32 i, int log_m, int log_r, int inv) {
+    tables();
+    return pass_table_entry(tw(), i, log_m, log_r, inv != 0);
+}
+
+// Largest pass block size (log) that takes its twiddles from a table; -1: none.
+void emu_set_table_max_log(int max_log) { g_table_max_log = max_log; }
 
 void emu_plan(int log_n, int *npass, int *log_r, int *log_c) {
     Plan pl = make_plan(log_n);
@@ -160,18 +212,20 @@ size_t emu_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
     const size_t nl = na + nb;
     std::vector<u64> Ls(nl), Hs(nl);
     const int P = pl.npass;
+    const PassTables tabs = tables_for(pl);
+    const StoreTw fst = inv_store_tw(pl, tabs, P - 1);
     // the same launch sequence as Ctx::enqueue_product
     const bool ab = !sqr && P > 1;
     if (sqr) {
-        if (P > 1) forward(pl, P - 1, xa.data(), a, na);
+        if (P > 1) forward(pl, P - 1, xa.data(), a, na, tabs);
     } else {
-        forward(pl, ab ? P - 1 : P, xa.data(), a, na);
-        if (P > 1) forward(pl, P - 1, xb.data(), b, nb);
+        forward(pl, ab ? P - 1 : P, xa.data(), a, na, tabs);
+        if (P > 1) forward(pl, P - 1, xb.data(), b, nb, tabs);
     }
     if (ab) {
         const PassDesc &d = pl.pass[P - 1];
 #define F(LR, LC, S) \
-    case shape_key(LR, LC, S): k_last_fused_ab<Geom<LR, LC, S>>(d, log_n, xa.data(), xb.data()); break;
+    case shape_key(LR, LC, S): k_last_fused_ab<Geom<LR, LC, S>>(d, log_n, xa.data(), xb.data(), fst); break;
         DISPATCH(d, F)
 #undef F
     } else {
@@ -183,12 +237,12 @@ size_t emu_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
 #define F(LR, LC, S)                                                                                       \
     case shape_key(LR, LC, S):                                                                             \
         k_last_fused<Geom<LR, LC, S>>(d, log_n, bsrc, blimbs, sqr ? na : nb, xa.data(), other,             \
-                                      only ? Ls.data() : nullptr, only ? Hs.data() : nullptr, n_out, nl); \
+                                      only ? Ls.data() : nullptr, only ? Hs.data() : nullptr, n_out, nl, fst); \
         break;
         DISPATCH(d, F)
 #undef F
     }
-    inverse(pl, P - 2, xa.data(), Ls.data(), Hs.data(), n_out, nl);
+    inverse(pl, P - 2, xa.data(), Ls.data(), Hs.data(), n_out, nl, tabs);
     // carry: the same limb rule as the kernels, run serially
     u64 Hprev = 0;
     u32 carry = 0;

@@ -177,3 +177,45 @@ def test_every_split_gives_the_same_product(emu, orc, log_n):
     # an uncovered split is refused
     bad = (ctypes.c_int * 4)(6, log_n - 6, 0, 0)
     assert E.emu_set_plan(log_n, 2, bad) != 0
+
+
+def test_pass_table_entries_are_the_stepped_twiddles(emu):
+    """Entry n*s + j of a pass's table is omega_M^(j * bitrev_R(n)) (inverse:
+    its reciprocal), the twiddle the stepping sweep produces for row n,
+    column j of a block."""
+    E = emu
+    E.emu_pass_table_entry.restype = U
+    E.emu_pass_table_entry.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    rnd = random.Random(11)
+    for log_m, log_r in [(14, 7), (19, 9), (22, 10), (17, 8)]:
+        w = emu.emu_root(log_m)
+        s = 1 << (log_m - log_r)
+        for _ in range(40):
+            n, j = rnd.randrange(1 << log_r), rnd.randrange(s)
+            kc = int(format(n, f"0{log_r}b")[::-1], 2)
+            want = pow(w, j * kc, P)
+            assert E.emu_pass_table_entry(n * s + j, log_m, log_r, 0) == want
+            assert E.emu_pass_table_entry(n * s + j, log_m, log_r, 1) * want % P == 1
+
+
+@pytest.mark.parametrize("max_log", [-1, 14, 30])
+def test_table_and_stepped_twiddles_give_the_same_products(emu, orc, max_log):
+    """Passes with a table multiply their twiddles from it (inverse ones in the
+    store of the stage before); passes without one step them.  Every mix of
+    the two must give the exact product and the exact transform."""
+    E = emu
+    E.emu_set_table_max_log.argtypes = [ctypes.c_int]
+    try:
+        E.emu_set_table_max_log(max_log)
+        for la, lb in [(2048, 2048), (5000, 3000), (1 << 15, 1 << 15), (1 << 18, 1 << 17)]:
+            a, b = orc.splitmix64(la, la), orc.splitmix64(lb + 7, lb)
+            assert np.array_equal(emu_mul(E, a, b), orc.mul_mag_ntt(a, b)), (la, lb)
+            assert np.array_equal(emu_mul(E, a), orc.mul_mag_ntt(a, a)), la
+        n = 1 << 17
+        x = np.random.default_rng(3).integers(0, 2**63, size=n, dtype=np.uint64)
+        y = x.copy()
+        E.emu_forward(y.ctypes.data_as(PU), 17)
+        E.emu_inverse(y.ctypes.data_as(PU), 17)
+        assert all(int(y[i]) == int(x[i]) * n % P for i in range(0, n, 991))
+    finally:
+        E.emu_set_table_max_log(22)

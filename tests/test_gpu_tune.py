@@ -61,3 +61,24 @@ def test_tune_writes_loadable_cache(ctx, orc, tmp_path):
     finally:
         for n in range(12, 19):
             gnt.set_plan(n)
+
+
+@pytest.mark.parametrize("max_log", [-1, 14, 30])
+def test_table_and_stepped_twiddles_exact(ctx, orc, max_log):
+    """Passes up to 2^max_log take twiddles from tables (the inverse ones applied
+    by the store of the stage before), larger ones step them: every mix gives
+    the exact product, single and batched."""
+    from pi_amd import gnt
+    prev = gnt.set_pass_tables(max_log)
+    try:
+        for la, lb in [(3000, 2000), (1 << 15, (1 << 15) - 9), (1 << 18, 1 << 17), (1 << 21, 1 << 20)]:
+            print(f"max_log {max_log} sizes {la} {lb}", flush=True)
+            a, b = orc.splitmix64(la, la), orc.splitmix64(lb + 3, lb)
+            assert np.array_equal(ctx.mul_mag_ntt(a, b), orc.gmp_mul(a, b)), (la, lb)
+            assert np.array_equal(ctx.sqr(a), orc.gmp_sqr(a)), la
+        pairs = [(orc.splitmix64(n, n), orc.splitmix64(n + 1, n // 3 + 1)) for n in (700, 9000, 70000, 300000)]
+        got = ctx.mul_batch(pairs)
+        for (a, b), r in zip(pairs, got):
+            assert np.array_equal(r, orc.gmp_mul(a, b))
+    finally:
+        gnt.set_pass_tables(prev)
