@@ -49,7 +49,7 @@ GL_HD u64 root_pow(const Tw &tw, u32 e) { return gl::mul(tw.wlo[e & ((1u << TW_S
 
 // Entry i = n*s + j of the twiddle table of a pass with block size
 // M = 2^log_m and radix 2^log_r (row n, column j of a block, s = M / R):
-// omega_M^(j * bitrev_R(n)), the twiddle tile_twiddle steps to, or its
+// omega_M^(j * bitrev_R(n)), the twiddle tile_store_step steps to, or its
 // inverse.  The table is indexed by an element's global position mod M.
 GL_HD u64 pass_table_entry(const Tw &tw, u32 i, int log_m, int log_r, bool inv) {
     const int log_s = log_m - log_r;
@@ -91,12 +91,16 @@ GL_HD G make_geom(const PassDesc &d, size_t tile_index) {
     return g;
 }
 
-// LDS position of tile element (row n, column c).  Columns are XOR-swizzled
-// by the row so that walking down a column (the contiguous last pass) and
-// walking along a row both hit distinct banks.
+// LDS position of tile element (row n, column c).  A strided pass only walks
+// its tile along rows (loads, stores and the column DFT tasks all put
+// consecutive lanes on consecutive columns), so it is plain row-major and a
+// task's R addresses are one base plus constant offsets.  The contiguous last
+// pass also walks down columns (its rows are adjacent in memory), so its
+// columns are XOR-swizzled by the row: both walks then hit distinct banks.
 template <class G>
 GL_HD int tix(const G &, int n, int c) {
-    return (n << G::log_c) + (c ^ (n & ((1 << G::log_c) - 1)));
+    if constexpr (G::strided) return (n << G::log_c) + c;
+    else return (n << G::log_c) + (c ^ (n & ((1 << G::log_c) - 1)));
 }
 
 // Global index of tile element (row n, tile column c).  Transforms hold at
@@ -211,8 +215,7 @@ GL_HD void tile_store(const u64 *tile, const G &g, u64 *__restrict__ x) {
 // Store a tile with every element multiplied by T[a mod 2^log_t], a its
 // global position: a forward pass's own table, or the table of the inverse
 // pass that reads this output next (applied here so that pass loads plain
-// data).  One multiply per element instead of tile_twiddle's two, and one
-// LDS round trip less.
+// data).  One multiply per element instead of the two of stepped twiddles.
 template <class X, class G>
 GL_HD void tile_store_tw(const u64 *tile, const G &g, u64 *__restrict__ x, const u64 *__restrict__ T, int log_t) {
     const u32 mt = (1u << log_t) - 1;
@@ -236,31 +239,60 @@ GL_HD void tile_store_any(const u64 *tile, const G &g, u64 *__restrict__ x, cons
     else tile_store<X, G>(tile, g, x);
 }
 
-// Pass twiddle omega_M^(j*kc) on the tile in LDS, kc = bitrev_R(p) for row
-// p, j = j0 + c for column c (a tile's columns are consecutive, j0 = cg0 mod
-// s).  Each task owns a run of up to 16 consecutive columns of one row and
-// steps the twiddle by omega_M^kc along it, so a run costs two table lookups.
-// INV applies the inverse twiddle.  Global loads and stores stay coalesced.
-template <class X, class G, bool INV>
-GL_HD void tile_twiddle(u64 *tile, const G &g, const Tw &tw) {
-    constexpr int LOGR = G::LOGR;
-    const int lc = G::log_c;
-    const int lch = lc < 4 ? lc : 4;
-    const int CH = 1 << lch;
-    const size_t j0 = g.cg0 & (((size_t)1 << g.log_s) - 1);
+// Pass twiddles of a strided pass without a table, stepped in registers on
+// the way out (forward) or in (inverse).  Element (row n, column j of the
+// block) takes omega_M^(j * kc), kc = bitrev_R(n).  A task owns one column
+// and the STEP_ROWS rows whose kc run over [STEP_ROWS * grp, +STEP_ROWS), so
+// the twiddle advances by omega_M^j from one to the next: two table lookups
+// and one multiply per element besides the element's own.  Consecutive tasks
+// take consecutive columns, so every LDS access walks a row and every global
+// access is a contiguous run of 2^log_c elements.
+constexpr int STEP_ROWS = 8;
+
+template <class G>
+GL_HD void step_task(const G &g, int L, int &c, u32 &kc0, u32 &j) {
+    c = L & ((1 << G::log_c) - 1);
+    kc0 = (u32)(L >> G::log_c) * STEP_ROWS;
+    j = (u32)((g.cg0 + (size_t)c) & (((size_t)1 << g.log_s) - 1));
+}
+
+template <class X, class G>
+GL_HD void tile_store_step(const u64 *tile, const G &g, u64 *__restrict__ x, const Tw &tw) {
+    static_assert(G::strided && (1 << G::LOGR) % STEP_ROWS == 0, "stepped twiddles: strided passes only");
     const int sh = ROOT_LOG - g.log_m;
-    GNT_FOR(X, L, G::tile_n >> lch) {
-        const int p = L >> (lc - lch);
-        const int c0 = (L & ((1 << (lc - lch)) - 1)) << lch;
-        const u32 kc = gl::bitrev((u32)p, LOGR);
-        u32 ew = (u32)(((u64)(j0 + (size_t)c0) * kc) << sh);
-        u32 eu = (u32)((u64)kc << sh);
-        if (INV) { ew = 0u - ew; eu = 0u - eu; }
-        u64 w = root_pow(tw, ew);
-        const u64 u = root_pow(tw, eu);
-        for (int c = 0; c < CH; c++) {
-            const int t = tix(g, p, c0 + c);
-            tile[t] = gl::mul(tile[t], w);
+    GNT_FOR(X, L, G::tile_n / STEP_ROWS) {
+        int c;
+        u32 kc0, j;
+        step_task(g, L, c, kc0, j);
+        u64 w = root_pow(tw, (j * kc0) << sh);
+        const u64 u = root_pow(tw, j << sh);
+#pragma unroll
+        for (int k = 0; k < STEP_ROWS; k++) {
+            const int n = (int)gl::bitrev(kc0 + k, G::LOGR);
+            x[tile_addr(g.cg0, n, c, g.log_m, g.log_s)] = gl::mul(tile[tix(g, n, c)], w);
+            w = gl::mul(w, u);
+        }
+    }
+}
+
+// Inverse: the loaded values times omega_M^(-j * kc), into the tile.
+template <class X, class G>
+GL_HD void tile_load_step(u64 *tile, const G &g, const u64 *__restrict__ x, const Tw &tw) {
+    static_assert(G::strided && (1 << G::LOGR) % STEP_ROWS == 0, "stepped twiddles: strided passes only");
+    const int sh = ROOT_LOG - g.log_m;
+    GNT_FOR(X, L, G::tile_n / STEP_ROWS) {
+        int c;
+        u32 kc0, j;
+        step_task(g, L, c, kc0, j);
+        u64 v[STEP_ROWS];
+#pragma unroll
+        for (int k = 0; k < STEP_ROWS; k++)
+            v[k] = x[tile_addr(g.cg0, (int)gl::bitrev(kc0 + k, G::LOGR), c, g.log_m, g.log_s)];
+        u64 w = root_pow(tw, 0u - ((j * kc0) << sh));
+        const u64 u = root_pow(tw, 0u - (j << sh));
+#pragma unroll
+        for (int k = 0; k < STEP_ROWS; k++) {
+            tile[tix(g, (int)gl::bitrev(kc0 + k, G::LOGR), c)] = gl::mul(v[k], w);
             w = gl::mul(w, u);
         }
     }
@@ -339,33 +371,31 @@ GL_HD void tile_limb_sums(const u64 *tile, const G &g, size_t n_coeffs, size_t n
 // ---- pass bodies: what each kernel runs on one tile ----
 
 // Forward pass.  With `limbs` set the tile is packed from the operand's limbs.
-// A strided pass multiplies its twiddles from its table T on the way out, or
-// steps them (T null).
+// A strided pass multiplies its twiddles on the way out, from its table T or
+// stepped (T null).
 template <class X, class G>
 GL_HD void body_fwd(u64 *tile, const G &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw, const u64 *T) {
     tile_load<X, G>(tile, g, x, limbs, n_limbs);
     tile_fwd<X, G, 0>(tile, g, tw);
     if constexpr (G::strided) {
-        if (T) {
-            tile_store_tw<X, G>(tile, g, x, T, g.log_m);
-            return;
-        }
-        tile_twiddle<X, G, false>(tile, g, tw);
+        if (T) tile_store_tw<X, G>(tile, g, x, T, g.log_m);
+        else tile_store_step<X, G>(tile, g, x, tw);
+    } else {
+        tile_store<X, G>(tile, g, x);
     }
-    tile_store<X, G>(tile, g, x);
 }
 
 // Inverse pass; SUMS: it is the last one and writes limb sums.  STEP: a
 // strided pass whose inverse twiddles were not applied by the producer of its
-// input (it has no table) steps them after the load.  TSTORE: the store
+// input (it has no table) steps them as it loads.  TSTORE: the store
 // applies the table of the next inverse pass (st).  Compile-time choices, so
 // each kernel carries only the phases it runs (a kernel holding both forms
 // of a phase needs registers it then spills).
 template <class X, class G, bool SUMS, bool STEP, bool TSTORE>
 GL_HD void body_inv(u64 *tile, const G &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
                     size_t n_limbs, const StoreTw &st) {
-    tile_load<X, G>(tile, g, x, nullptr, 0);
-    if constexpr (G::strided && STEP) tile_twiddle<X, G, true>(tile, g, tw);
+    if constexpr (G::strided && STEP) tile_load_step<X, G>(tile, g, x, tw);
+    else tile_load<X, G>(tile, g, x, nullptr, 0);
     tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
     if constexpr (SUMS) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
     else if constexpr (TSTORE) tile_store_tw<X, G>(tile, g, x, st.T, st.log_t);
