@@ -180,7 +180,9 @@ void Ctx::pass_tables(const Plan &pl, PassTables &out) {
         const PassDesc &d = pl.pass[p];
         if (!pass_has_table(d)) continue;
         for (int inv = 0; inv < 2; inv++) {
-            const int key = (d.log_m << 8) | (d.log_r << 1) | inv;
+            // the inverse of pass 0 carries the transform's 1/N (inv_scale_e)
+            const int scale_e = (inv && p == 0) ? inv_scale_e(d.log_m) : 0;
+            const int key = (d.log_m << 8) | ((scale_e ? 1 : 0) << 6) | (d.log_r << 1) | inv;
             auto it = ptab.find(key);
             uint64_t *T = nullptr;
             if (it != ptab.end()) {
@@ -189,7 +191,7 @@ void Ctx::pass_tables(const Plan &pl, PassTables &out) {
                 if (hipMalloc(&T, sizeof(uint64_t) << d.log_m) != hipSuccess) {
                     (void)hipGetLastError();
                     T = nullptr;
-                } else if (launch_pass_table(T, d.log_m, d.log_r, inv != 0, tw(), stream) != 0) {
+                } else if (launch_pass_table(T, d.log_m, d.log_r, inv != 0, scale_e, tw(), stream) != 0) {
                     (void)hipFree(T);
                     T = nullptr;
                 }

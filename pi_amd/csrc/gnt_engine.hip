@@ -66,16 +66,17 @@ __global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     }
     const G g = make_geom<G>(it.d, t - it.tile0);
     const StoreTw st{it.T, it.log_t};
+    const int scale_e = it.d.log_m == it.log_n ? inv_scale_e(it.log_n) : 0;  // pass 0 carries 1/N
     if constexpr (KIND == K_FWD) body_fwd<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, tw, it.T);
-    else if constexpr (KIND == K_INV) body_inv<X, G, false, true, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, st);
+    else if constexpr (KIND == K_INV) body_inv<X, G, false, true, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, scale_e, st);
     else if constexpr (KIND == K_INV_PLAIN)
-        body_inv<X, G, false, false, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, st);
+        body_inv<X, G, false, false, false>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, scale_e, st);
     else if constexpr (KIND == K_INV_TSTORE)
-        body_inv<X, G, false, false, true>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, st);
+        body_inv<X, G, false, false, true>(tile, g, it.x, tw, nullptr, nullptr, 0, 0, scale_e, st);
     else if constexpr (KIND == K_INV_SUMS)
-        body_inv<X, G, true, true, false>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs, st);
+        body_inv<X, G, true, true, false>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs, scale_e, st);
     else if constexpr (KIND == K_INV_SUMS_PLAIN)
-        body_inv<X, G, true, false, false>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs, st);
+        body_inv<X, G, true, false, false>(tile, g, it.x, tw, it.Ls, it.Hs, it.n_coeffs, it.n_limbs, scale_e, st);
     else body_fused<X, G>(tile, g, it.x, it.limbs, it.n_limbs_in, it.xa, it.other, it.log_n, tw, it.Ls, it.Hs,
                           it.n_coeffs, it.n_limbs, st);
 }

@@ -54,10 +54,11 @@ __global__ GNT_TBOUNDS void k_fwd_pass_pair(u64 *__restrict__ xa, const u64 *__r
 // Inverse pass; SUMS: the last one, writing limb sums.
 template <class G, bool SUMS, int FORM>
 __global__ GNT_TBOUNDS void k_inv_pass(u64 *__restrict__ x, PassDesc d, Tw tw, u64 *__restrict__ Ls,
-                                       u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs, StoreTw st) {
+                                       u64 *__restrict__ Hs, size_t n_coeffs, size_t n_limbs, int scale_e,
+                                       StoreTw st) {
     extern __shared__ u64 tile[];
     body_inv<X, G, SUMS, FORM == INV_STEP, FORM == INV_TSTORE>(tile, make_geom<G>(d, blockIdx.x), x, tw, Ls, Hs,
-                                                               n_coeffs, n_limbs, st);
+                                                               n_coeffs, n_limbs, scale_e, st);
 }
 
 // Last forward pass of B (read from xb, or packed from limbs), pointwise with
@@ -140,9 +141,9 @@ __global__ __launch_bounds__(CARRY_RIPPLE_BLOCK) void k_carry_ripple(u64 *__rest
     if (threadIdx.x == 0 && hi) atomicMax(out_len, hi);
 }
 
-__global__ void k_pass_table(u64 *__restrict__ T, int log_m, int log_r, int inv, Tw tw) {
+__global__ void k_pass_table(u64 *__restrict__ T, int log_m, int log_r, int inv, int scale_e, Tw tw) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ((size_t)1 << log_m)) T[i] = pass_table_entry(tw, (u32)i, log_m, log_r, inv != 0);
+    if (i < ((size_t)1 << log_m)) T[i] = pass_table_entry(tw, (u32)i, log_m, log_r, inv != 0, scale_e);
 }
 
 __global__ void k_zero_u32(u32 *p, int n) {
@@ -252,6 +253,8 @@ int launch_last_fused_ab(const Plan &pl, u64 *xa, const u64 *xb, const Tw &tw, c
 template <class G>
 static int launch_inv_shape(const Plan &pl, const PassDesc &d, u64 *x, const Tw &tw, u64 *ls, u64 *hs,
                             size_t n_coeffs, size_t n_limbs, int form, const StoreTw &stw, hipStream_t st) {
+    // pass 0 (block N) carries the transform's 1/N in its stepped twiddles
+    const int scale_e = d.log_m == pl.log_n ? inv_scale_e(pl.log_n) : 0;
     if constexpr (!G::strided) {
         return -1;
     } else {
@@ -259,19 +262,19 @@ static int launch_inv_shape(const Plan &pl, const PassDesc &d, u64 *x, const Tw 
         const size_t lds = pass_lds(d);
         if (ls && form == INV_STEP)
             hipLaunchKernelGGL((k_inv_pass<G, true, INV_STEP>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
-                               n_coeffs, n_limbs, stw);
+                               n_coeffs, n_limbs, scale_e, stw);
         else if (ls)
             hipLaunchKernelGGL((k_inv_pass<G, true, INV_PLAIN>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
-                               n_coeffs, n_limbs, stw);
+                               n_coeffs, n_limbs, scale_e, stw);
         else if (form == INV_STEP)
             hipLaunchKernelGGL((k_inv_pass<G, false, INV_STEP>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
-                               n_coeffs, n_limbs, stw);
+                               n_coeffs, n_limbs, scale_e, stw);
         else if (form == INV_TSTORE)
             hipLaunchKernelGGL((k_inv_pass<G, false, INV_TSTORE>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
-                               n_coeffs, n_limbs, stw);
+                               n_coeffs, n_limbs, scale_e, stw);
         else
             hipLaunchKernelGGL((k_inv_pass<G, false, INV_PLAIN>), grid, dim3(TBLOCK), lds, st, x, d, tw, ls, hs,
-                               n_coeffs, n_limbs, stw);
+                               n_coeffs, n_limbs, scale_e, stw);
         return 0;
     }
 }
@@ -298,10 +301,10 @@ int launch_inverse_rest(const Plan &pl, u64 *x, const Tw &tw, const PassTables &
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_pass_table(u64 *T, int log_m, int log_r, bool inverse, const Tw &tw, hipStream_t st) {
+int launch_pass_table(u64 *T, int log_m, int log_r, bool inverse, int scale_e, const Tw &tw, hipStream_t st) {
     const size_t n = (size_t)1 << log_m;
     hipLaunchKernelGGL(k_pass_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, log_m, log_r,
-                       inverse ? 1 : 0, tw);
+                       inverse ? 1 : 0, scale_e, tw);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -36,7 +36,7 @@ int launch_last_fused_ab(const Plan &pl, uint64_t *xa, const uint64_t *xb, const
 int launch_inverse_rest(const Plan &pl, uint64_t *x, const Tw &tw, const PassTables &tabs, uint64_t *Ls,
                         uint64_t *Hs, size_t n_coeffs, size_t n_limbs, hipStream_t st);
 // T[i] = pass_table_entry(i) for i < 2^log_m (a pass's twiddle table).
-int launch_pass_table(uint64_t *T, int log_m, int log_r, bool inverse, const Tw &tw, hipStream_t st);
+int launch_pass_table(uint64_t *T, int log_m, int log_r, bool inverse, int scale_e, const Tw &tw, hipStream_t st);
 // Exact carry of the limb sums into n_limbs limbs; *out_len receives the
 // trimmed length.  tile_gp / tile_cin hold carry_tiles(n_limbs) bytes.
 size_t carry_tiles(size_t n_limbs);

@@ -50,13 +50,16 @@ GL_HD u64 root_pow(const Tw &tw, u32 e) { return gl::mul(tw.wlo[e & ((1u << TW_S
 // Entry i = n*s + j of the twiddle table of a pass with block size
 // M = 2^log_m and radix 2^log_r (row n, column j of a block, s = M / R):
 // omega_M^(j * bitrev_R(n)), the twiddle tile_store_step steps to, or its
-// inverse.  The table is indexed by an element's global position mod M.
-GL_HD u64 pass_table_entry(const Tw &tw, u32 i, int log_m, int log_r, bool inv) {
+// inverse, times 2^scale_e.  The table is indexed by an element's global
+// position mod M.  The inverse of pass 0 (M = N) of a plan of two or more
+// passes carries the inverse transform's 1/N = 2^(192 - log N) this way.
+GL_HD u64 pass_table_entry(const Tw &tw, u32 i, int log_m, int log_r, bool inv, int scale_e = 0) {
     const int log_s = log_m - log_r;
     const u32 n = i >> log_s, j = i & ((1u << log_s) - 1);
     u32 e = (u32)(((u64)j * gl::bitrev(n, log_r)) << (ROOT_LOG - log_m));
     if (inv) e = 0u - e;
-    return root_pow(tw, e);
+    const u64 w = root_pow(tw, e);
+    return scale_e ? gl::mul_pow2(w, scale_e) : w;
 }
 
 // Sub-steps of an R-point column DFT: radix <= 8 each (8 values in registers
@@ -275,9 +278,10 @@ GL_HD void tile_store_step(const u64 *tile, const G &g, u64 *__restrict__ x, con
     }
 }
 
-// Inverse: the loaded values times omega_M^(-j * kc), into the tile.
+// Inverse: the loaded values times omega_M^(-j * kc) * 2^scale_e, into the
+// tile (scale_e: the 1/N the inverse pass 0 carries, or 0).
 template <class X, class G>
-GL_HD void tile_load_step(u64 *tile, const G &g, const u64 *__restrict__ x, const Tw &tw) {
+GL_HD void tile_load_step(u64 *tile, const G &g, const u64 *__restrict__ x, const Tw &tw, int scale_e) {
     static_assert(G::strided && (1 << G::LOGR) % STEP_ROWS == 0, "stepped twiddles: strided passes only");
     const int sh = ROOT_LOG - g.log_m;
     GNT_FOR(X, L, G::tile_n / STEP_ROWS) {
@@ -289,6 +293,7 @@ GL_HD void tile_load_step(u64 *tile, const G &g, const u64 *__restrict__ x, cons
         for (int k = 0; k < STEP_ROWS; k++)
             v[k] = x[tile_addr(g.cg0, (int)gl::bitrev(kc0 + k, G::LOGR), c, g.log_m, g.log_s)];
         u64 w = root_pow(tw, 0u - ((j * kc0) << sh));
+        if (scale_e) w = gl::mul_pow2(w, scale_e);
         const u64 u = root_pow(tw, 0u - (j << sh));
 #pragma unroll
         for (int k = 0; k < STEP_ROWS; k++) {
@@ -320,18 +325,26 @@ GL_HD void tile_inv(u64 *tile, const G &g, const Tw &tw) {
     }
 }
 
-// tile *= other (at the same global positions, or the tile itself) * 2^(192 - log_n):
-// the pointwise product with the 1/N of the inverse (ntt.rs:845-853, 376-391).
+// The exponent e of the 1/N = 2^e of the inverse transform (ntt.rs:376-391),
+// applied by the pointwise product of a single-pass plan; a plan of two or
+// more passes folds it into the inverse twiddles of pass 0 (its block is N):
+// the stepped twiddles' start value or that pass's table.
+GL_HD int inv_scale_e(int log_n) { return 192 - log_n; }
+
+// tile *= other (at the same global positions, or the tile itself), times
+// 2^inv_scale_e(log_n) when the tile's pass is the plan's only one
+// (ntt.rs:845-853).
 template <class X, class G>
 GL_HD void tile_pointwise(u64 *tile, const G &g, const u64 *__restrict__ other, int log_n) {
-    const int e = 192 - log_n;
+    const int e = g.log_m == log_n ? inv_scale_e(log_n) : 0;
     GNT_FOR(X, idx, G::tile_n) {
         int n, c;
         tile_rc<G>(idx, n, c);
         const int t = tix(g, n, c);
         u64 v = tile[t];
         u64 w = other ? other[tile_addr(g.cg0, n, c, g.log_m, g.log_s)] : v;
-        tile[t] = gl::mul_pow2(gl::mul(v, w), e);
+        v = gl::mul(v, w);
+        tile[t] = e ? gl::mul_pow2(v, e) : v;
     }
     X::sync();
 }
@@ -388,13 +401,14 @@ GL_HD void body_fwd(u64 *tile, const G &g, u64 *x, const u64 *limbs, size_t n_li
 // Inverse pass; SUMS: it is the last one and writes limb sums.  STEP: a
 // strided pass whose inverse twiddles were not applied by the producer of its
 // input (it has no table) steps them as it loads.  TSTORE: the store
-// applies the table of the next inverse pass (st).  Compile-time choices, so
+// applies the table of the next inverse pass (st).  scale_e: the 1/N the
+// stepped twiddles of pass 0 carry (0 for other passes).  Compile-time choices, so
 // each kernel carries only the phases it runs (a kernel holding both forms
 // of a phase needs registers it then spills).
 template <class X, class G, bool SUMS, bool STEP, bool TSTORE>
 GL_HD void body_inv(u64 *tile, const G &g, u64 *x, const Tw &tw, u64 *Ls, u64 *Hs, size_t n_coeffs,
-                    size_t n_limbs, const StoreTw &st) {
-    if constexpr (G::strided && STEP) tile_load_step<X, G>(tile, g, x, tw);
+                    size_t n_limbs, int scale_e, const StoreTw &st) {
+    if constexpr (G::strided && STEP) tile_load_step<X, G>(tile, g, x, tw, scale_e);
     else tile_load<X, G>(tile, g, x, nullptr, 0);
     tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
     if constexpr (SUMS) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
@@ -454,12 +468,12 @@ GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_
     }
     X::sync();
     tile_fwd_pair<X, G, 0>(tile, g, tw);
-    const int e = 192 - log_n;
+    // the 1/N rides on the inverse twiddles of pass 0 (two or more passes)
     GNT_FOR(X, idx, G::tile_n) {
         int n, c;
         tile_rc<G>(idx, n, c);
         const int t = tix(g, n, c);
-        ta[t] = gl::mul_pow2(gl::mul(ta[t], tb[t]), e);
+        ta[t] = gl::mul(ta[t], tb[t]);
     }
     X::sync();
     tile_inv<X, G, Sub<G::LOGR>::count - 1>(ta, g, tw);

@@ -33,10 +33,12 @@ PassTables tables_for(const Plan &pl) {
         const PassDesc &d = pl.pass[p];
         if (!pass_has_table(d)) continue;
         for (int inv = 0; inv < 2; inv++) {
-            std::vector<u64> &T = g_ptab[(d.log_m << 8) | (d.log_r << 1) | inv];
+            const int scale_e = (inv && p == 0) ? inv_scale_e(d.log_m) : 0;
+            std::vector<u64> &T = g_ptab[(d.log_m << 8) | ((scale_e ? 1 : 0) << 6) | (d.log_r << 1) | inv];
             if (T.empty()) {
                 T.resize((size_t)1 << d.log_m);
-                for (size_t i = 0; i < T.size(); i++) T[i] = pass_table_entry(tw(), (u32)i, d.log_m, d.log_r, inv != 0);
+                for (size_t i = 0; i < T.size(); i++)
+                    T[i] = pass_table_entry(tw(), (u32)i, d.log_m, d.log_r, inv != 0, scale_e);
             }
             (inv ? t.inv : t.fwd)[p] = T.data();
         }
@@ -64,8 +66,10 @@ void k_inv_variant(const PassDesc &d, int log_n, u64 *x, u64 *Ls, u64 *Hs, size_
                    const StoreTw &st) {
     std::vector<u64> tile(G::tile_n);
     const size_t grid = ((size_t)1 << log_n) / G::tile_n;
+    const int scale_e = d.log_m == log_n ? inv_scale_e(log_n) : 0;  // pass 0 carries 1/N
     for (size_t t = 0; t < grid; t++)
-        body_inv<HX, G, SUMS, STEP, TSTORE>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs, st);
+        body_inv<HX, G, SUMS, STEP, TSTORE>(tile.data(), make_geom<G>(d, t), x, tw(), Ls, Hs, n_coeffs, n_limbs,
+                                            scale_e, st);
 }
 
 // The variant the library launches for this pass (launch_inverse_rest).
@@ -152,6 +156,9 @@ void emu_forward(u64 *x, int log_n) {
     forward(pl, pl.npass, x, nullptr, 0, tables_for(pl));
 }
 
+// Inverse transform as the library runs it: a plan of two or more passes
+// includes the 1/N (in the inverse twiddles of pass 0), a single-pass plan
+// leaves it to the pointwise product (so its inverse is unscaled).
 void emu_inverse(u64 *x, int log_n) {
     tables();
     Plan pl = make_plan(log_n);

@@ -120,8 +120,9 @@ def test_round_trip_multi_pass(emu, log_n):
     y = x.copy()
     emu.emu_forward(y.ctypes.data_as(PU), log_n)
     emu.emu_inverse(y.ctypes.data_as(PU), log_n)
-    # inverse is unscaled: y = n * x mod P
-    assert all(int(y[i]) == int(x[i]) * n % P for i in range(0, n, 997))
+    # a plan of two or more passes carries the 1/N in the inverse twiddles of
+    # pass 0, so the round trip is the identity
+    assert all(int(y[i]) == int(x[i]) for i in range(0, n, 997))
 
 
 @pytest.mark.parametrize("la,lb", [(1, 1), (1, 4), (8, 5), (200, 200), (1000, 3), (2048, 2048), (5000, 3000),
@@ -216,6 +217,6 @@ def test_table_and_stepped_twiddles_give_the_same_products(emu, orc, max_log):
         y = x.copy()
         E.emu_forward(y.ctypes.data_as(PU), 17)
         E.emu_inverse(y.ctypes.data_as(PU), 17)
-        assert all(int(y[i]) == int(x[i]) * n % P for i in range(0, n, 991))
+        assert all(int(y[i]) == int(x[i]) for i in range(0, n, 991))
     finally:
         E.emu_set_table_max_log(22)
