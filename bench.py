@@ -72,7 +72,7 @@ def operand_dist(dist: str, seed: int, n: int) -> np.ndarray:
 # (KiB per dispatch; FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
 # factors a plain 1 GiB device copy reads), keyed by (op, log2 N, radices).
 HBM_TRAFFIC = {
-    ("mul", 27, (8, 9, 10)): {"fused": 3221595025, "product": 15067956551},
+    ("mul", 27, (8, 9, 10)): {"fused": 3226732305, "product": 14934773694},
 }
 
 
