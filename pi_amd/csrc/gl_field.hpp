@@ -41,13 +41,24 @@ GL_HD u64 add(u64 a, u64 b) {
     return s + (u64)m;
 }
 
+// 64-bit a - b as two 32-bit subtracts with borrow (v_sub_co_u32 /
+// v_subb_co_u32); *borrow receives the borrow out of bit 63.  Written with
+// the carry builtins because a 64-bit __builtin_sub_overflow is lowered to the
+// subtract plus a 64-bit compare that recomputes the borrow.
+GL_HD u64 sub_borrow(u64 a, u64 b, u32 *borrow) {
+    unsigned b1, b2;
+    const unsigned lo = __builtin_subc((unsigned)a, (unsigned)b, 0u, &b1);
+    const unsigned hi = __builtin_subc((unsigned)(a >> 32), (unsigned)(b >> 32), b1, &b2);
+    *borrow = b2;
+    return ((u64)hi << 32) | lo;
+}
+
 // (a - b) mod p for a, b in [0, p).  On borrow add p, i.e. subtract EPS
 // (cannot underflow again: a - b + 2^64 >= 2^64 - p + 1 > EPS).
 GL_HD u64 sub(u64 a, u64 b) {
-    u64 d;
-    bool bw = __builtin_sub_overflow(a, b, &d);
-    u32 m = bw ? (u32)EPS : 0u;
-    return d - (u64)m;
+    u32 bw, bw2;
+    const u64 d = sub_borrow(a, b, &bw);
+    return sub_borrow(d, bw ? EPS : 0u, &bw2);
 }
 
 GL_HD u64 neg(u64 a) { return sub(0, a); }
@@ -57,9 +68,9 @@ GL_HD u64 neg(u64 a) { return sub(0, a); }
 GL_HD u64 reduce128(u64 lo, u64 hi) {
     u64 hh = hi >> 32;
     u64 hl = hi & 0xFFFFFFFFull;
-    u64 t0;
-    bool bw = __builtin_sub_overflow(lo, hh, &t0);  // on borrow: lo - hh + p, no second borrow
-    t0 -= (u64)(bw ? (u32)EPS : 0u);
+    u32 bw, bw2;
+    u64 t0 = sub_borrow(lo, hh, &bw);             // on borrow: lo - hh + p, no second borrow
+    t0 = sub_borrow(t0, bw ? EPS : 0u, &bw2);
     u64 t1 = (hl << 32) - hl;                   // hl * EPS < p
     u64 s;
     bool c = __builtin_add_overflow(t0, t1, &s);    // wrapped: true value s + 2^64 = s + EPS (mod p)
