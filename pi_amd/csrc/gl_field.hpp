@@ -171,10 +171,19 @@ GL_HD void dft_dit_inv(u64 *v) {
         for (int b = 0; b < R; b += m) {
 #pragma unroll
             for (int j = 0; j < h; j++) {
-                u64 u = v[b + j];
-                u64 w = mul_pow2(v[b + j + h], (192 - (192 / m) * j) % 192);
-                v[b + j] = add(u, w);
-                v[b + j + h] = sub(u, w);
+                const u64 u = v[b + j];
+                const int e = (192 - (192 / m) * j) % 192;
+                // 2^e = -2^(e-96) for e >= 96: multiply by 2^(e-96) and swap
+                // the butterfly's add and subtract instead of negating
+                if (e >= 96) {
+                    const u64 w = mul_pow2(v[b + j + h], e - 96);
+                    v[b + j] = sub(u, w);
+                    v[b + j + h] = add(u, w);
+                } else {
+                    const u64 w = mul_pow2(v[b + j + h], e);
+                    v[b + j] = add(u, w);
+                    v[b + j + h] = sub(u, w);
+                }
             }
         }
     }
