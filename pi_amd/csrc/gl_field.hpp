@@ -99,6 +99,23 @@ GL_HD u64 shl_reduce(u64 x, int f) {
     return reduce128(x << f, x >> (64 - f));
 }
 
+// x * 2^72 mod p with one reduction instead of two: with x = xh*2^32 + xl,
+// 2^96 = -1 and 2^64 = 2^32 - 1 give x*2^72 = xl*2^40 - (xl + xh)*2^8, and
+// with y = xl*2^8 = yh*2^32 + yl that is A*2^32 - B for A = yl + yh (33 bits)
+// and B = yh + (xl + xh)*2^8 (42 bits); A's bit 32 is 2^64 = EPS.
+GL_HD u64 mul_2_72(u64 x) {
+    const u32 xl = (u32)x, xh = (u32)(x >> 32);
+    const u32 yl = xl << 8, yh = xl >> 24;
+    unsigned ca;
+    const unsigned al = __builtin_addc(yl, yh, 0u, &ca);
+    const u64 B = ((((u64)xl + xh) << 8)) + yh;
+    const u64 V0 = ((u64)al << 32) | (ca ? (u32)EPS : 0u);
+    u32 bw, bw2;
+    u64 v = sub_borrow(V0, B, &bw);         // on borrow add p: subtract EPS (V0 - B > -p)
+    v = sub_borrow(v, bw ? EPS : 0u, &bw2);
+    return v >= P ? v + EPS : v;            // wraps to v - p
+}
+
 // 2^e mod p for 0 <= e < 192, as a plain constant.
 GL_HD u64 pow2_const(int e) {
     // 2^e for e < 64 is exact; beyond, reduce by doubling.
@@ -114,6 +131,7 @@ GL_HD u64 mul_pow2(u64 x, int e) {
     int f = negate ? e - 96 : e;
     u64 r;
     if (f == 0) r = x;
+    else if (f == 72) r = mul_2_72(x);
     else if (f < 64) r = shl_reduce(x, f);
     else r = shl_reduce(shl_reduce(x, f - 48), 48);
     return negate ? neg(r) : r;

@@ -70,6 +70,19 @@ def test_shift_multiplies(emu):
             assert emu.emu_mul_pow2(x, e) == x * pow(2, e, P) % P
 
 
+def test_shift_by_72_edge_cases(emu):
+    """x * 2^72 has its own one-reduction path (x*2^72 = xl*2^40 - (xl + xh)*2^8):
+    words at their extremes drive every borrow and wrap in it."""
+    rnd = random.Random(72)
+    words = [0, 1, 2, 0xFF, 0xFFFFFF, 0x1000000, 0x7FFFFFFF, 0x80000000, 0xFFFFFF00, 0xFFFFFFFE, 0xFFFFFFFF]
+    xs = [(h << 32) | l for h in words for l in words]
+    xs += [rnd.randrange(P) for _ in range(20000)]
+    xs += [rnd.randrange(2**64) for _ in range(2000)]          # non-canonical inputs too
+    for x in xs:
+        assert emu.emu_mul_pow2(x, 72) == x * pow(2, 72, P) % P, hex(x)
+        assert emu.emu_mul_pow2(x, 168) == x * pow(2, 168, P) % P, hex(x)
+
+
 def test_library_root(emu):
     for log_n in range(1, 31):
         w = emu.emu_root(log_n)
