@@ -8,7 +8,9 @@ one full product: pack + two forward transforms + pointwise + inverse + carry.
 Prints ONE JSON line.  `value` is Mlimbs/s = (na + nb) / t / 1e6.
 `roofline.achieved` is the algorithmic byte count of a product,
 B_mul = 16 (na + nb) + 80 N, divided by the measured product time (HIP events
-on the library's stream), against the 8 TB/s HBM peak.  `cpu_baseline` times
+on the library's stream), against the 8 TB/s HBM peak; `roofline.valu` puts
+the same launch against VALU instruction issue, the bound the transform
+kernels actually meet (instruction counts from rocprofv3).  `cpu_baseline` times
 the CPU restatement of the reference multiply on a bounded sample of the same
 workload on the host cores.
 """
@@ -71,9 +73,30 @@ def operand_dist(dist: str, seed: int, n: int) -> np.ndarray:
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (KiB per dispatch; FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
 # factors a plain 1 GiB device copy reads), keyed by (op, log2 N, radices).
+# VALU wave instructions per launch of the dominant (fused) kernel, measured
+# with rocprofv3 --pmc SQ_INSTS_VALU, keyed like HBM_TRAFFIC.  The secondary
+# bound: 1024 SIMDs issue one wave64 instruction each per 4 cycles at 2.4 GHz
+# for everything but plain 32-bit add/sub/xor/mov (2 cycles).
+VALU_INSTS = {
+    ("mul", 27, (8, 9, 10)): 1057226752,
+    ("sqr", 27, (8, 9, 10)): 753926144,
+}
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4  # wave64 instructions per second at 4 cycles each
+
 HBM_TRAFFIC = {
     ("mul", 27, (8, 9, 10)): {"fused": 3226732305, "product": 14934773694},
 }
+
+
+def valu_block(insts, ms):
+    """The launch against VALU issue: measured wave instructions / time, over the
+    chip's issue rate at 4 cycles per instruction (the kernels' real bound)."""
+    if not insts:
+        return None
+    rate = insts / (ms * 1e-3)
+    return {"wave_instructions_per_launch": insts, "achieved": round(rate / 1e9, 1),
+            "peak": round(VALU_ISSUE_PEAK / 1e9, 1), "unit": "G wave64 instructions/s (4-cycle issue)",
+            "frac": round(rate / VALU_ISSUE_PEAK, 4)}
 
 
 def b_mul(na: int, nb: int, log_n: int) -> float:
@@ -269,6 +292,7 @@ def main():
             "traffic": fused_traffic,
             "algorithmic_bytes_per_launch": fused_bytes,
             "ms_per_launch": round(fused_ms, 4),
+            "valu": valu_block(VALU_INSTS.get((args.op, log_n, tuple(radices[:3]))), fused_ms),
             "whole_product": {
                 "achieved": round(achieved, 2), "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "bytes_model": "B_mul = 16(na+nb)+80N" if args.op == "mul" else "B_sqr = 24n+48N",

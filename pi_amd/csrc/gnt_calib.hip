@@ -5,6 +5,10 @@
 //                             FETCH_SIZE / WRITE_SIZE counters
 //   gnt_calib valu            Goldilocks mul / add+sub / shift-multiply
 //                             throughput (G ops/s over the whole device)
+//   gnt_calib isa             issue cost of single VALU instructions the
+//                             field arithmetic is made of (SIMD cycles per
+//                             wave64 instruction; 8 independent chains per
+//                             lane, many waves per SIMD)
 // Prints one JSON line per measurement.
 #include <hip/hip_runtime.h>
 
@@ -83,6 +87,35 @@ __global__ __launch_bounds__(256) void k_shift(u64 *out, int iters, u64 seed) {
     out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
 }
 
+// One instruction per chain step, written as inline assembly so that
+// exactly that instruction is issued (ALU only: no memory access in asm).
+constexpr int ISA_ITERS = 4096;
+#define GNT_ISA_KERNEL(NAME, T, ASM, ...)                                         \
+    __global__ __launch_bounds__(256) void NAME(u64 *out, unsigned seed) {       \
+        T a[CHAINS];                                                              \
+        const T b = (T)(seed ^ threadIdx.x);                                      \
+        for (int c = 0; c < CHAINS; c++) a[c] = (T)(c + threadIdx.x);             \
+        for (int it = 0; it < ISA_ITERS; it++) {                                  \
+            _Pragma("unroll") for (int c = 0; c < CHAINS; c++) { ASM; }           \
+        }                                                                         \
+        u64 r = 0;                                                                \
+        for (int c = 0; c < CHAINS; c++) r += (u64)a[c];                          \
+        out[(size_t)blockIdx.x * 256 + threadIdx.x] = r;                          \
+    }
+GNT_ISA_KERNEL(k_isa_add_u32, unsigned, asm volatile("v_add_u32 %0, %0, %1" : "+v"(a[c]) : "v"(b)))
+GNT_ISA_KERNEL(k_isa_xor_b32, unsigned, asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[c]) : "v"(b)))
+GNT_ISA_KERNEL(k_isa_lshlrev_b32, unsigned, asm volatile("v_lshlrev_b32 %0, %1, %0" : "+v"(a[c]) : "v"(b & 31)))
+GNT_ISA_KERNEL(k_isa_add_co_u32, unsigned,
+               u64 sc; asm volatile("v_add_co_u32_e64 %0, %1, %0, %2" : "+v"(a[c]), "=s"(sc) : "v"(b)))
+GNT_ISA_KERNEL(k_isa_cndmask_b32, unsigned,
+               asm volatile("v_cndmask_b32_e64 %0, %0, %1, %2" : "+v"(a[c]) : "v"(b), "s"(~0ull)))
+GNT_ISA_KERNEL(k_isa_mad_u64_u32, u64,
+               u64 sc; asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a[c]), "=s"(sc) : "v"((unsigned)b), "v"((unsigned)b)))
+GNT_ISA_KERNEL(k_isa_lshl_add_u64, u64, asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(a[c]) : "v"(b)))
+GNT_ISA_KERNEL(k_isa_cmp_lt_u64, u64,
+               u64 m; asm volatile("v_cmp_lt_u64_e64 %0, %1, %2" : "=s"(m) : "v"(a[c]), "v"(b)))
+GNT_ISA_KERNEL(k_isa_lshlrev_b64, u64, asm volatile("v_lshlrev_b64 %0, %1, %0" : "+v"(a[c]) : "v"((unsigned)b & 31)))
+
 static float time_ms(void (*launch)(void *), void *arg, int reps) {
     hipEvent_t a, b;
     CK(hipEventCreate(&a));
@@ -113,6 +146,13 @@ static void launch_valu(void *p) {
     else hipLaunchKernelGGL(k_shift, dim3(VALU_BLOCKS), dim3(256), 0, 0, v->out, v->iters, 12345ull);
 }
 
+struct IsaArg { u64 *out; void (*k)(u64 *, unsigned); };
+static const int ISA_BLOCKS = 8192;
+static void launch_isa(void *p) {
+    IsaArg *a = (IsaArg *)p;
+    hipLaunchKernelGGL(a->k, dim3(ISA_BLOCKS), dim3(256), 0, 0, a->out, 12345u);
+}
+
 int main(int argc, char **argv) {
     const char *mode = argc > 1 ? argv[1] : "all";
     if (!strcmp(mode, "copy") || !strcmp(mode, "all")) {
@@ -141,6 +181,29 @@ int main(int argc, char **argv) {
             printf("{\"calib\": \"%s\", \"Gops_per_s\": %.1f, \"ms\": %.3f}\n", names[k], ops / (ms * 1e-3) / 1e9, ms);
         }
         CK(hipFree(v.out));
+    }
+    if (!strcmp(mode, "isa") || !strcmp(mode, "all")) {
+        IsaArg a;
+        CK(hipMalloc(&a.out, (size_t)ISA_BLOCKS * 256 * sizeof(u64)));
+        int clk_khz = 0, cus = 0;
+        CK(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, 0));
+        CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+        struct { const char *name; void (*k)(u64 *, unsigned); } ks[] = {
+            {"v_add_u32", k_isa_add_u32},         {"v_xor_b32", k_isa_xor_b32},
+            {"v_lshlrev_b32", k_isa_lshlrev_b32}, {"v_add_co_u32", k_isa_add_co_u32},
+            {"v_cndmask_b32", k_isa_cndmask_b32}, {"v_mad_u64_u32", k_isa_mad_u64_u32},
+            {"v_lshl_add_u64", k_isa_lshl_add_u64}, {"v_cmp_lt_u64", k_isa_cmp_lt_u64},
+            {"v_lshlrev_b64", k_isa_lshlrev_b64}};
+        for (auto &k : ks) {
+            a.k = k.k;
+            const float ms = time_ms(launch_isa, &a, 5);
+            const double wave_insts = (double)ISA_BLOCKS * 256 / 64 * ISA_ITERS * CHAINS;
+            const double simd_cycles = ms * 1e-3 * clk_khz * 1e3 * cus * 4;
+            printf("{\"calib\": \"isa\", \"instruction\": \"%s\", \"simd_cycles_per_wave_instruction\": %.2f, "
+                   "\"ms\": %.3f, \"clock_khz\": %d, \"cus\": %d}\n",
+                   k.name, simd_cycles / wave_insts, ms, clk_khz, cus);
+        }
+        CK(hipFree(a.out));
     }
     return 0;
 }
