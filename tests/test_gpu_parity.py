@@ -202,6 +202,43 @@ def test_2p24_distributions(ctx, orc, kind):
     assert _fingerprints_ok(orc, s, a, a)
 
 
+def test_transform_lengths_up_to_the_cap(ctx, orc):
+    """One product and one square per transform length 2^25 .. 2^30, each with
+    4(na+nb) - 1 just under the power of two; 2^30 is the cap
+    (4(na+nb) <= 2^30).  Length, low limb and residue fingerprints (GMP at these
+    sizes would take minutes); at the cap also the all-ones closed form, and one
+    limb more is refused with GNT_E_TOO_LARGE, as ntt_fits refuses it
+    (integer.rs:476-478)."""
+    from pi_amd import gnt
+    for log_n in range(25, 31):
+        tot = 1 << (log_n - 2)
+        na = tot // 2 + 12345
+        nb = tot - na
+        log(f"log_n={log_n} na={na} nb={nb}")
+        a, b = orc.splitmix64(300 + log_n, na), orc.splitmix64(400 + log_n, nb)
+        c = ctx.mul_mag_ntt(a, b)
+        assert c.size in (na + nb, na + nb - 1)
+        assert int(c[0]) == (int(a[0]) * int(b[0])) % (1 << 64)
+        assert _fingerprints_ok(orc, c, a, b), log_n
+        del c
+        s = ctx.sqr(a[: tot // 2])
+        assert _fingerprints_ok(orc, s, a[: tot // 2], a[: tot // 2]), log_n
+        del s, a, b
+    # all-ones at the cap: (B^m - 1)(B^n - 1) = B^(m+n) - B^m - B^n + 1
+    m, n = (1 << 27) + 5, (1 << 27) - 5
+    log(f"all-ones {m} x {n}")
+    ones = np.full(m, np.uint64(2**64 - 1))
+    c = ctx.mul_mag_ntt(ones, ones[:n])
+    assert c.size == m + n
+    assert c[0] == 1 and not c[1:n].any()
+    assert c[n] == np.uint64(2**64 - 1) and (c[n:m] == np.uint64(2**64 - 1)).all()
+    assert c[m] == np.uint64(2**64 - 2) and (c[m + 1:] == np.uint64(2**64 - 1)).all()
+    del c
+    with pytest.raises(gnt.GntError) as e:
+        ctx.mul_mag_ntt(ones, ones[: n + 1])
+    assert e.value.code == gnt.GNT_E_TOO_LARGE
+
+
 # ---- device-resident operands, batches, dispatch ----
 
 def test_device_chain_reuse(ctx, orc):
