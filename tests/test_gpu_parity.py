@@ -409,7 +409,7 @@ def _limbs(x):
 
 def test_serving_add_sub(ctx, orc):
     """a + b and a - b (a >= b) through the serving loop, exact against CPython
-    int, with carries and borrows that cross carry tiles (1024 limbs) and
+    int, with carries and borrows that cross carry tiles (2048 limbs) and
     differences whose top limbs cancel."""
     from pi_amd import gnt
     rng = np.random.default_rng(5)
