@@ -187,6 +187,10 @@ def main():
     ap.add_argument("--tune", default="", help="LO:HI: autotune transform lengths 2^LO..2^HI, write --tune-out, exit")
     ap.add_argument("--tune-out", default=DEFAULT_PLANS)
     ap.add_argument("--digits", type=int, default=100_000_000, help="chudnovsky: decimal digits")
+    ap.add_argument("--latency-rate", type=float, default=400.0,
+                    help="single: arrival rate (requests/s) of the serving run whose p50/p99 the line reports")
+    ap.add_argument("--latency-requests", type=int, default=300)
+    ap.add_argument("--no-latency", action="store_true", help="single: skip the serving latency run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -263,7 +267,7 @@ def main():
     ok = out.size in (2 * n, 2 * n - 1) and int(out[0]) == (int(a[0]) * int(bb[0])) % (1 << 64)
 
     line = {
-        "metric": "Goldilocks-NTT multiply Mlimbs/s at 2^24 limbs",
+        "metric": "Goldilocks-NTT multiply Mlimbs/s at 2^24 limbs; batched p50/p99 latency",
         "value": round(value, 3),
         "unit": "Mlimbs/s",
         "n_gpus": 1,
@@ -304,6 +308,18 @@ def main():
         },
     }
     ctx.close()
+    if not args.no_latency:
+        # the metric's second half: p50/p99 latency of mixed-size requests
+        # through the continuously batching serving loop
+        sctx, srv, src = serve_open(args)
+        pt = serve_rate(srv, src, np.random.default_rng(8), args.latency_rate, args.latency_requests)
+        srv.stop()
+        sctx.close()
+        line["latency"] = {"p50_ms": pt["p50_ms"], "p99_ms": pt["p99_ms"], "unit": "ms",
+                           "workload": SERVE_WORKLOAD + f"; {args.latency_rate:g} req/s offered",
+                           "measured_from": "scheduled arrival, operands in host memory (upload included), "
+                                            "to result in HBM",
+                           "run": pt}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(min(args.cpu_log_size, args.log_size), args.op, host_threads())
     print(json.dumps(line), flush=True)
@@ -400,124 +416,153 @@ def cpu_baseline_batched(pairs, threads: int, sample: int = 128) -> dict:
             "cpu_model": cpu_model()}
 
 
-def bench_serve(args):
-    """Config: serving.  Open-loop Poisson arrivals; n = round(2^U[8,24]) limbs;
-    70% multiply / 30% square; 20% of requests take a finished result that is
-    still on the device as operand; HBM admission cap 200 GB.  Latency is
-    submit-to-completion as recorded by the library.  Every 25th request with
-    fresh operands is checked exactly (residue mod 2^61-1) after the run."""
-    import threading
+SERVE_POOL_LIMBS = (1 << 25) + (1 << 24)
+Q61 = (1 << 61) - 1
 
+
+def _residue(x) -> int:
+    return int.from_bytes(np.ascontiguousarray(x, dtype="<u8").tobytes(), "little") % Q61
+
+
+def serve_open(args):
+    """A serving loop with the HBM admission cap, its operand pool (slices of
+    one random buffer, seed 8) and every size class allocated once."""
     from pi_amd import gnt
 
-    rng = np.random.default_rng(8)
-    pool_limbs = (1 << 25) + (1 << 24)
-    src = operand(8, pool_limbs)  # operands are slices of one random buffer
-    q61 = (1 << 61) - 1
-
-    def residue(x):
-        return int.from_bytes(np.ascontiguousarray(x, dtype="<u8").tobytes(), "little") % q61
-
+    src = operand(8, SERVE_POOL_LIMBS)
     ctx = make_ctx(args, hbm_cap_bytes=200 * 10**9)
     srv = gnt.Server(ctx)
-    # warm-up: first-use allocations of every size class
-    for e in range(8, 25, 2):
+    for e in range(8, 25, 2):  # warm-up: first-use allocations of every size class
         n = 1 << e
         t = srv.submit(src[:n], src[n:2 * n])
         buf, _ = srv.wait(t)
         buf.release()
+    return ctx, srv, src
 
+
+def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
+    """One open-loop Poisson run at `rate` requests/s: n = round(2^U[8,24])
+    limbs, 70% multiply / 30% square, 20% of requests take a finished result
+    that is still on the device as operand.  p50/p99 run from each request's
+    scheduled arrival to its completion (the library's submit-to-completion
+    time plus any delay of the submit call itself, so a stalled submitter does
+    not hide queueing); the library's own submit-to-completion figures are
+    reported beside them.  Every 25th request with fresh operands is checked
+    exactly (residue mod 2^61-1) after the run."""
+    import threading
+
+    # the request schedule, drawn before the clock starts
+    sched = []
+    gap = rng.exponential(1.0 / rate, size=requests)
+    for i in range(requests):
+        n = int(round(2 ** rng.uniform(8, 24)))
+        sched.append(dict(at=float(gap[:i + 1].sum()), n=n, sq=bool(rng.uniform() < 0.3),
+                          reuse=bool(rng.uniform() < 0.2), keep=bool(rng.uniform() < 0.5),
+                          oa=int(rng.integers(0, SERVE_POOL_LIMBS - n)),
+                          ob=int(rng.integers(0, SERVE_POOL_LIMBS - n))))
+    lock = threading.Lock()
+    ready = []                      # finished results kept on the device for reuse
+    consumed, sampled = [], []
+    lat, lat_open, limbs_done = [], [], [0]
+    tickets, meta, late = [], {}, {}
+    done_evt = threading.Event()
+
+    def waiter():
+        for k in range(requests):
+            while k >= len(tickets):
+                time.sleep(0.0001)
+            t = tickets[k]
+            buf, us = srv.wait(t)
+            lat.append(us)
+            lat_open.append(us + late[t])
+            n_in, check, keep = meta[t]
+            limbs_done[0] += n_in
+            if check is not None:
+                sampled.append((buf, check))
+            elif keep and n_in <= (1 << 24):
+                with lock:
+                    ready.append((buf, n_in))
+                    while len(ready) > 8:
+                        ready.pop(0)[0].release()
+            else:
+                buf.release()
+        done_evt.set()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    for i, r in enumerate(sched):
+        wait_s = t0 + r["at"] - time.perf_counter()
+        if wait_s > 0:
+            time.sleep(wait_s)
+        n = r["n"]
+        a = src[r["oa"]:r["oa"] + n]
+        b = src[r["ob"]:r["ob"] + n]
+        a_buf = None
+        if r["reuse"]:
+            with lock:
+                if ready:
+                    a_buf, n_a = ready.pop(int(rng.integers(0, len(ready))))
+        if a_buf is None:
+            n_a = n
+        t_call = time.perf_counter()
+        check = None
+        if a_buf is None and i % 25 == 0:
+            check = ("sq", r["oa"], n) if r["sq"] else ("mul", r["oa"], r["ob"], n)
+        if r["sq"]:
+            t = srv.submit(None if a_buf is not None else a, square=True, a_buf=a_buf)
+            n_in = 2 * n_a
+        else:
+            t = srv.submit(None if a_buf is not None else a, b, a_buf=a_buf)
+            n_in = n_a + n
+        meta[t] = (n_in, check, r["keep"])
+        # time between the scheduled arrival and the submit call, so that a
+        # submit delayed by an earlier one still counts from its arrival
+        late[t] = max(0.0, t_call - (t0 + r["at"])) * 1e6
+        if a_buf is not None:
+            consumed.append(a_buf)
+        tickets.append(t)
+    done_evt.wait()
+    th.join()
+    wall = time.perf_counter() - t0
+    # exactness of the sampled requests, off the clock
+    ok = 0
+    for buf, chk in sampled:
+        c = _residue(buf.download())
+        if chk[0] == "sq":
+            want = _residue(src[chk[1]:chk[1] + chk[2]]) ** 2 % Q61
+        else:
+            want = _residue(src[chk[1]:chk[1] + chk[3]]) * _residue(src[chk[2]:chk[2] + chk[3]]) % Q61
+        ok += int(c == want)
+        buf.release()
+    for buf, _ in ready:
+        buf.release()
+    for buf in consumed:
+        buf.release()
+    lat_ms = np.array(lat_open) / 1e3
+    sub_ms = np.array(lat) / 1e3
+    return {"rate_req_per_s": rate, "requests": requests,
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "submit_to_done_p50_ms": round(float(np.percentile(sub_ms, 50)), 3),
+            "submit_to_done_p99_ms": round(float(np.percentile(sub_ms, 99)), 3),
+            "throughput_Mlimbs_per_s": round(limbs_done[0] / wall / 1e6, 1),
+            "achieved_req_per_s": round(requests / wall, 1),
+            "exact_checks_passed": f"{ok}/{len(sampled)}"}
+
+
+SERVE_WORKLOAD = ("open-loop Poisson; n = round(2^U[8,24]); 70/30 mul/sqr; 20% reuse of device results; "
+                  "HBM cap 200 GB")
+
+
+def bench_serve(args):
+    """Config: serving.  The arrival-rate sweep of serve_rate through one
+    serving loop; `value` is the best throughput over the sweep."""
+    rng = np.random.default_rng(8)
+    ctx, srv, src = serve_open(args)
     points = []
     for rate in [float(r) for r in args.rates.split(",")]:
-        # the request schedule, drawn before the clock starts
-        sched = []
-        gap = rng.exponential(1.0 / rate, size=args.requests)
-        for i in range(args.requests):
-            n = int(round(2 ** rng.uniform(8, 24)))
-            sched.append(dict(at=float(gap[:i + 1].sum()), n=n, sq=bool(rng.uniform() < 0.3),
-                              reuse=bool(rng.uniform() < 0.2), keep=bool(rng.uniform() < 0.5),
-                              oa=int(rng.integers(0, pool_limbs - n)), ob=int(rng.integers(0, pool_limbs - n))))
-        lock = threading.Lock()
-        ready = []                      # finished results kept on the device for reuse
-        consumed, sampled = [], []
-        lat, limbs_done = [], [0]
-        tickets, meta = [], {}
-        done_evt = threading.Event()
-
-        def waiter():
-            for k in range(args.requests):
-                while k >= len(tickets):
-                    time.sleep(0.0001)
-                t = tickets[k]
-                buf, us = srv.wait(t)
-                lat.append(us)
-                n_in, check, keep = meta[t]
-                limbs_done[0] += n_in
-                if check is not None:
-                    sampled.append((buf, check))
-                elif keep and n_in <= (1 << 24):
-                    with lock:
-                        ready.append((buf, n_in))
-                        while len(ready) > 8:
-                            ready.pop(0)[0].release()
-                else:
-                    buf.release()
-            done_evt.set()
-
-        th = threading.Thread(target=waiter)
-        th.start()
-        t0 = time.perf_counter()
-        for i, r in enumerate(sched):
-            wait_s = t0 + r["at"] - time.perf_counter()
-            if wait_s > 0:
-                time.sleep(wait_s)
-            n = r["n"]
-            a = src[r["oa"]:r["oa"] + n]
-            b = src[r["ob"]:r["ob"] + n]
-            a_buf = None
-            if r["reuse"]:
-                with lock:
-                    if ready:
-                        a_buf, n_a = ready.pop(int(rng.integers(0, len(ready))))
-            if a_buf is None:
-                n_a = n
-            check = None
-            if a_buf is None and i % 25 == 0:
-                check = ("sq", r["oa"], n) if r["sq"] else ("mul", r["oa"], r["ob"], n)
-            if r["sq"]:
-                t = srv.submit(None if a_buf is not None else a, square=True, a_buf=a_buf)
-                n_in = 2 * n_a
-            else:
-                t = srv.submit(None if a_buf is not None else a, b, a_buf=a_buf)
-                n_in = n_a + n
-            meta[t] = (n_in, check, r["keep"])
-            if a_buf is not None:
-                consumed.append(a_buf)
-            tickets.append(t)
-        done_evt.wait()
-        th.join()
-        wall = time.perf_counter() - t0
-        # exactness of the sampled requests, off the clock
-        ok = 0
-        for buf, chk in sampled:
-            c = residue(buf.download())
-            if chk[0] == "sq":
-                want = residue(src[chk[1]:chk[1] + chk[2]]) ** 2 % q61
-            else:
-                want = residue(src[chk[1]:chk[1] + chk[3]]) * residue(src[chk[2]:chk[2] + chk[3]]) % q61
-            ok += int(c == want)
-            buf.release()
-        for buf, _ in ready:
-            buf.release()
-        for buf in consumed:
-            buf.release()
-        lat_ms = np.array(lat) / 1e3
-        points.append({"rate_req_per_s": rate, "requests": args.requests,
-                       "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-                       "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-                       "throughput_Mlimbs_per_s": round(limbs_done[0] / wall / 1e6, 1),
-                       "achieved_req_per_s": round(args.requests / wall, 1),
-                       "exact_checks_passed": f"{ok}/{len(sampled)}"})
+        points.append(serve_rate(srv, src, rng, rate, args.requests))
         print(json.dumps({"serve_point": points[-1]}), file=sys.stderr, flush=True)
     stats = srv.stats()
     srv.stop()
@@ -530,11 +575,9 @@ def bench_serve(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (SplitMix64 limbs, Poisson arrivals, seed 8)",
         "config": {"plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in",
-                   "workload": "open-loop Poisson; n = round(2^U[8,24]); 70/30 mul/sqr; 20% reuse of device results; "
-                               "HBM cap 200 GB", "sweep": points, "server": stats},
+                   "workload": SERVE_WORKLOAD, "sweep": points, "server": stats},
     }
     print(json.dumps(line), flush=True)
-
 
 
 def bench_chudnovsky(args):

@@ -170,6 +170,13 @@ int gnt_load_plans(const char *path);
  * tables; the default is 22.  *previous (if not null) receives the old
  * setting.  Process-wide, like the plans. */
 int gnt_set_pass_tables(int max_log_m, int *previous);
+/* Copies between caller (pageable) host memory and the device of at least
+ * min_bytes go through pinned staging: chunks of chunk_bytes, copied on the
+ * host by `threads` threads (the caller's included) while the DMA engine
+ * moves the previous chunk.  threads = 0 sends every copy straight to the
+ * runtime.  Defaults: 4 threads, 8 MiB chunks, 4 MiB minimum.  Process-wide;
+ * no reference counterpart (the reference's multiply never leaves the host). */
+int gnt_set_staging(int threads, size_t chunk_bytes, size_t min_bytes);
 /* Time every valid split of each length in [min_log_n, max_log_n] (4..28) on
  * the device (median of reps products), install the fastest, and write the
  * choices and timings to cache_path (JSON) when it is not NULL. */

@@ -363,6 +363,8 @@ int gnt_shutdown(gnt_ctx *ctx) {
     gnt_serve_stop(ctx);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    c->up.release();
+    c->down.release();
     c->pool.release_all();
     c->upload_pool.release_all();
     c->free_pass_tables();
@@ -429,15 +431,12 @@ static int host_product(Ctx *c, const u64 *a, size_t na, const u64 *b, size_t nb
     int rc = GNT_OK;
     if (!da || (!sqr && !db) || !dout || !dlen) rc = fail(GNT_E_NOMEM, "device memory exhausted");
     u32 len = 0;
-    if (!rc && hipMemcpyAsync(da, a, na * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-        rc = fail(GNT_E_DEVICE, "upload failed");
-    if (!rc && !sqr && hipMemcpyAsync(db, b, nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-        rc = fail(GNT_E_DEVICE, "upload failed");
+    if (!rc && c->up.copy(da, a, na * 8, c->stream) != hipSuccess) rc = fail(GNT_E_DEVICE, "upload failed");
+    if (!rc && !sqr && c->up.copy(db, b, nb * 8, c->stream) != hipSuccess) rc = fail(GNT_E_DEVICE, "upload failed");
     if (!rc) rc = c->enqueue_product(da, na, db, nb, dout, dlen);
     if (!rc && hipMemcpyAsync(&len, dlen, sizeof(u32), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         rc = fail(GNT_E_DEVICE, "download failed");
-    if (!rc && hipMemcpyAsync(out, dout, nl * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-        rc = fail(GNT_E_DEVICE, "download failed");
+    if (!rc && c->down.copy(out, dout, nl * 8, c->stream) != hipSuccess) rc = fail(GNT_E_DEVICE, "download failed");
     hipError_t se = hipStreamSynchronize(c->stream);
     if (!rc && se != hipSuccess) rc = fail(GNT_E_DEVICE, "stream: %s", hipGetErrorString(se));
     c->pool.release(da, na * 8);
@@ -488,7 +487,7 @@ int gnt_buf_upload(gnt_ctx *ctx, const uint64_t *limbs, size_t n, gnt_buf **out)
     while (len > 0 && limbs[len - 1] == 0) len--;
     gnt_buf *b = new_buf(c, len);
     if (!b) return fail(GNT_E_NOMEM, "device memory exhausted");
-    if (len) HIP_TRY(hipMemcpyAsync(b->d, limbs, len * 8, hipMemcpyHostToDevice, c->stream));
+    if (len) HIP_TRY(c->up.copy(b->d, limbs, len * 8, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     b->len = len;
     b->len_known = true;
@@ -532,7 +531,7 @@ int gnt_buf_download(gnt_ctx *ctx, gnt_buf *b, uint64_t *out, size_t cap, size_t
     if (cap < len || (len && !out)) return fail(GNT_E_ARG, "capacity %zu < %zu", cap, len);
     if (len) {
         hipStream_t st = b->complete ? c->copy_stream : c->stream;
-        HIP_TRY(hipMemcpyAsync(out, b->d, len * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c->down.copy(out, b->d, len * 8, st));
         HIP_TRY(hipStreamSynchronize(st));
     }
     *out_len = len;

@@ -164,9 +164,9 @@ static int upload_inputs(gnt_batch *b, const gnt_job *jobs) {
     for (size_t i = 0; i < b->prods.size(); i++) {
         const Prod &p = b->prods[i];
         if (!jobs[i].a || (!p.sqr() && !jobs[i].b)) return set_error(GNT_E_ARG, "null operand");
-        if (hipMemcpyAsync(b->in_a[i], jobs[i].a, p.na * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        if (c->up.copy(b->in_a[i], jobs[i].a, p.na * 8, c->stream) != hipSuccess)
             return set_error(GNT_E_DEVICE, "upload failed");
-        if (!p.sqr() && hipMemcpyAsync(b->in_b[i], jobs[i].b, p.nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        if (!p.sqr() && c->up.copy(b->in_b[i], jobs[i].b, p.nb * 8, c->stream) != hipSuccess)
             return set_error(GNT_E_DEVICE, "upload failed");
     }
     return GNT_OK;
@@ -183,8 +183,7 @@ static int download_outputs(gnt_batch *b, gnt_job *jobs) {
     if (hipStreamSynchronize(c->stream) != hipSuccess) return set_error(GNT_E_DEVICE, "stream failure");
     for (size_t i = 0; i < b->prods.size(); i++) {
         if (jobs[i].out_cap < lens[i]) { jobs[i].status = GNT_E_ARG; continue; }
-        if (lens[i] && hipMemcpyAsync(jobs[i].out, b->prods[i].dout, (size_t)lens[i] * 8, hipMemcpyDeviceToHost,
-                                      c->stream) != hipSuccess)
+        if (lens[i] && c->down.copy(jobs[i].out, b->prods[i].dout, (size_t)lens[i] * 8, c->stream) != hipSuccess)
             return set_error(GNT_E_DEVICE, "download failed");
         jobs[i].out_len = lens[i];
     }

@@ -11,6 +11,7 @@
 #include "../../include/gnt.h"
 #include "gnt_pass.hpp"
 #include "gnt_plan.hpp"
+#include "gnt_stage.hpp"
 
 namespace gnt {
 
@@ -51,6 +52,7 @@ struct Ctx {
     Tw tw() const { return Tw{wlo, whi, wsm}; }
     Pool pool;         // buffers used in compute-stream order
     Pool upload_pool;  // serving operands written on the copy stream; freed only after completion
+    Stager up{true}, down{false};  // host <-> device copies of caller memory
     size_t hbm_cap = 0;
     bool timing = false;
     double phase_ms[5] = {0, 0, 0, 0, 0};

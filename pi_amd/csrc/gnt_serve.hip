@@ -359,12 +359,12 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
         int rc = GNT_OK;
         if (!req->a_buf) {
             q->own_a = (u64 *)c->upload_pool.acquire(p.na * 8);
-            if (!q->own_a || hipMemcpyAsync(q->own_a, req->a, p.na * 8, hipMemcpyHostToDevice, c->copy_stream))
+            if (!q->own_a || c->up.copy(q->own_a, req->a, p.na * 8, c->copy_stream))
                 rc = GNT_E_DEVICE;
         }
         if (!rc && !sq && !req->b_buf) {
             q->own_b = (u64 *)c->upload_pool.acquire(p.nb * 8);
-            if (!q->own_b || hipMemcpyAsync(q->own_b, req->b, p.nb * 8, hipMemcpyHostToDevice, c->copy_stream))
+            if (!q->own_b || c->up.copy(q->own_b, req->b, p.nb * 8, c->copy_stream))
                 rc = GNT_E_DEVICE;
         }
         if (!rc && hipStreamSynchronize(c->copy_stream) != hipSuccess) rc = GNT_E_DEVICE;

@@ -21,7 +21,9 @@
 
 #include "../../include/gnt.h"
 #include "gnt_plan.hpp"
+#include "gnt_stage.hpp"
 #include "gnt_runtime.hpp"
+#include "gnt_stage.hpp"
 
 using u64 = uint64_t;
 
@@ -83,6 +85,18 @@ int gnt_set_pass_tables(int max_log_m, int *previous) {
     std::lock_guard<std::mutex> g(gnt::g_plan_mu);
     if (previous) *previous = gnt::g_table_max_log;
     gnt::g_table_max_log = max_log_m;
+    return GNT_OK;
+}
+
+int gnt_set_staging(int threads, size_t chunk_bytes, size_t min_bytes) {
+    if (threads < 0 || threads > 64) return gnt::set_error(GNT_E_ARG, "threads out of range (0..64)");
+    if (threads > 0 && (chunk_bytes < (size_t(1) << 16) || chunk_bytes > (size_t(1) << 30)))
+        return gnt::set_error(GNT_E_ARG, "chunk_bytes out of range (64 KiB..1 GiB)");
+    gnt::StageConfig s;
+    s.threads = threads;
+    s.chunk = chunk_bytes;
+    s.min_bytes = min_bytes;
+    gnt::set_stage_config(s);
     return GNT_OK;
 }
 

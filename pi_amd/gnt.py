@@ -37,6 +37,7 @@ EXPORTS = (
     "gnt_batch_create", "gnt_batch_upload", "gnt_batch_run", "gnt_batch_download", "gnt_batch_info",
     "gnt_batch_destroy", "gnt_serve_start", "gnt_serve_stop", "gnt_submit", "gnt_poll", "gnt_wait",
     "gnt_serve_stats", "gnt_set_plan", "gnt_load_plans", "gnt_set_pass_tables", "gnt_tune",
+    "gnt_set_staging",
 )
 
 
@@ -107,6 +108,7 @@ def load() -> ctypes.CDLL:
     L.gnt_set_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.gnt_set_pass_tables.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.gnt_load_plans.argtypes = [ctypes.c_char_p]
+    L.gnt_set_staging.argtypes = [ctypes.c_int, SZ, SZ]
     L.gnt_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
     L.gnt_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.gnt_get_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
@@ -365,6 +367,13 @@ def set_pass_tables(max_log_m: int) -> int:
     prev = ctypes.c_int()
     _check(load().gnt_set_pass_tables(max_log_m, ctypes.byref(prev)))
     return prev.value
+
+
+def set_staging(threads: int = 4, chunk_bytes: int = 8 << 20, min_bytes: int = 4 << 20):
+    """Host <-> device copies of at least min_bytes go through pinned staging
+    chunks copied by `threads` host threads; threads=0 copies directly.
+    Process-wide; the arguments' defaults are the library's."""
+    _check(load().gnt_set_staging(threads, chunk_bytes, min_bytes))
 
 
 def set_plan(log_n: int, log_r=None):
