@@ -174,11 +174,12 @@ def main():
     ap.add_argument("--op", choices=["mul", "sqr"], default="mul")
     ap.add_argument("--cpu-log-size", type=int, default=22, help="log2 limbs of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky", "c2"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky", "c2", "copies"], default="single",
                     help="single: the metric (one 2^24 product per step); batched: 1024 uneven pairs per step; "
                          "serve: open-loop Poisson request stream through the serving loop; "
                          "chudnovsky: pi to --digits digits end to end; "
-                         "c2: mul and sqr at 2^20 and 2^24 limbs over three operand distributions")
+                         "c2: mul and sqr at 2^20 and 2^24 limbs over three operand distributions; "
+                         "copies: host <-> device copy rates and the PCIe-inclusive host-buffer multiply")
     ap.add_argument("--rates", default="100,200,400,800,1600", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--pairs", type=int, default=1024)
@@ -216,6 +217,8 @@ def main():
         return bench_chudnovsky(args)
     if args.workload == "c2":
         return bench_c2(args)
+    if args.workload == "copies":
+        return bench_copies(args)
 
     n = 1 << args.log_size
     a, b = operand(3, n), operand(4, n)
@@ -620,7 +623,8 @@ def bench_chudnovsky(args):
         "dtype": "u64",
         "data": "the Chudnovsky series (no input data)",
         "config": {"workload": f"pi to {args.digits} digits", "n_terms": st["n_terms"], "prec_bits": st["prec_bits"],
-                   "gpu_min_limbs": st["gpu_min_limbs"], "host_threads": st["threads"],
+                   "gpu_min_limbs": st["gpu_min_limbs"], "fa_min_limbs": st.get("fa_min_limbs"),
+                   "host_threads": st["threads"],
                    "plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in"},
         "digits_sha256": h.hexdigest(),
         "byte_identical_to_pinned": (h.hexdigest() == gold["sha256"]) if gold else None,
@@ -629,9 +633,76 @@ def bench_chudnovsky(args):
                                          "t_final_assembly", "t_decimal", "t_write")},
         "gpu_merge_share": round(merge_s / wall, 4),
         "levels": st["levels"],
+        "decimal_levels": st.get("decimal_levels"),
     }
     print(json.dumps(line), flush=True)
 
+
+
+def bench_copies(args):
+    """Host <-> device copies of pageable host memory (128 MiB), straight
+    through the runtime and through pinned staging, and the PCIe-inclusive
+    rate of the host-buffer multiply gnt_mul (upload two 2^24-limb operands,
+    product, download) with each.  Every copy and product is checked."""
+    import ctypes
+
+    from pi_amd import gnt
+    L = gnt.load()
+    ctx = make_ctx(args)
+    n = 16 << 20
+    a = operand(3, n)
+    out = np.ones(n, dtype=np.uint64)
+    m = 1 << 24
+    x, y = operand(3, m), operand(4, m)
+    o = np.ones(2 * m, dtype=np.uint64)
+    modes = {}
+    ref_low = None
+    for name, cfg in (("direct", (0, 8 << 20, 4 << 20)), ("staged", (4, 8 << 20, 4 << 20))):
+        gnt.set_staging(*cfg)
+        up, dn, mul = [], [], []
+        for _ in range(max(3, args.steps // 4) + 1):
+            h = ctypes.c_void_p()
+            t0 = time.perf_counter()
+            gnt._check(L.gnt_buf_upload(ctx.h, gnt._p(a), n, ctypes.byref(h)))
+            t1 = time.perf_counter()
+            got = gnt.SZ()
+            gnt._check(L.gnt_buf_download(ctx.h, h, gnt._p(out), n, ctypes.byref(got)))
+            t2 = time.perf_counter()
+            gnt._check(L.gnt_buf_release(ctx.h, h))
+            up.append(t1 - t0)
+            dn.append(t2 - t1)
+            ok_copy = got.value == n and np.array_equal(out, a)
+            out[:] = 1
+            ln = gnt.SZ()
+            t0 = time.perf_counter()
+            gnt._check(L.gnt_mul(ctx.h, gnt._p(x), m, gnt._p(y), m, gnt._p(o), 2 * m, ctypes.byref(ln)))
+            mul.append(time.perf_counter() - t0)
+            # every run gives the same product: low limb exact, whole result by a fingerprint
+            low = (int(x[0]) * int(y[0])) % (1 << 64)
+            fp = int(np.bitwise_xor.reduce(o[:ln.value] * np.uint64(0x9E3779B97F4A7C15)))
+            ref_low = ref_low or (ln.value, fp)
+            ok_mul = int(o[0]) == low and (ln.value, fp) == ref_low
+            o[:] = 1
+            if not (ok_copy and ok_mul):
+                raise SystemExit(f"copies: check failed in mode {name}")
+        t_up, t_dn, t_mul = (float(np.median(v[1:])) for v in (up, dn, mul))
+        modes[name] = {"threads": cfg[0], "chunk_bytes": cfg[1], "min_bytes": cfg[2],
+                       "upload_GBs": round(n * 8 / t_up / 1e9, 2), "download_GBs": round(n * 8 / t_dn / 1e9, 2),
+                       "gnt_mul_2p24_ms": round(t_mul * 1e3, 3),
+                       "gnt_mul_2p24_Mlimbs_per_s": round(2 * m / t_mul / 1e6, 1), "checks_ok": True}
+        print(json.dumps({name: modes[name]}), file=sys.stderr, flush=True)
+    gnt.set_staging()
+    ctx.close()
+    line = {
+        "metric": "host-buffer multiply Mlimbs/s at 2^24 limbs (PCIe-inclusive)",
+        "value": modes["staged"]["gnt_mul_2p24_Mlimbs_per_s"], "unit": "Mlimbs/s", "n_gpus": 1,
+        "steps": max(3, args.steps // 4), "warmup": 1,
+        "ms_per_step": modes["staged"]["gnt_mul_2p24_ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (SplitMix64 uniform limbs, seeds 3/4)",
+        "config": {"workload": "128 MiB pageable copies each way; gnt_mul of 2^24 x 2^24 limbs from and to "
+                               "pageable host memory", "modes": modes},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_c2(args):

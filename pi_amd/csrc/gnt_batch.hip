@@ -18,7 +18,13 @@ using gnt::Ctx;
 struct gnt_batch {
     Ctx *c = nullptr;
     std::vector<gnt::Prod> prods;
-    std::vector<u64 *> in_a, in_b;             // device operand buffers (owned)
+    std::vector<u64 *> in_a, in_b;             // device operands: slices of in_blob
+    // Operands, results and trimmed lengths of all products live in three
+    // device ranges, so a batch crosses the link in a few large transfers.
+    u64 *in_blob = nullptr, *out_blob = nullptr;
+    uint32_t *len_blob = nullptr;
+    size_t in_bytes = 0, out_bytes = 0;
+    std::vector<size_t> out_off;               // byte offset of each result in out_blob
     std::vector<gnt::Iteration> iters;
     std::vector<size_t> t_off, c_off;          // item offsets of each iteration in the blob
     void *blob = nullptr;                      // device descriptor tables
@@ -34,12 +40,11 @@ static void release_batch(gnt_batch *b) {
     Ctx *c = b->c;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    c->pool.release(b->in_blob, b->in_bytes);
+    c->pool.release(b->out_blob, b->out_bytes);
+    c->pool.release(b->len_blob, b->prods.size() * 4);
     for (size_t i = 0; i < b->prods.size(); i++) {
         Prod &p = b->prods[i];
-        c->pool.release(b->in_a[i], p.na * 8);
-        c->pool.release(b->in_b[i], p.sqr() ? 0 : p.nb * 8);
-        c->pool.release(p.dout, p.nl * 8);
-        c->pool.release(p.dlen, 4);
         c->pool.release(p.xa, p.N * 8);
         c->pool.release(p.xb, p.sqr() ? 0 : p.N * 8);
         c->pool.release(p.lh, p.nl * 16);
@@ -57,27 +62,51 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
     b->prods.resize(n);
     b->in_a.assign(n, nullptr);
     b->in_b.assign(n, nullptr);
+    b->out_off.assign(n, 0);
+    // shapes first, then one range each for operands, results and lengths
+    // (pieces on 256-byte boundaries)
+    auto rup = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    std::vector<size_t> a_off(n), b_off(n);
     for (size_t i = 0; i < n; i++) {
         Prod &p = b->prods[i];
+        const bool sq = is_sqr && is_sqr[i];
         p.na = na[i];
-        p.nb = is_sqr && is_sqr[i] ? na[i] : nb[i];
+        p.nb = sq ? na[i] : nb[i];
         if (p.na == 0 || p.nb == 0) { release_batch(b); return set_error(GNT_E_ARG, "empty operand in batch plan"); }
-        b->in_a[i] = (u64 *)c->pool.acquire(p.na * 8);
-        bool sq = is_sqr && is_sqr[i];
-        if (!sq) b->in_b[i] = (u64 *)c->pool.acquire(p.nb * 8);
+        p.db = sq ? nullptr : (const u64 *)1;  // shape only
+        if (!prod_init(p)) { release_batch(b); return set_error(GNT_E_TOO_LARGE, "product exceeds the transform cap"); }
+        a_off[i] = b->in_bytes;
+        b->in_bytes += rup(p.na * 8);
+        b_off[i] = b->in_bytes;
+        if (!sq) b->in_bytes += rup(p.nb * 8);
+        b->out_off[i] = b->out_bytes;
+        b->out_bytes += rup(p.nl * 8);
+    }
+    if (n) {
+        b->in_blob = (u64 *)c->pool.acquire(b->in_bytes);
+        b->out_blob = (u64 *)c->pool.acquire(b->out_bytes);
+        b->len_blob = (uint32_t *)c->pool.acquire(n * 4);
+        if (!b->in_blob || !b->out_blob || !b->len_blob) {
+            release_batch(b);
+            return set_error(GNT_E_NOMEM, "device memory exhausted");
+        }
+    }
+    for (size_t i = 0; i < n; i++) {
+        Prod &p = b->prods[i];
+        const bool sq = p.sqr();
+        b->in_a[i] = (u64 *)((char *)b->in_blob + a_off[i]);
+        if (!sq) b->in_b[i] = (u64 *)((char *)b->in_blob + b_off[i]);
         p.da = b->in_a[i];
         p.db = sq ? nullptr : b->in_b[i];
-        if (!prod_init(p)) { release_batch(b); return set_error(GNT_E_TOO_LARGE, "product exceeds the transform cap"); }
         c->pass_tables(p.pl, p.tabs);
-        p.dout = (u64 *)c->pool.acquire(p.nl * 8);
-        p.dlen = (uint32_t *)c->pool.acquire(4);
+        p.dout = (u64 *)((char *)b->out_blob + b->out_off[i]);
+        p.dlen = b->len_blob + i;
         p.xa = (u64 *)c->pool.acquire(p.N * 8);
         if (!sq) p.xb = (u64 *)c->pool.acquire(p.N * 8);
         p.lh = (u64 *)c->pool.acquire(p.nl * 16);
         p.cs = (unsigned char *)c->pool.acquire(p.cs_bytes());
         b->scratch_bytes += p.scratch_bytes() + p.na * 8 + p.nb * 8 + p.nl * 8;
-        if (!b->in_a[i] || (!sq && !b->in_b[i]) || !p.dout || !p.dlen || !p.xa || (!sq && !p.xb) || !p.lh ||
-            !p.cs) {
+        if (!p.xa || (!sq && !p.xb) || !p.lh || !p.cs) {
             release_batch(b);
             return set_error(GNT_E_NOMEM, "device memory exhausted");
         }
@@ -159,35 +188,42 @@ static int run_batch_serial(gnt_batch *b) {
     return GNT_OK;
 }
 
+// Every operand in one packed transfer of in_blob.
 static int upload_inputs(gnt_batch *b, const gnt_job *jobs) {
     Ctx *c = b->c;
+    std::vector<Seg> segs;
+    segs.reserve(2 * b->prods.size());
     for (size_t i = 0; i < b->prods.size(); i++) {
         const Prod &p = b->prods[i];
         if (!jobs[i].a || (!p.sqr() && !jobs[i].b)) return set_error(GNT_E_ARG, "null operand");
-        if (c->up.copy(b->in_a[i], jobs[i].a, p.na * 8, c->stream) != hipSuccess)
-            return set_error(GNT_E_DEVICE, "upload failed");
-        if (!p.sqr() && c->up.copy(b->in_b[i], jobs[i].b, p.nb * 8, c->stream) != hipSuccess)
-            return set_error(GNT_E_DEVICE, "upload failed");
+        segs.push_back(Seg{size_t((char *)b->in_a[i] - (char *)b->in_blob), (void *)jobs[i].a, p.na * 8});
+        if (!p.sqr()) segs.push_back(Seg{size_t((char *)b->in_b[i] - (char *)b->in_blob), (void *)jobs[i].b, p.nb * 8});
     }
+    if (c->up.gather(b->in_blob, b->in_bytes, segs, c->stream) != hipSuccess)
+        return set_error(GNT_E_DEVICE, "upload failed");
     return GNT_OK;
 }
 
+// The lengths in one transfer, then every trimmed result in one packed
+// transfer of out_blob.
 static int download_outputs(gnt_batch *b, gnt_job *jobs) {
     Ctx *c = b->c;
-    std::vector<uint32_t> lens(b->prods.size());
-    for (size_t i = 0; i < b->prods.size(); i++) {
-        const Prod &p = b->prods[i];
-        if (hipMemcpyAsync(&lens[i], p.dlen, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-            return set_error(GNT_E_DEVICE, "download failed");
-    }
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return set_error(GNT_E_DEVICE, "stream failure");
-    for (size_t i = 0; i < b->prods.size(); i++) {
+    const size_t n = b->prods.size();
+    if (n == 0) return GNT_OK;
+    std::vector<uint32_t> lens(n);
+    if (hipMemcpyAsync(lens.data(), b->len_blob, n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return set_error(GNT_E_DEVICE, "download failed");
+    std::vector<Seg> segs;
+    segs.reserve(n);
+    for (size_t i = 0; i < n; i++) {
         if (jobs[i].out_cap < lens[i]) { jobs[i].status = GNT_E_ARG; continue; }
-        if (lens[i] && c->down.copy(jobs[i].out, b->prods[i].dout, (size_t)lens[i] * 8, c->stream) != hipSuccess)
-            return set_error(GNT_E_DEVICE, "download failed");
+        if (lens[i]) segs.push_back(Seg{b->out_off[i], jobs[i].out, (size_t)lens[i] * 8});
         jobs[i].out_len = lens[i];
     }
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return set_error(GNT_E_DEVICE, "stream failure");
+    if (c->down.scatter(b->out_blob, b->out_bytes, segs, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return set_error(GNT_E_DEVICE, "download failed");
     return GNT_OK;
 }
 

@@ -12,6 +12,7 @@
 
 #include <condition_variable>
 #include <cstdint>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -28,11 +29,13 @@ struct StageConfig {
 StageConfig stage_config();
 void set_stage_config(const StageConfig &s);
 
-// A few host threads that copy one region in slices; the calling thread
-// takes the first slice.
+// A few host threads that share one job; the calling thread takes part 0.
 class CopyTeam {
   public:
     ~CopyTeam() { stop(); }
+    // fn(part, parts) for every part in [0, parts), parts = max(threads, 1)
+    void run(int threads, const std::function<void(int, int)> &fn);
+    // memcpy in slices of whole cache lines
     void copy(void *dst, const void *src, size_t bytes, int threads);
     void stop();
 
@@ -41,12 +44,19 @@ class CopyTeam {
     std::vector<std::thread> th_;
     std::mutex mu_;
     std::condition_variable go_, done_;
-    char *dst_ = nullptr;
-    const char *src_ = nullptr;
-    size_t slice_ = 0, bytes_ = 0;
+    const std::function<void(int, int)> *fn_ = nullptr;
+    int parts_ = 0;
     uint64_t gen_ = 0;
     int left_ = 0;
     bool quit_ = false;
+};
+
+// One piece of a packed transfer: bytes at offset off of a device range, and
+// the host memory they come from (to the device) or go to (to the host).
+struct Seg {
+    size_t off;
+    void *host;
+    size_t bytes;
 };
 
 class Stager {
@@ -58,11 +68,25 @@ class Stager {
     // memory has been read (its last transfers may still be in flight); a
     // staged copy to the host returns with dst already written.
     hipError_t copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+    // Many pieces of one device range [dev, dev + total) in one pass: the
+    // pieces (sorted by offset, not overlapping) are packed into the pinned
+    // chunks, and each chunk crosses the link as one transfer; bytes between
+    // pieces are left undefined on the device and ignored on the host.  The
+    // same ordering and completion rules as copy().
+    hipError_t gather(void *dev, size_t total, const std::vector<Seg> &segs, hipStream_t st);
+    hipError_t scatter(const void *dev, size_t total, const std::vector<Seg> &segs, hipStream_t st);
     void release();  // frees the pinned slots and stops the team
 
   private:
     static constexpr int SLOTS = 4;
     hipError_t ensure(size_t chunk);
+    hipError_t direct(void *dst, const void *src, size_t bytes, hipStream_t st);
+    // pipelined chunks of [0, total): fill(slot, lo, hi) packs a chunk to
+    // send, drain(slot, lo, hi) unpacks a received one
+    hipError_t up_chunks(void *dev, size_t total, int threads,
+                         const std::function<void(char *, size_t, size_t)> &fill, hipStream_t st);
+    hipError_t down_chunks(const void *dev, size_t total, int threads,
+                           const std::function<void(const char *, size_t, size_t)> &drain, hipStream_t st);
     const bool up_;
     std::mutex mu_;
     CopyTeam team_;

@@ -113,7 +113,7 @@ struct Options {
     int threads = 0;
     bool host_merge = false;      // merge the upper tree on the host too (comparison run)
     bool fa_gmp = false;          // final assembly and radix conversion by GMP (comparison run)
-    double fa_min_limbs = 8192;   // final-assembly / conversion products on the GPU from this size
+    double fa_min_limbs = 2048;   // final-assembly / conversion products on the GPU from this size
     int device = 0;
     std::string plans;            // plan cache from the autotuner
     std::string algorithm = "chudnovsky";
@@ -963,14 +963,14 @@ int main(int argc, char **argv) {
     }
     lv += "]";
     std::printf("{\"digits\": %llu, \"n_terms\": %llu, \"prec_bits\": %llu, \"merge\": \"%s\", \"threads\": %d, "
-                "\"gpu_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
+                "\"gpu_min_limbs\": %.0f, \"fa_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
                 "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
                 "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_shutdown\": %.3f, \"t_total\": %.3f, "
                 "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"fa\": \"%s\", \"fa_gpu_products\": %zu, "
                 "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"decimal_levels\": %s, \"levels\": %s}\n",
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
                 !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"), o.threads, o.gpu_min_limbs,
-                d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
+                o.fa_min_limbs, d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
                 t_download, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
                 M.host_products, M.gpu_limbs, g_decimal_stats.c_str(), lv.c_str());

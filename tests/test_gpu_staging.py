@@ -90,3 +90,35 @@ def test_concurrent_copies(ctx, orc, staging):
     for t in ths:
         t.join()
     assert not errors
+
+
+def test_batch_packed_transfers(ctx, orc, staging):
+    """Many uneven pairs and squares in one batch: the packed operand and
+    result transfers cross chunk boundaries inside and between pieces."""
+    import random
+    rnd = random.Random(12)
+    pairs, want = [], []
+    for k in range(300):
+        na = int(2 ** rnd.uniform(0, 16))
+        nb = int(2 ** rnd.uniform(0, 16))
+        a = _limbs(orc, 3 * k + 1, na)
+        if k % 7 == 0:
+            pairs.append((a, None))
+            want.append(orc.gmp_sqr(a))
+        else:
+            b = _limbs(orc, 3 * k + 2, nb)
+            pairs.append((a, b))
+            want.append(orc.gmp_mul(a, b))
+    for cfg in [(0, 8 << 20, 0), (4, CHUNK, 0), (3, 192 << 10, 0), (4, 8 << 20, 4 << 20)]:
+        print(f"staging {cfg}", flush=True)
+        staging(*cfg)
+        got = ctx.mul_batch(pairs)
+        for i, (g, w) in enumerate(zip(got, want)):
+            assert np.array_equal(g, w), (cfg, i)
+        bp = ctx.batch_plan(pairs, graph=False)
+        try:
+            bp.run()
+            for i, (g, w) in enumerate(zip(bp.results(), want)):
+                assert np.array_equal(g, w), (cfg, "plan", i)
+        finally:
+            bp.close()
