@@ -190,7 +190,7 @@ def main():
     ap.add_argument("--digits", type=int, default=100_000_000, help="chudnovsky: decimal digits")
     ap.add_argument("--latency-rate", type=float, default=400.0,
                     help="single: arrival rate (requests/s) of the serving run whose p50/p99 the line reports")
-    ap.add_argument("--latency-requests", type=int, default=300)
+    ap.add_argument("--latency-requests", type=int, default=1000)
     ap.add_argument("--no-latency", action="store_true", help="single: skip the serving latency run")
     args = ap.parse_args()
 
@@ -443,6 +443,18 @@ def serve_open(args):
     return ctx, srv, src
 
 
+def _latency_by_size(lat_ms, sizes) -> dict:
+    """p50/p99 (ms, from scheduled arrival) per operand-size band 2^[8,12),
+    2^[12,16), 2^[16,20), 2^[20,24]."""
+    out = {}
+    for lo, hi in ((8, 12), (12, 16), (16, 20), (20, 25)):
+        sel = lat_ms[(sizes >= (1 << lo)) & (sizes < (1 << hi))]
+        if sel.size:
+            out[f"2^{lo}-2^{min(hi, 24)}"] = {"requests": int(sel.size), "p50_ms": round(float(np.percentile(sel, 50)), 3),
+                                                "p99_ms": round(float(np.percentile(sel, 99)), 3)}
+    return out
+
+
 def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
     """One open-loop Poisson run at `rate` requests/s: n = round(2^U[8,24])
     limbs, 70% multiply / 30% square, 20% of requests take a finished result
@@ -467,7 +479,7 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
     ready = []                      # finished results kept on the device for reuse
     consumed, sampled = [], []
     lat, lat_open, limbs_done = [], [], [0]
-    tickets, meta, late = [], {}, {}
+    tickets, meta, late, size_of, sizes = [], {}, {}, {}, []
     done_evt = threading.Event()
 
     def waiter():
@@ -478,6 +490,7 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
             buf, us = srv.wait(t)
             lat.append(us)
             lat_open.append(us + late[t])
+            sizes.append(size_of[t])
             n_in, check, keep = meta[t]
             limbs_done[0] += n_in
             if check is not None:
@@ -519,6 +532,7 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
             t = srv.submit(None if a_buf is not None else a, b, a_buf=a_buf)
             n_in = n_a + n
         meta[t] = (n_in, check, r["keep"])
+        size_of[t] = n
         # time between the scheduled arrival and the submit call, so that a
         # submit delayed by an earlier one still counts from its arrival
         late[t] = max(0.0, t_call - (t0 + r["at"])) * 1e6
@@ -549,6 +563,7 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
             "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
             "submit_to_done_p50_ms": round(float(np.percentile(sub_ms, 50)), 3),
             "submit_to_done_p99_ms": round(float(np.percentile(sub_ms, 99)), 3),
+            "by_size": _latency_by_size(lat_ms, np.array(sizes)),
             "throughput_Mlimbs_per_s": round(limbs_done[0] / wall / 1e6, 1),
             "achieved_req_per_s": round(requests / wall, 1),
             "exact_checks_passed": f"{ok}/{len(sampled)}"}

@@ -37,7 +37,9 @@ typedef struct gnt_config {
     int device;                 /* HIP device ordinal */
     size_t hbm_cap_bytes;       /* admission cap for serving/batches; 0 = 200 GB */
     const char *plan_cache_path;/* autotuner plan cache (JSON); NULL = built-in plans */
-    size_t reserved[4];
+    size_t serve_split_log_n;   /* serving: products with transforms up to 2^this run on the
+                                   small-request lane, larger ones on the large lane; 0 = 22 */
+    size_t reserved[3];
 } gnt_config;
 
 /* ---- lifetime ---- */
@@ -125,7 +127,8 @@ int gnt_batch_destroy(gnt_batch *b);
  * (one launch per kernel shape per iteration) and admits waiting requests, in
  * arrival order, between iterations while the bytes in flight stay under
  * gnt_config.hbm_cap_bytes.  Host operands must stay valid until the request
- * completes.  An operand may instead be a gnt_buf (e.g. an earlier result).
+ * completes.  An operand may instead be a gnt_buf (e.g. an earlier result),
+ * which likewise must not be released before the request completes.
  * No reference counterpart: the reference has no serving mode. */
 enum {
     GNT_OP_MUL = 0, /* a * b */
@@ -151,7 +154,8 @@ int gnt_poll(gnt_ctx *ctx, uint64_t ticket, int *done);
  * device buffer owned by the caller, its trimmed length already known (no
  * device round trip in gnt_buf_len); *latency_us = submit-to-completion. */
 int gnt_wait(gnt_ctx *ctx, uint64_t ticket, int64_t timeout_us, gnt_buf **result, double *latency_us);
-/* [0] iterations [1] launches [2] completed [3] peak admitted [4] peak bytes in flight [5] bytes in flight */
+/* [0] iterations [1] launches [2] completed [3] peak admitted [4] peak bytes in flight [5] bytes in flight
+ * [6] iterations of the small-request lane [7] of the large-request lane (n <= 8 values) */
 int gnt_serve_stats(gnt_ctx *ctx, uint64_t *stats, int n);
 
 /* ---- transform plans and the autotuner (no reference counterpart: the

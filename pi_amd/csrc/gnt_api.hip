@@ -191,7 +191,9 @@ void Ctx::pass_tables(const Plan &pl, PassTables &out) {
                 if (hipMalloc(&T, sizeof(uint64_t) << d.log_m) != hipSuccess) {
                     (void)hipGetLastError();
                     T = nullptr;
-                } else if (launch_pass_table(T, d.log_m, d.log_r, inv != 0, scale_e, tw(), stream) != 0) {
+                } else if (launch_pass_table(T, d.log_m, d.log_r, inv != 0, scale_e, tw(), stream) != 0 ||
+                           hipStreamSynchronize(stream) != hipSuccess) {
+                    // built before first use: the serving lanes read it on their own streams
                     (void)hipFree(T);
                     T = nullptr;
                 }
@@ -340,6 +342,11 @@ int gnt_init(gnt_ctx **out, const gnt_config *cfg) {
     Ctx *c = new Ctx();
     c->device = dev;
     c->hbm_cap = (cfg && cfg->hbm_cap_bytes) ? cfg->hbm_cap_bytes : (size_t)200 * 1000 * 1000 * 1000;
+    if (cfg && cfg->serve_split_log_n > (size_t)gnt::MAX_LOG_N) {
+        delete c;
+        return fail(GNT_E_ARG, "serve_split_log_n above %d", gnt::MAX_LOG_N);
+    }
+    c->serve_split_log_n = (cfg && cfg->serve_split_log_n) ? (int)cfg->serve_split_log_n : 22;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;

@@ -54,6 +54,7 @@ struct Ctx {
     Pool upload_pool;  // serving operands written on the copy stream; freed only after completion
     Stager up{true}, down{false};  // host <-> device copies of caller memory
     size_t hbm_cap = 0;
+    int serve_split_log_n = 22;  // serving: largest transform (log2) on the small-request lane
     bool timing = false;
     double phase_ms[5] = {0, 0, 0, 0, 0};
     uint64_t products = 0;

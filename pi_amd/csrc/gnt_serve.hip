@@ -1,18 +1,25 @@
 // Serving loop: continuous batching of multiply/square requests.
 //
-// A scheduler thread owns the context's stream while serving.  Each loop
-// iteration (1) admits waiting requests in arrival order while the bytes in
-// flight stay under the HBM cap, uploading their host operands, (2) advances
-// every admitted product by one stage through the stage engine (one launch
-// per kernel shape for the whole iteration), and (3) retires products whose
-// last stage has completed on the device.  Requests that arrive while others
-// run join at the next iteration, so their work overlaps on the device.
+// Two lanes, each a scheduler thread with its own stream: products up to a
+// transform length go to the small lane, larger ones to the large lane, so
+// a small request never waits behind a large product's stages while both
+// streams share the device.  Each loop iteration of a lane (1) admits its
+// waiting requests in arrival order while the bytes in flight (both lanes)
+// stay under the HBM cap, (2) advances every admitted product by one stage
+// through the stage engine (one launch per kernel shape for the whole
+// iteration), and (3) retires products whose last stage has completed on the
+// device.  Requests that arrive while others run join at the next iteration,
+// so their work overlaps on the device.
 // Results stay in HBM as gnt_buf handles (handed out once complete, and not
 // counted against the cap afterwards); a request may take such a result as an
-// operand without a host round trip.  Host operands are uploaded by
-// gnt_submit on a separate copy stream, so PCIe transfers overlap compute.
+// operand without a host round trip.  Host operands are uploaded by an
+// uploader thread on a separate copy stream, so PCIe transfers overlap
+// compute: gnt_submit returns at once, and the uploader moves one slice at a
+// time of the request with the fewest bytes left, so a small request never
+// waits behind the whole upload of a large one.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -51,6 +58,9 @@ struct Req {
     hipEvent_t last = nullptr;               // recorded after the product's last stage
     int len_slot = -1;                       // pinned word the trimmed length is copied to
     int64_t len = -1;                        // trimmed result length once known
+    size_t up_total = 0, up_done = 0;        // host operand bytes to upload, uploaded so far
+    hipEvent_t uploaded = nullptr;           // recorded on the copy stream after the last slice
+    void *lane = nullptr;                    // the lane that runs it
 };
 
 struct Slot {  // descriptor staging for one iteration in flight
@@ -64,75 +74,79 @@ struct Slot {  // descriptor staging for one iteration in flight
 
 namespace gnt {
 
-struct Server {
-    Ctx *c;
+struct Server;
+
+// One scheduler thread over one stream.  It admits its own requests in
+// arrival order while the server's bytes in flight stay under the cap,
+// advances every admitted product one stage per iteration, and retires the
+// finished ones.  Scratch comes from the lane's own pool, whose reuse follows
+// the lane's stream order.
+struct Lane {
+    Server *s = nullptr;
+    Ctx *c = nullptr;
+    hipStream_t stream = nullptr;
+    Pool scratch;
     std::thread th;
-    std::mutex mu;         // queue, tickets
-    std::mutex upload_mu;  // submissions share the copy stream
-    std::condition_variable cv_work, cv_done;
-    bool stop = false;
-    std::deque<Req *> waiting;
-    std::vector<Req *> active;                  // admitted, stages still to enqueue
-    std::vector<Req *> draining;                // all stages enqueued, waiting for completion
-    std::map<uint64_t, Req *> tickets;
-    uint64_t next_ticket = 1;
-    size_t bytes_in_flight = 0, peak_bytes = 0;
-    uint64_t iterations = 0, launches = 0, completed = 0, peak_active = 0;
+    std::condition_variable cv_work;
+    std::deque<Req *> waiting;   // guarded by Server::mu
+    std::vector<Req *> active;   // admitted, stages still to enqueue
+    std::vector<Req *> draining; // all stages enqueued, waiting for completion
     static constexpr int NSLOT = 2;
     Slot slots[NSLOT];
     int slot_next = 0;
     // Pinned words that receive each finished product's trimmed length on the
-    // compute stream, so results are handed out with their length known.
-    static constexpr int NLEN = 1 << 16;
+    // lane's stream, so results are handed out with their length known.
+    static constexpr int NLEN = 1 << 15;
     uint32_t *lens = nullptr;
-    std::vector<int> lens_free;
+    std::vector<int> lens_free;  // guarded by Server::mu
+    hipEvent_t ctx_ev = nullptr; // orders admissions after the context stream's work
 
-    explicit Server(Ctx *ctx) : c(ctx) {
-        if (hipHostMalloc((void **)&lens, NLEN * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
-            lens = nullptr;
-        if (lens)
-            for (int i = NLEN - 1; i >= 0; i--) lens_free.push_back(i);
+    int init(Server *srv, Ctx *ctx);
+    void destroy();
+    bool admissible() const;     // Server::mu held
+    int admit(Req *q);
+    void free_scratch(Req *q);
+    void fail_request(Req *q, int code);
+    Slot &slot_for(size_t bytes);
+    void loop();
+    void run_iteration();
+    void retire();
+};
+
+struct Server {
+    Ctx *c;
+    std::mutex mu;         // queues, tickets, counters
+    std::condition_variable cv_done;
+    bool stop = false;
+    std::map<uint64_t, Req *> tickets;
+    uint64_t next_ticket = 1;
+    size_t bytes_in_flight = 0, peak_bytes = 0;
+    uint64_t iterations = 0, launches = 0, completed = 0, peak_active = 0;
+    uint64_t n_admitted = 0;  // admitted by either lane and not yet finished
+    uint64_t lane_iterations[2] = {0, 0};
+    std::condition_variable cv_up;
+    std::deque<Req *> uploads;   // host operands still to upload (guarded by mu)
+    bool stop_up = false;
+    std::thread up_th;
+    static constexpr size_t UPLOAD_SLICE = size_t(8) << 20;
+    void upload_loop();
+    void fail_upload(Req *q, int code);
+    // Products whose transform length is at most 2^small_max_log_n run on
+    // lane 0, larger ones on lane 1: a small request then waits for small
+    // stages only, while the two streams share the device.
+    int small_max_log_n;
+    Lane lanes[2];
+
+    explicit Server(Ctx *ctx, int split) : c(ctx), small_max_log_n(split) {}
+
+    Lane &lane_for(const gnt::Prod &p) {
+        const size_t N = p.linear() ? 4 * p.nl : p.N;
+        return N <= (size_t(1) << small_max_log_n) ? lanes[0] : lanes[1];
     }
 
     size_t footprint(Req *q) {
         gnt::Prod &p = q->p;
         return p.scratch_bytes() + p.nl * 8 + (q->own_a ? p.na * 8 : 0) + (q->own_b ? p.nb * 8 : 0);
-    }
-
-    // Buffers of an admitted request (scheduler thread only).  Host operands
-    // were uploaded at submission (copy stream), so only scratch is allocated.
-    int admit(Req *q) {
-        gnt::Prod &p = q->p;
-        const bool sq = q->r.op == GNT_OP_SQR;
-        u64 *d;
-        size_t cap;
-        if (q->r.a_buf) { buf_info(q->r.a_buf, &d, &cap); p.da = d; }
-        else p.da = q->own_a;
-        if (sq) p.db = nullptr;
-        else if (q->r.b_buf) { buf_info(q->r.b_buf, &d, &cap); p.db = d; }
-        else p.db = q->own_b;
-        p.dout = (u64 *)c->pool.acquire(p.nl * 8);
-        p.dlen = (uint32_t *)c->pool.acquire(4);
-        if (p.linear()) {
-            p.cs = (unsigned char *)c->pool.acquire(p.cs_bytes());
-            return (p.dout && p.dlen && p.cs) ? GNT_OK : GNT_E_NOMEM;
-        }
-        p.xa = (u64 *)c->pool.acquire(p.N * 8);
-        p.xb = sq ? nullptr : (u64 *)c->pool.acquire(p.N * 8);
-        p.lh = (u64 *)c->pool.acquire(p.nl * 16);
-        p.cs = (unsigned char *)c->pool.acquire(p.cs_bytes());
-        if (!p.dout || !p.dlen || !p.xa || (!sq && !p.xb) || !p.lh || !p.cs) return GNT_E_NOMEM;
-        return GNT_OK;
-    }
-
-    void free_scratch(Req *q) {
-        gnt::Prod &p = q->p;
-        c->pool.release(p.xa, p.N * 8);
-        c->pool.release(p.xb, p.N * 8);
-        c->pool.release(p.lh, p.nl * 16);
-        c->pool.release(p.cs, p.cs_bytes());
-        p.xa = p.xb = p.lh = nullptr;
-        p.cs = nullptr;
     }
 
     // Operands uploaded on the copy stream: released only once the product
@@ -143,135 +157,294 @@ struct Server {
         q->own_a = q->own_b = nullptr;
     }
 
-    Slot &slot_for(size_t bytes) {
-        Slot &s = slots[slot_next];
-        slot_next = (slot_next + 1) % NSLOT;
-        if (s.used) (void)hipEventSynchronize(s.ev);  // its copy and kernels are done
-        if (s.bytes < bytes) {
-            if (s.host) (void)hipHostFree(s.host);
-            if (s.dev) (void)hipFree(s.dev);
-            s.bytes = std::max(bytes, (size_t)1 << 16);
-            if (hipHostMalloc(&s.host, s.bytes, hipHostMallocDefault) != hipSuccess) s.host = nullptr;
-            if (hipMalloc(&s.dev, s.bytes) != hipSuccess) s.dev = nullptr;
-        }
-        if (!s.ev) (void)hipEventCreateWithFlags(&s.ev, hipEventDisableTiming);
-        s.used = true;
-        return s;
+    // Bytes came free: every lane may now admit its head request.
+    void wake_lanes() {
+        for (Lane &l : lanes) l.cv_work.notify_one();
     }
+};
 
-    void fail_request(Req *q, int code) {
-        (void)hipStreamSynchronize(c->stream);
-        free_scratch(q);
-        free_operands(q);
-        c->pool.release(q->p.dout, q->p.nl * 8);
-        c->pool.release(q->p.dlen, 4);
+void Server::fail_upload(Req *q, int code) {
+    (void)hipStreamSynchronize(c->copy_stream);
+    free_operands(q);
+    {
         std::lock_guard<std::mutex> g(mu);
-        bytes_in_flight -= q->bytes;
         q->status = code;
         q->done = true;
         q->t_done = Clock::now();
-        cv_done.notify_all();
     }
+    cv_done.notify_all();
+}
 
-    void loop() {
-        (void)hipSetDevice(c->device);
-        for (;;) {
-            // pace with the device: at most two iterations queued, so requests
-            // that arrive meanwhile join the next iteration
-            Slot &pace = slots[slot_next];
-            if (pace.used) (void)hipEventSynchronize(pace.ev);
-            std::vector<Req *> admitted;
-            {
-                std::unique_lock<std::mutex> g(mu);
-                cv_work.wait(g, [&] { return stop || !waiting.empty() || !active.empty() || !draining.empty(); });
-                if (stop && waiting.empty() && active.empty() && draining.empty()) return;
-                // admission in arrival order while the bytes in flight fit the cap
-                while (!waiting.empty()) {
-                    Req *q = waiting.front();
-                    // bytes_in_flight counts everything admitted and not retired, including
-                    // requests admitted earlier in this pass; an empty device admits anything
-                    if (bytes_in_flight > 0 && bytes_in_flight + q->bytes > c->hbm_cap) break;
-                    waiting.pop_front();
-                    bytes_in_flight += q->bytes;
-                    peak_bytes = std::max(peak_bytes, bytes_in_flight);
-                    admitted.push_back(q);
-                }
+// One slice at a time of the pending request with the fewest bytes left
+// (operand a, then b), through the context's staged copies on the copy
+// stream.  A request whose last slice is issued moves to its lane's queue
+// with an event its lane waits on before the first stage.
+void Server::upload_loop() {
+    (void)hipSetDevice(c->device);
+    for (;;) {
+        Req *q;
+        size_t done;
+        {
+            std::unique_lock<std::mutex> g(mu);
+            cv_up.wait(g, [&] { return stop_up || !uploads.empty(); });
+            if (uploads.empty()) return;  // stopping, and nothing left
+            q = *std::min_element(uploads.begin(), uploads.end(), [](const Req *x, const Req *y) {
+                return x->up_total - x->up_done < y->up_total - y->up_done;
+            });
+            done = q->up_done;
+        }
+        const size_t a_bytes = q->own_a ? q->p.na * 8 : 0;
+        size_t len;
+        hipError_t e;
+        if (done < a_bytes) {
+            len = std::min(UPLOAD_SLICE, a_bytes - done);
+            e = c->up.copy((char *)q->own_a + done, (const char *)q->r.a + done, len, c->copy_stream);
+        } else {
+            const size_t o = done - a_bytes;
+            len = std::min(UPLOAD_SLICE, q->p.nb * 8 - o);
+            e = c->up.copy((char *)q->own_b + o, (const char *)q->r.b + o, len, c->copy_stream);
+        }
+        const bool finished = done + len == q->up_total;
+        if (e == hipSuccess && finished) {
+            e = hipEventCreateWithFlags(&q->uploaded, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventRecord(q->uploaded, c->copy_stream);
+        }
+        Lane *lane = (Lane *)q->lane;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            q->up_done = done + len;
+            if (e != hipSuccess || finished) uploads.erase(std::find(uploads.begin(), uploads.end(), q));
+            if (e == hipSuccess && finished) lane->waiting.push_back(q);
+        }
+        if (e != hipSuccess) fail_upload(q, GNT_E_DEVICE);
+        else if (finished) lane->cv_work.notify_one();
+    }
+}
+
+int Lane::init(Server *srv, Ctx *ctx) {
+    s = srv;
+    c = ctx;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return GNT_E_DEVICE;
+    if (hipEventCreateWithFlags(&ctx_ev, hipEventDisableTiming) != hipSuccess) return GNT_E_DEVICE;
+    if (hipHostMalloc((void **)&lens, NLEN * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+        lens = nullptr;
+    if (lens)
+        for (int i = NLEN - 1; i >= 0; i--) lens_free.push_back(i);
+    return GNT_OK;
+}
+
+void Lane::destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &slot : slots) {
+        if (slot.ev) (void)hipEventDestroy(slot.ev);
+        if (slot.host) (void)hipHostFree(slot.host);
+        if (slot.dev) (void)hipFree(slot.dev);
+        slot = Slot{};
+    }
+    if (lens) (void)hipHostFree(lens);
+    lens = nullptr;
+    if (ctx_ev) (void)hipEventDestroy(ctx_ev);
+    ctx_ev = nullptr;
+    scratch.release_all();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+// The head request fits beside the bytes in flight (an empty device admits
+// anything).
+bool Lane::admissible() const {
+    if (waiting.empty()) return false;
+    return s->bytes_in_flight == 0 || s->bytes_in_flight + waiting.front()->bytes <= c->hbm_cap;
+}
+
+// Buffers of an admitted request (lane thread only).  Host operands were
+// uploaded at submission (copy stream), so only the result and scratch are
+// allocated: the result from the context's pool (it outlives the request),
+// scratch from the lane's.
+int Lane::admit(Req *q) {
+    gnt::Prod &p = q->p;
+    const bool sq = q->r.op == GNT_OP_SQR;
+    u64 *d;
+    size_t cap;
+    if (q->r.a_buf) { buf_info(q->r.a_buf, &d, &cap); p.da = d; }
+    else p.da = q->own_a;
+    if (sq) p.db = nullptr;
+    else if (q->r.b_buf) { buf_info(q->r.b_buf, &d, &cap); p.db = d; }
+    else p.db = q->own_b;
+    p.dout = (u64 *)c->pool.acquire(p.nl * 8);
+    p.dlen = (uint32_t *)c->pool.acquire(4);
+    if (p.linear()) {
+        p.cs = (unsigned char *)scratch.acquire(p.cs_bytes());
+        return (p.dout && p.dlen && p.cs) ? GNT_OK : GNT_E_NOMEM;
+    }
+    p.xa = (u64 *)scratch.acquire(p.N * 8);
+    p.xb = sq ? nullptr : (u64 *)scratch.acquire(p.N * 8);
+    p.lh = (u64 *)scratch.acquire(p.nl * 16);
+    p.cs = (unsigned char *)scratch.acquire(p.cs_bytes());
+    if (!p.dout || !p.dlen || !p.xa || (!sq && !p.xb) || !p.lh || !p.cs) return GNT_E_NOMEM;
+    return GNT_OK;
+}
+
+void Lane::free_scratch(Req *q) {
+    gnt::Prod &p = q->p;
+    scratch.release(p.xa, p.N * 8);
+    scratch.release(p.xb, p.N * 8);
+    scratch.release(p.lh, p.nl * 16);
+    scratch.release(p.cs, p.cs_bytes());
+    p.xa = p.xb = p.lh = nullptr;
+    p.cs = nullptr;
+}
+
+Slot &Lane::slot_for(size_t bytes) {
+    Slot &sl = slots[slot_next];
+    slot_next = (slot_next + 1) % NSLOT;
+    if (sl.used) (void)hipEventSynchronize(sl.ev);  // its copy and kernels are done
+    if (sl.bytes < bytes) {
+        if (sl.host) (void)hipHostFree(sl.host);
+        if (sl.dev) (void)hipFree(sl.dev);
+        sl.bytes = std::max(bytes, (size_t)1 << 16);
+        if (hipHostMalloc(&sl.host, sl.bytes, hipHostMallocDefault) != hipSuccess) sl.host = nullptr;
+        if (hipMalloc(&sl.dev, sl.bytes) != hipSuccess) sl.dev = nullptr;
+    }
+    if (!sl.ev) (void)hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+    sl.used = true;
+    return sl;
+}
+
+void Lane::fail_request(Req *q, int code) {
+    (void)hipStreamSynchronize(stream);
+    free_scratch(q);
+    if (q->uploaded) (void)hipEventDestroy(q->uploaded);
+    q->uploaded = nullptr;
+    s->free_operands(q);
+    c->pool.release(q->p.dout, q->p.nl * 8);
+    c->pool.release(q->p.dlen, 4);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->bytes_in_flight -= q->bytes;
+        s->n_admitted--;
+        q->status = code;
+        q->done = true;
+        q->t_done = Clock::now();
+    }
+    s->cv_done.notify_all();
+    s->wake_lanes();
+}
+
+void Lane::loop() {
+    (void)hipSetDevice(c->device);
+    for (;;) {
+        // pace with the device: at most two iterations queued, so requests
+        // that arrive meanwhile join the next iteration
+        Slot &pace = slots[slot_next];
+        if (pace.used) (void)hipEventSynchronize(pace.ev);
+        std::vector<Req *> admitted;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            cv_work.wait(g, [&] { return s->stop || admissible() || !active.empty() || !draining.empty(); });
+            if (s->stop && waiting.empty() && active.empty() && draining.empty()) return;
+            // admission in arrival order while the bytes in flight fit the cap
+            // (bytes_in_flight counts everything admitted by either lane and
+            // not retired, including requests admitted earlier in this pass)
+            while (admissible()) {
+                Req *q = waiting.front();
+                waiting.pop_front();
+                s->bytes_in_flight += q->bytes;
+                s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_flight);
+                s->peak_active = std::max(s->peak_active, ++s->n_admitted);
+                admitted.push_back(q);
             }
-            {
-                std::lock_guard<std::mutex> cg(c->mu);
-                for (Req *q : admitted) {
-                    int rc = admit(q);
-                    if (rc) fail_request(q, rc);
-                    else active.push_back(q);
-                }
-                peak_active = std::max<uint64_t>(peak_active, active.size());
-                if (!active.empty()) run_iteration();
+        }
+        {
+            std::lock_guard<std::mutex> cg(c->mu);
+            if (!admitted.empty()) {
+                // result buffers come from the context's pool, whose earlier
+                // users ran on the context stream
+                (void)hipEventRecord(ctx_ev, c->stream);
+                (void)hipStreamWaitEvent(stream, ctx_ev, 0);
             }
-            retire();
+            for (Req *q : admitted) {
+                // the operands' last slices may still be on the copy stream
+                if (q->uploaded) (void)hipStreamWaitEvent(stream, q->uploaded, 0);
+                int rc = admit(q);
+                if (rc) fail_request(q, rc);
+                else active.push_back(q);
+            }
+            if (!active.empty()) run_iteration();
+        }
+        retire();
+    }
+}
+
+void Lane::run_iteration() {
+    Iteration it;
+    for (Req *q : active) add_stage(it, q->p);
+    finalize_iteration(it);
+    const size_t tb = (it.titems.size() * sizeof(TItem) + 255) & ~(size_t)255;
+    const size_t bytes = tb + it.citems.size() * sizeof(CItem) + 256;
+    Slot &sl = slot_for(bytes);
+    std::memcpy(sl.host, it.titems.data(), it.titems.size() * sizeof(TItem));
+    std::memcpy((unsigned char *)sl.host + tb, it.citems.data(), it.citems.size() * sizeof(CItem));
+    (void)hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, stream);
+    launch_iteration(it, (const TItem *)sl.dev, (const CItem *)((unsigned char *)sl.dev + tb), c->tw(), stream);
+    (void)hipEventRecord(sl.ev, stream);
+    std::vector<Req *> still;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->iterations++;
+    s->lane_iterations[this == &s->lanes[0] ? 0 : 1]++;
+    s->launches += it.groups.size();
+    for (Req *q : active) {
+        if (q->p.stage >= q->p.nstages) {
+            if (!lens_free.empty()) {
+                q->len_slot = lens_free.back();
+                lens_free.pop_back();
+                (void)hipMemcpyAsync(lens + q->len_slot, q->p.dlen, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+            }
+            (void)hipEventCreateWithFlags(&q->last, hipEventDisableTiming);
+            (void)hipEventRecord(q->last, stream);
+            free_scratch(q);  // later work on this lane is stream-ordered after this product
+            draining.push_back(q);
+        } else {
+            still.push_back(q);
         }
     }
+    active.swap(still);
+}
 
-    void run_iteration() {
-        Iteration it;
-        for (Req *q : active) add_stage(it, q->p);
-        finalize_iteration(it);
-        const size_t tb = (it.titems.size() * sizeof(TItem) + 255) & ~(size_t)255;
-        const size_t bytes = tb + it.citems.size() * sizeof(CItem) + 256;
-        Slot &s = slot_for(bytes);
-        std::memcpy(s.host, it.titems.data(), it.titems.size() * sizeof(TItem));
-        std::memcpy((unsigned char *)s.host + tb, it.citems.data(), it.citems.size() * sizeof(CItem));
-        (void)hipMemcpyAsync(s.dev, s.host, bytes, hipMemcpyHostToDevice, c->stream);
-        launch_iteration(it, (const TItem *)s.dev, (const CItem *)((unsigned char *)s.dev + tb), c->tw(), c->stream);
-        (void)hipEventRecord(s.ev, c->stream);
-        iterations++;
-        launches += it.groups.size();
-        std::vector<Req *> still;
-        for (Req *q : active) {
-            if (q->p.stage >= q->p.nstages) {
-                if (!lens_free.empty()) {
-                    q->len_slot = lens_free.back();
-                    lens_free.pop_back();
-                    (void)hipMemcpyAsync(lens + q->len_slot, q->p.dlen, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                         c->stream);
-                }
-                (void)hipEventCreateWithFlags(&q->last, hipEventDisableTiming);
-                (void)hipEventRecord(q->last, c->stream);
-                free_scratch(q);  // later work is stream-ordered after this product
-                draining.push_back(q);
-            } else {
-                still.push_back(q);
+// Retire products whose last stage has finished.  With nothing left to
+// launch, wait for the oldest one rather than spin.
+void Lane::retire() {
+    if (active.empty() && !draining.empty()) (void)hipEventSynchronize(draining.front()->last);
+    std::vector<Req *> keep;
+    bool freed = false;
+    for (Req *q : draining) {
+        if (hipEventQuery(q->last) != hipSuccess) { keep.push_back(q); continue; }
+        (void)hipEventDestroy(q->last);
+        q->last = nullptr;
+        if (q->uploaded) (void)hipEventDestroy(q->uploaded);
+        q->uploaded = nullptr;
+        s->free_operands(q);
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (q->len_slot >= 0) {
+                q->len = lens[q->len_slot];
+                lens_free.push_back(q->len_slot);
+                q->len_slot = -1;
             }
+            s->bytes_in_flight -= q->bytes;
+            s->n_admitted--;
+            q->t_done = Clock::now();
+            q->done = true;
+            s->completed++;
         }
-        active.swap(still);
+        freed = true;
     }
-
-    // Retire products whose last stage has finished.  With nothing left to
-    // launch, wait for the oldest one rather than spin.
-    void retire() {
-        if (active.empty() && !draining.empty()) (void)hipEventSynchronize(draining.front()->last);
-        std::vector<Req *> keep;
-        for (Req *q : draining) {
-            if (hipEventQuery(q->last) != hipSuccess) { keep.push_back(q); continue; }
-            (void)hipEventDestroy(q->last);
-            q->last = nullptr;
-            free_operands(q);
-            {
-                std::lock_guard<std::mutex> g(mu);
-                if (q->len_slot >= 0) {
-                    q->len = lens[q->len_slot];
-                    lens_free.push_back(q->len_slot);
-                    q->len_slot = -1;
-                }
-                bytes_in_flight -= q->bytes;
-                q->t_done = Clock::now();
-                q->done = true;
-                completed++;
-            }
-            cv_done.notify_all();
-        }
-        draining.swap(keep);
+    draining.swap(keep);
+    if (freed) {
+        s->cv_done.notify_all();
+        s->wake_lanes();
     }
-};
+}
 
 Server *server_of(Ctx *c) { return reinterpret_cast<Server *>(c->server); }
 
@@ -283,9 +456,16 @@ int gnt_serve_start(gnt_ctx *ctx) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c) return gnt::set_error(GNT_E_ARG, "null context");
     if (c->server) return GNT_OK;
-    gnt::Server *s = new gnt::Server(c);
+    gnt::Server *s = new gnt::Server(c, c->serve_split_log_n);
+    for (gnt::Lane &l : s->lanes)
+        if (l.init(s, c) != GNT_OK) {
+            for (gnt::Lane &m : s->lanes) m.destroy();
+            delete s;
+            return gnt::set_error(GNT_E_DEVICE, "serving lane setup failed");
+        }
     c->server = s;
-    s->th = std::thread([s] { s->loop(); });
+    for (gnt::Lane &l : s->lanes) l.th = std::thread([&l] { l.loop(); });
+    s->up_th = std::thread([s] { s->upload_loop(); });
     return GNT_OK;
 }
 
@@ -293,19 +473,21 @@ int gnt_serve_stop(gnt_ctx *ctx) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c || !c->server) return GNT_OK;
     gnt::Server *s = gnt::server_of(c);
+    // pending uploads finish and reach their lanes before the lanes drain
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop_up = true;
+    }
+    s->cv_up.notify_all();
+    s->up_th.join();
     {
         std::lock_guard<std::mutex> g(s->mu);
         s->stop = true;
     }
-    s->cv_work.notify_all();
-    s->th.join();
-    for (auto &slot : s->slots) {
-        if (slot.ev) (void)hipEventDestroy(slot.ev);
-        if (slot.host) (void)hipHostFree(slot.host);
-        if (slot.dev) (void)hipFree(slot.dev);
-    }
+    s->wake_lanes();
+    for (gnt::Lane &l : s->lanes) l.th.join();
+    for (gnt::Lane &l : s->lanes) l.destroy();
     for (auto &kv : s->tickets) delete kv.second;  // unclaimed results stay owned by their gnt_buf
-    if (s->lens) (void)hipHostFree(s->lens);
     delete s;
     c->server = nullptr;
     return GNT_OK;
@@ -352,36 +534,28 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
         return gnt::set_error(GNT_E_NOMEM, "request exceeds the HBM admission cap");
     }
     q->t_submit = Clock::now();
-    // Upload host operands now, on the copy stream, so PCIe transfers overlap
-    // the scheduler's kernels.  They count against the cap once admitted.
-    {
-        std::lock_guard<std::mutex> ug(s->upload_mu);
-        int rc = GNT_OK;
-        if (!req->a_buf) {
-            q->own_a = (u64 *)c->upload_pool.acquire(p.na * 8);
-            if (!q->own_a || c->up.copy(q->own_a, req->a, p.na * 8, c->copy_stream))
-                rc = GNT_E_DEVICE;
-        }
-        if (!rc && !sq && !req->b_buf) {
-            q->own_b = (u64 *)c->upload_pool.acquire(p.nb * 8);
-            if (!q->own_b || c->up.copy(q->own_b, req->b, p.nb * 8, c->copy_stream))
-                rc = GNT_E_DEVICE;
-        }
-        if (!rc && hipStreamSynchronize(c->copy_stream) != hipSuccess) rc = GNT_E_DEVICE;
-        if (rc) {
-            c->upload_pool.release(q->own_a, p.na * 8);
-            c->upload_pool.release(q->own_b, p.nb * 8);
-            delete q;
-            return gnt::set_error(rc, "operand upload failed");
-        }
+    // Device buffers for host operands now (an allocation failure is this
+    // call's error); the uploader thread fills them.  They count against the
+    // cap once admitted.
+    if (!req->a_buf) q->own_a = (u64 *)c->upload_pool.acquire(p.na * 8);
+    if (!sq && !req->b_buf) q->own_b = (u64 *)c->upload_pool.acquire(p.nb * 8);
+    if ((!req->a_buf && !q->own_a) || (!sq && !req->b_buf && !q->own_b)) {
+        s->free_operands(q);
+        delete q;
+        return gnt::set_error(GNT_E_NOMEM, "no device memory for the operands");
     }
+    q->up_total = (q->own_a ? p.na * 8 : 0) + (q->own_b ? p.nb * 8 : 0);
+    gnt::Lane &lane = s->lane_for(p);
+    q->lane = &lane;
     {
         std::lock_guard<std::mutex> g(s->mu);
         q->ticket = s->next_ticket++;
         s->tickets[q->ticket] = q;
-        s->waiting.push_back(q);
+        if (q->up_total) s->uploads.push_back(q);
+        else lane.waiting.push_back(q);
     }
-    s->cv_work.notify_one();
+    if (q->up_total) s->cv_up.notify_one();
+    else lane.cv_work.notify_one();
     *ticket = q->ticket;
     return GNT_OK;
 }
@@ -426,8 +600,9 @@ int gnt_serve_stats(gnt_ctx *ctx, uint64_t *stats, int n) {
     if (!c->server) return gnt::set_error(GNT_E_STATE, "serving loop not running");
     gnt::Server *s = gnt::server_of(c);
     std::lock_guard<std::mutex> g(s->mu);
-    uint64_t v[6] = {s->iterations, s->launches, s->completed, s->peak_active, s->peak_bytes, s->bytes_in_flight};
-    for (int i = 0; i < n && i < 6; i++) stats[i] = v[i];
+    uint64_t v[8] = {s->iterations, s->launches,       s->completed,         s->peak_active,
+                     s->peak_bytes, s->bytes_in_flight, s->lane_iterations[0], s->lane_iterations[1]};
+    for (int i = 0; i < n && i < 8; i++) stats[i] = v[i];
     return GNT_OK;
 }
 

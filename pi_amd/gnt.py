@@ -49,7 +49,7 @@ class GntError(RuntimeError):
 
 class GntConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_cap_bytes", SZ), ("plan_cache_path", ctypes.c_char_p),
-                ("reserved", SZ * 4)]
+                ("serve_split_log_n", SZ), ("reserved", SZ * 3)]
 
 
 class GntReq(ctypes.Structure):
@@ -166,10 +166,12 @@ class DeviceBuf:
 class Context:
     """One device context (gnt_ctx)."""
 
-    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0, plan_cache: Optional[str] = None):
+    def __init__(self, device: int = 0, hbm_cap_bytes: int = 0, plan_cache: Optional[str] = None,
+                 serve_split_log_n: int = 0):
         L = load()
         cfg = GntConfig(device=device, hbm_cap_bytes=hbm_cap_bytes,
-                        plan_cache_path=plan_cache.encode() if plan_cache else None)
+                        plan_cache_path=plan_cache.encode() if plan_cache else None,
+                        serve_split_log_n=serve_split_log_n)
         h = ctypes.c_void_p()
         _check(L.gnt_init(ctypes.byref(h), ctypes.byref(cfg)))
         self.h = h
@@ -352,10 +354,10 @@ class Server:
         return DeviceBuf(self.ctx, h), lat.value
 
     def stats(self):
-        v = (ctypes.c_uint64 * 6)()
-        _check(load().gnt_serve_stats(self.ctx.h, v, 6))
-        return dict(zip(["iterations", "launches", "completed", "peak_admitted", "peak_bytes", "bytes_in_flight"],
-                        list(v)))
+        v = (ctypes.c_uint64 * 8)()
+        _check(load().gnt_serve_stats(self.ctx.h, v, 8))
+        return dict(zip(["iterations", "launches", "completed", "peak_admitted", "peak_bytes", "bytes_in_flight",
+                         "small_lane_iterations", "large_lane_iterations"], list(v)))
 
     def stop(self):
         _check(load().gnt_serve_stop(self.ctx.h))

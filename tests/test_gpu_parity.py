@@ -396,6 +396,50 @@ def test_serving_admission_cap(orc):
         c.close()
 
 
+def test_serving_two_lanes(orc):
+    """Both lanes at once (a split at 2^14 sends larger transforms to the
+    large lane), a small-lane result taken as a large-lane operand, and a
+    cap that makes one lane wait for bytes the other frees: every result
+    exact and the same as alone."""
+    from pi_amd import gnt
+    rng = np.random.default_rng(21)
+    reqs = []
+    for i in range(30):
+        a = orc.splitmix64(7000 + i, int(round(2 ** rng.uniform(3, 13))))
+        b = None if i % 4 == 0 else orc.splitmix64(8000 + i, int(round(2 ** rng.uniform(3, 13))))
+        reqs.append((a, b))
+    for cap in (0, 3 << 20):
+        print(f"cap {cap}", flush=True)
+        c = gnt.Context(0, hbm_cap_bytes=cap, serve_split_log_n=14)
+        srv = gnt.Server(c)
+        try:
+            ts = [srv.submit(a, b, square=b is None) for a, b in reqs]
+            outs = []
+            for t in ts:
+                buf, _ = srv.wait(t)
+                outs.append(buf.download())
+                buf.release()
+            # small lane (N = 2^11) feeding the large lane (N = 2^15)
+            x, y = orc.splitmix64(91, 150), orc.splitmix64(92, 200)
+            z = orc.splitmix64(93, 5000)
+            t = srv.submit(x, y)
+            xy, _ = srv.wait(t)
+            t2 = srv.submit(b=z, a_buf=xy)
+            xyz, _ = srv.wait(t2)
+            assert orc.to_int(xyz.download()) == orc.to_int(x) * orc.to_int(y) * orc.to_int(z)
+            xy.release()
+            xyz.release()
+            st = srv.stats()
+            assert st["small_lane_iterations"] > 0 and st["large_lane_iterations"] > 0, st
+            if cap:
+                assert st["peak_bytes"] <= cap or st["peak_admitted"] == 1, st
+        finally:
+            srv.stop()
+            c.close()
+        for (a, b), got in zip(reqs, outs):
+            assert np.array_equal(got, orc.gmp_sqr(a) if b is None else orc.gmp_mul(a, b))
+
+
 # ---- serving loop: sums and differences (binary splitting's t = x +- y) ----
 
 def _rand_int(rng, n):
