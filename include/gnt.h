@@ -123,12 +123,16 @@ int gnt_batch_info(gnt_batch *b, size_t *launches_per_run, size_t *iterations, s
 int gnt_batch_destroy(gnt_batch *b);
 
 /* ---- serving loop (continuous batching) ----
- * A scheduler thread advances all admitted requests one stage per iteration
- * (one launch per kernel shape per iteration) and admits waiting requests, in
- * arrival order, between iterations while the bytes in flight stay under
- * gnt_config.hbm_cap_bytes.  Host operands must stay valid until the request
- * completes.  An operand may instead be a gnt_buf (e.g. an earlier result),
- * which likewise must not be released before the request completes.
+ * Two lanes (transforms up to 2^gnt_config.serve_split_log_n, and larger),
+ * each a scheduler thread on its own stream, advance all their admitted
+ * requests one stage per iteration (one launch per kernel shape per
+ * iteration) and admit waiting requests, in the order their operands are on
+ * the device, between iterations while the bytes in flight of both lanes stay
+ * under gnt_config.hbm_cap_bytes.  gnt_submit returns once the operands'
+ * device buffers exist; an uploader thread copies host operands, the request
+ * with the fewest bytes left first.  Host operands must stay valid until the
+ * request completes.  An operand may instead be a gnt_buf (e.g. an earlier
+ * result), which likewise must not be released before the request completes.
  * No reference counterpart: the reference has no serving mode. */
 enum {
     GNT_OP_MUL = 0, /* a * b */
