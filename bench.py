@@ -623,7 +623,7 @@ def bench_chudnovsky(args):
         with open(out, "rb") as f:
             for chunk in iter(lambda: f.read(1 << 24), b""):
                 h.update(chunk)
-    merge_s = sum(L["s"] for L in st["levels"])  # the GPU merge levels
+    merge_s = st.get("merge_busy_s", sum(L["s"] for L in st["levels"]))  # GPU merge levels or waves
     line = {
         "metric": f"Chudnovsky pi to {args.digits} digits, wall time",
         "value": round(wall, 3),
@@ -646,6 +646,8 @@ def bench_chudnovsky(args):
         "in_process_s": st["t_total"],
         "phases_s": {k: st[k] for k in ("t_init", "t_host_bs", "t_upload", "t_merge", "t_download",
                                          "t_final_assembly", "t_decimal", "t_write")},
+        "merge_order": st.get("merge_order"),
+        "merge_busy_s": st.get("merge_busy_s"),
         "gpu_merge_share": round(merge_s / wall, 4),
         "levels": st["levels"],
         "decimal_levels": st.get("decimal_levels"),

@@ -27,12 +27,14 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -94,16 +96,19 @@ struct Node {
     uint64_t a = 0, b = 0;
     int depth = 0;
     int left = -1, right = -1;  // children of a merged node
+    int parent = -1;
     bool host = true;           // computed whole on the host
     std::unique_ptr<Mpz> P, Q, T;
     gnt_buf *bp = nullptr, *bq = nullptr, *bt = nullptr;
 };
 
 struct LevelStat {
-    int depth;
+    int depth;     // a merge wave: the smallest depth in it
     size_t nodes, products;
     double limbs;  // operand limbs (na + nb) summed over the level's products
     double secs;
+    int depth_max = -1;  // a merge wave: the largest depth in it (-1: one tree level)
+    int wave = -1;
 };
 
 struct Options {
@@ -117,6 +122,8 @@ struct Options {
     int device = 0;
     std::string plans;            // plan cache from the autotuner
     std::string algorithm = "chudnovsky";
+    bool merge_levels = false;    // merge a whole tree level at a time after all host nodes
+                                  // (comparison run); default: merge nodes as their children finish
     bool shutdown = false;        // free the context's device memory before exiting
 };
 
@@ -145,6 +152,8 @@ struct Driver {
             int r = build(m, b, depth + 1);
             nodes[id].left = l;
             nodes[id].right = r;
+            nodes[l].parent = id;
+            nodes[r].parent = id;
         } else {
             host_nodes.push_back(id);
         }
@@ -154,9 +163,12 @@ struct Driver {
     // Host nodes on `threads` threads; with upload set, each node's P, |T|, Q
     // go to the device as soon as it is done (overlapping the other nodes'
     // compute) and the host copies are dropped.
-    void run_host_nodes(int threads, bool upload_done = false) {
+    void run_host_nodes(int threads, bool upload_done = false, bool left_to_right = false) {
         std::vector<int> order = host_nodes;
-        std::sort(order.begin(), order.end(), [&](int x, int y) { return nodes[x].b - nodes[x].a > nodes[y].b - nodes[y].a; });
+        if (left_to_right)  // siblings finish together, so their parent can merge early
+            std::sort(order.begin(), order.end(), [&](int x, int y) { return nodes[x].a < nodes[y].a; });
+        else
+            std::sort(order.begin(), order.end(), [&](int x, int y) { return nodes[x].b - nodes[x].a > nodes[y].b - nodes[y].a; });
         std::atomic<size_t> next{0};
         auto work = [&] {
             for (;;) {
@@ -175,6 +187,7 @@ struct Driver {
                     n.P.reset();
                     n.Q.reset();
                     n.T.reset();
+                    mark_available(order[i]);
                 }
             }
         };
@@ -219,9 +232,18 @@ struct Driver {
             if (!nodes[i].host && nodes[i].depth == depth) ids.push_back(i);
         if (ids.empty()) return;
         auto t0 = Clock::now();
+        LevelStat st{depth, ids.size(), 0, 0.0, 0.0};
+        merge_nodes_device(ids, st);
+        st.secs = since(t0);
+        levels.push_back(st);
+    }
+
+    // Merge the given device nodes, whose children are all on the device: all
+    // their products submitted together (continuous batching groups them),
+    // each t = x +- y as soon as its two products finish.
+    void merge_nodes_device(const std::vector<int> &ids, LevelStat &st) {
         struct Tk { uint64_t x, y, q, p, t; };
         std::vector<Tk> tk(ids.size());
-        LevelStat st{depth, ids.size(), 0, 0.0, 0.0};
         for (size_t i = 0; i < ids.size(); i++) {
             Node &n = nodes[ids[i]];
             Node &L = nodes[n.left], &R = nodes[n.right];
@@ -254,11 +276,77 @@ struct Driver {
                 ch.bp = ch.bq = ch.bt = nullptr;
             }
         }
-        st.secs = since(t0);
-        levels.push_back(st);
     }
 
-    // Host merge of the same level (comparison run).
+    // ---- merges as children finish (overlapping the host nodes) ----
+    std::mutex avail_mu;
+    std::condition_variable avail_cv;
+    std::vector<int> newly_available;  // nodes whose P, Q, |T| just reached the device
+
+    void mark_available(int id) {
+        {
+            std::lock_guard<std::mutex> g(avail_mu);
+            newly_available.push_back(id);
+        }
+        avail_cv.notify_one();
+    }
+
+    // Host nodes on `threads` threads, left to right, while this thread merges
+    // every device node whose two children are on the device, in waves of
+    // whatever is ready; returns when the root is merged.  *t_host receives
+    // the time to the last host node, *t_tail the merging left after it.
+    void run_dataflow(int threads, double *t_host, double *t_tail) {
+        auto t0 = Clock::now();
+        std::atomic<bool> host_done{false};
+        Clock::time_point t_host_end;
+        std::thread hosts([&] {
+            run_host_nodes(threads, true, true);
+            t_host_end = Clock::now();
+            host_done = true;
+        });
+        std::vector<char> avail(nodes.size(), 0), queued(nodes.size(), 0);
+        size_t left = 0;
+        for (const Node &n : nodes) left += n.host ? 0 : 1;
+        int wave = 0;
+        while (left > 0) {
+            std::vector<int> ready;
+            {
+                std::unique_lock<std::mutex> g(avail_mu);
+                avail_cv.wait(g, [&] { return !newly_available.empty(); });
+                for (int id : newly_available) {
+                    avail[id] = 1;
+                    const int p = nodes[id].parent;
+                    if (p >= 0 && !queued[p] && avail[nodes[p].left] && avail[nodes[p].right]) {
+                        queued[p] = 1;
+                        ready.push_back(p);
+                    }
+                }
+                newly_available.clear();
+            }
+            if (ready.empty()) continue;
+            auto tw = Clock::now();
+            int dmin = 1 << 30, dmax = -1;
+            for (int id : ready) {
+                dmin = std::min(dmin, nodes[id].depth);
+                dmax = std::max(dmax, nodes[id].depth);
+            }
+            LevelStat st{dmin, ready.size(), 0, 0.0, 0.0};
+            merge_nodes_device(ready, st);
+            st.secs = since(tw);
+            st.depth_max = dmax;
+            st.wave = wave++;
+            levels.push_back(st);
+            left -= ready.size();
+            {
+                std::lock_guard<std::mutex> g(avail_mu);
+                for (int id : ready) newly_available.push_back(id);
+            }
+        }
+        hosts.join();
+        *t_host = std::chrono::duration<double>(t_host_end - t0).count();
+        *t_tail = std::chrono::duration<double>(Clock::now() - t_host_end).count();
+    }
+
     void merge_level_host(int depth) {
         std::vector<int> ids;
         for (int i = 0; i < (int)nodes.size(); i++)
@@ -777,7 +865,7 @@ std::string final_assembly_gmp(const Mpz &Q, const Mpz &Tabs, uint64_t digits, u
 void usage() {
     std::fprintf(stderr,
                  "usage: pi_compute DIGITS OUT_FILE [--algorithm chudnovsky|gauss-legendre] [--gpu-min-limbs L]\n"
-                 "                  [--threads T] [--host-merge] [--shutdown]\n"
+                 "                  [--threads T] [--host-merge] [--merge dataflow|levels] [--shutdown]\n"
                  "                     [--fa-gmp] [--fa-min-limbs L] [--device D] [--plans CACHE.json]\n");
     std::exit(2);
 }
@@ -856,6 +944,11 @@ int main(int argc, char **argv) {
         else if (a == "--plans" && i + 1 < argc) o.plans = argv[++i];
         else if (a == "--algorithm" && i + 1 < argc) o.algorithm = argv[++i];
         else if (a == "--shutdown") o.shutdown = true;
+        else if (a == "--merge" && i + 1 < argc) {
+            const std::string m = argv[++i];
+            if (m == "levels") o.merge_levels = true;
+            else if (m != "dataflow") usage();
+        }
         else usage();
     }
     if (o.threads <= 0) {
@@ -895,11 +988,17 @@ int main(int argc, char **argv) {
         t_init = since(t0);
     }
     auto t0 = Clock::now();
-    d.run_host_nodes(o.threads, gpu_merge);  // merged on the device: uploads overlap the host work
-    const double t_host = since(t0);
+    double t_host = 0, t_merge = 0;
+    const bool dataflow = gpu_merge && !o.merge_levels;
+    if (dataflow) {
+        // merges as children finish; t_merge is what is left after the host nodes
+        d.run_dataflow(o.threads, &t_host, &t_merge);
+    } else {
+        d.run_host_nodes(o.threads, gpu_merge);  // merged on the device: uploads overlap the host work
+        t_host = since(t0);
+    }
 
     Mpz Q, Tabs;
-    double t_merge = 0;
     if (!any_merge) {
         __gmpz_set(&Q, &*d.nodes[0].Q);
         __gmpz_neg(&Tabs, &*d.nodes[0].T);
@@ -910,9 +1009,11 @@ int main(int argc, char **argv) {
         __gmpz_set(&Q, &*d.nodes[0].Q);
         __gmpz_neg(&Tabs, &*d.nodes[0].T);
     } else {
-        t0 = Clock::now();
-        for (int depth = d.max_depth; depth >= 0; depth--) d.merge_level_device(depth);
-        t_merge = since(t0);
+        if (!dataflow) {
+            t0 = Clock::now();
+            for (int depth = d.max_depth; depth >= 0; depth--) d.merge_level_device(depth);
+            t_merge = since(t0);
+        }
         t0 = Clock::now();
         Node &root = d.nodes[0];
         size_t nq = Driver::buf_len(d.ctx, root.bq), nt = Driver::buf_len(d.ctx, root.bt);
@@ -957,23 +1058,30 @@ int main(int argc, char **argv) {
         const LevelStat &L = d.levels[i];
         t_levels += L.secs;
         char buf[256];
-        std::snprintf(buf, sizeof buf, "%s{\"depth\": %d, \"nodes\": %zu, \"products\": %zu, \"operand_limbs\": %.0f, \"s\": %.4f}",
-                      i ? ", " : "", L.depth, L.nodes, L.products, L.limbs, L.secs);
+        if (L.wave >= 0)  // a merge wave: every node whose children were ready, of depths [depth, depth_max]
+            std::snprintf(buf, sizeof buf,
+                          "%s{\"wave\": %d, \"depth\": %d, \"depth_max\": %d, \"nodes\": %zu, \"products\": %zu, "
+                          "\"operand_limbs\": %.0f, \"s\": %.4f}",
+                          i ? ", " : "", L.wave, L.depth, L.depth_max, L.nodes, L.products, L.limbs, L.secs);
+        else
+            std::snprintf(buf, sizeof buf, "%s{\"depth\": %d, \"nodes\": %zu, \"products\": %zu, \"operand_limbs\": %.0f, \"s\": %.4f}",
+                          i ? ", " : "", L.depth, L.nodes, L.products, L.limbs, L.secs);
         lv += buf;
     }
     lv += "]";
-    std::printf("{\"digits\": %llu, \"n_terms\": %llu, \"prec_bits\": %llu, \"merge\": \"%s\", \"threads\": %d, "
+    std::printf("{\"digits\": %llu, \"n_terms\": %llu, \"prec_bits\": %llu, \"merge\": \"%s\", \"merge_order\": \"%s\", "
+                "\"merge_busy_s\": %.3f, \"threads\": %d, "
                 "\"gpu_min_limbs\": %.0f, \"fa_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
                 "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
                 "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_shutdown\": %.3f, \"t_total\": %.3f, "
                 "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"fa\": \"%s\", \"fa_gpu_products\": %zu, "
                 "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"decimal_levels\": %s, \"levels\": %s}\n",
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
-                !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"), o.threads, o.gpu_min_limbs,
+                !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"),
+                dataflow ? "dataflow" : "levels", t_levels, o.threads, o.gpu_min_limbs,
                 o.fa_min_limbs, d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
                 t_download, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
                 M.host_products, M.gpu_limbs, g_decimal_stats.c_str(), lv.c_str());
-    (void)t_levels;
     return finish(o);
 }

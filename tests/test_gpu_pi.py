@@ -43,11 +43,16 @@ def test_pi_10000_gpu_merges(tmp_path, gpu_min_limbs):
         assert out.read_bytes() == f.read()
 
 
-@pytest.mark.parametrize("digits,gpu_min_limbs", [(100000, "32"), (1000000, "256"), (10000000, "1024")])
-def test_pi_sha256(tmp_path, digits, gpu_min_limbs):
+@pytest.mark.parametrize("digits,gpu_min_limbs,order", [
+    (100000, "32", "dataflow"), (1000000, "256", "dataflow"), (10000000, "1024", "dataflow"),
+    (100000, "32", "levels"), (1000000, "256", "levels")])
+def test_pi_sha256(tmp_path, digits, gpu_min_limbs, order):
+    """Merges as children finish (dataflow, waves of mixed depths overlapping
+    the host nodes) or a whole tree level at a time: the same digits."""
     gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
-    out, st = _run(digits, tmp_path, "--gpu-min-limbs", gpu_min_limbs)
-    assert st["merge"] == "gpu" and st["merge_nodes"] > 0
+    out, st = _run(digits, tmp_path, "--gpu-min-limbs", gpu_min_limbs, "--merge", order)
+    assert st["merge"] == "gpu" and st["merge_nodes"] > 0 and st["merge_order"] == order
+    assert sum(L["nodes"] for L in st["levels"]) == st["merge_nodes"]
     assert os.path.getsize(out) == gold[str(digits)]["bytes"]
     assert _digest(out) == gold[str(digits)]["sha256"]
 
