@@ -644,8 +644,8 @@ def bench_chudnovsky(args):
         "digits_sha256": h.hexdigest(),
         "byte_identical_to_pinned": (h.hexdigest() == gold["sha256"]) if gold else None,
         "in_process_s": st["t_total"],
-        "phases_s": {k: st[k] for k in ("t_init", "t_host_bs", "t_upload", "t_merge", "t_download",
-                                         "t_final_assembly", "t_decimal", "t_write")},
+        "phases_s": {k: st[k] for k in ("t_init", "t_host_bs", "t_upload", "t_merge", "t_download", "t_serve_stop",
+                                         "t_final_assembly", "t_decimal", "t_write", "t_shutdown") if k in st},
         "merge_order": st.get("merge_order"),
         "merge_busy_s": st.get("merge_busy_s"),
         "gpu_merge_share": round(merge_s / wall, 4),

@@ -160,6 +160,25 @@ void Pool::release_all() {
     slab_left_ = 0;
 }
 
+// Take over everything `from` holds (free lists and slabs); nothing of
+// `from` may be in use or have work pending on it.
+void Pool::absorb(Pool &from) {
+    if (&from == this) return;
+    std::scoped_lock g(mu, from.mu);
+    for (auto &kv : from.free_) {
+        auto &v = free_[kv.first];
+        v.insert(v.end(), kv.second.begin(), kv.second.end());
+    }
+    slabs_.insert(slabs_.end(), from.slabs_.begin(), from.slabs_.end());
+    live_bytes += from.live_bytes;
+    pooled_bytes += from.pooled_bytes;
+    from.free_.clear();
+    from.slabs_.clear();
+    from.live_bytes = from.pooled_bytes = 0;
+    from.slab_cur_ = nullptr;
+    from.slab_left_ = 0;
+}
+
 int Ctx::init_tables() {
     std::vector<u64> lo, hi, sm;
     if (!build_root_tables(lo, hi, sm)) return set_error(GNT_E_DEVICE, "root table construction failed");

@@ -33,6 +33,7 @@ public:
     void release(void *p, size_t bytes);
     void trim();         // give idle large buffers back to the device
     void release_all();  // everything, slabs included (nothing may be in use)
+    void absorb(Pool &from);  // take over from's buffers and slabs (nothing of from in use)
     size_t live_bytes = 0;    // allocated from the device (pooled or in use)
     size_t pooled_bytes = 0;  // of which idle in the free lists
 private:

@@ -245,7 +245,9 @@ void Lane::destroy() {
     lens = nullptr;
     if (ctx_ev) (void)hipEventDestroy(ctx_ev);
     ctx_ev = nullptr;
-    scratch.release_all();
+    // the idle scratch serves the context's later products (the stream is
+    // drained, so no work of this lane still uses it)
+    c->pool.absorb(scratch);
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
 }

@@ -1030,9 +1030,12 @@ int main(int argc, char **argv) {
     }
 
     uint64_t sstats[6] = {0, 0, 0, 0, 0, 0};
+    double t_serve_stop = 0;
     if (gpu_merge) {
+        t0 = Clock::now();
         gnt_serve_stats(d.ctx, sstats, 6);
         gnt_serve_stop(d.ctx);
+        t_serve_stop = since(t0);
     }
     double t_fa = 0, t_dec = 0;
     Mul M;
@@ -1073,14 +1076,15 @@ int main(int argc, char **argv) {
                 "\"merge_busy_s\": %.3f, \"threads\": %d, "
                 "\"gpu_min_limbs\": %.0f, \"fa_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
                 "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
-                "\"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, \"t_shutdown\": %.3f, \"t_total\": %.3f, "
+                "\"t_serve_stop\": %.3f, \"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, "
+                "\"t_shutdown\": %.3f, \"t_total\": %.3f, "
                 "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"fa\": \"%s\", \"fa_gpu_products\": %zu, "
                 "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"decimal_levels\": %s, \"levels\": %s}\n",
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
                 !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"),
                 dataflow ? "dataflow" : "levels", t_levels, o.threads, o.gpu_min_limbs,
                 o.fa_min_limbs, d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
-                t_download, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
+                t_download, t_serve_stop, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
                 M.host_products, M.gpu_limbs, g_decimal_stats.c_str(), lv.c_str());
     return finish(o);
