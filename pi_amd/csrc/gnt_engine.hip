@@ -161,6 +161,13 @@ __global__ __launch_bounds__(CARRY_RIPPLE_BLOCK) void k_cripple(const CItem *__r
     }
 }
 
+// After the ripple: each finished item's trimmed length to its pinned host
+// word, one thread per item (replaces a 4-byte copy per product).
+__global__ __launch_bounds__(256) void k_lens(const CItem *__restrict__ items, int n_items) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_items && items[i].len_host) *items[i].len_host = *items[i].out_len;
+}
+
 }  // namespace
 
 size_t Prod::cs_bytes() const { return (2 * nt + 15) / 16 * 16 + 4 * nt; }
@@ -239,6 +246,10 @@ void add_stage(Iteration &it, Prod &p) {
         c.n_limbs = p.nl;
         c.ntiles = p.nt;
         c.tile0 = (uint64_t)s;  // the carry step, resolved in finalize_iteration
+        if (s == 2 && p.len_host) {
+            c.len_host = p.len_host;
+            it.want_lens = true;
+        }
         it.citems.push_back(c);
         return;
     }
@@ -298,6 +309,10 @@ void add_stage(Iteration &it, Prod &p) {
     c.n_limbs = p.nl;
     c.ntiles = p.nt;
     c.tile0 = (uint64_t)cstep;  // the step, resolved in finalize_iteration
+    if (cstep == 2 && p.len_host) {
+        c.len_host = p.len_host;
+        it.want_lens = true;
+    }
     it.citems.push_back(c);
 }
 
@@ -378,7 +393,11 @@ int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c
             dim3 grid((unsigned)std::min<uint64_t>(g.tiles, 65536));
             if (g.kind == 10) hipLaunchKernelGGL(k_ctiles, grid, dim3(CARRY_BLOCK), 0, st, items, n, g.tiles);
             else if (g.kind == 11) hipLaunchKernelGGL(k_cscan, dim3((unsigned)n), dim3(SCAN_BLOCK), 0, st, items);
-            else hipLaunchKernelGGL(k_cripple, grid, dim3(CARRY_RIPPLE_BLOCK), 0, st, items, n, g.tiles);
+            else {
+                hipLaunchKernelGGL(k_cripple, grid, dim3(CARRY_RIPPLE_BLOCK), 0, st, items, n, g.tiles);
+                if (it.want_lens)
+                    hipLaunchKernelGGL(k_lens, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, items, n);
+            }
         }
         if (rc) return rc;
     }

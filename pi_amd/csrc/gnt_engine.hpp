@@ -69,6 +69,7 @@ struct CItem {
     unsigned char *gp, *cin;  // per carry tile
     uint32_t *hi;             // per carry tile: highest nonzero limb + 1
     uint32_t *out_len;
+    uint32_t *len_host;       // last carry step: copy *out_len here too (pinned host word), or null
     uint64_t n_limbs;
     uint64_t ntiles;
     uint64_t tile0;
@@ -93,6 +94,7 @@ struct Prod {
     int stage = 0;
     int nstages = 0;
     int op = OP_MUL;  // OP_ADD / OP_SUB: carry stages only, no transform
+    uint32_t *len_host = nullptr;  // pinned word that receives the trimmed length (or null)
     bool sqr() const { return db == nullptr; }
     bool linear() const { return op == OP_ADD || op == OP_SUB; }
     size_t scratch_bytes() const;            // xa + xb + lh + cs
@@ -115,6 +117,7 @@ struct Iteration {
     std::vector<TItem> titems;
     std::vector<CItem> citems;
     std::vector<Group> groups;
+    bool want_lens = false;  // some last carry step has a len_host: copy after the ripple
 };
 
 // Append the stage p.stage of product p to `it` and advance p.stage.

@@ -55,7 +55,6 @@ struct Req {
     size_t bytes = 0;                        // footprint counted against the cap
     bool done = false;
     int status = GNT_OK;
-    hipEvent_t last = nullptr;               // recorded after the product's last stage
     int len_slot = -1;                       // pinned word the trimmed length is copied to
     int64_t len = -1;                        // trimmed result length once known
     size_t up_total = 0, up_done = 0;        // host operand bytes to upload, uploaded so far
@@ -90,7 +89,14 @@ struct Lane {
     std::condition_variable cv_work;
     std::deque<Req *> waiting;   // guarded by Server::mu
     std::vector<Req *> active;   // admitted, stages still to enqueue
-    std::vector<Req *> draining; // all stages enqueued, waiting for completion
+    // Products whose last stage went out in one iteration, with the event
+    // recorded after it; retired in stream order.
+    struct Done {
+        hipEvent_t ev;
+        std::vector<Req *> reqs;
+    };
+    std::deque<Done> draining;
+    std::vector<hipEvent_t> free_events;
     static constexpr int NSLOT = 2;
     Slot slots[NSLOT];
     int slot_next = 0;
@@ -243,6 +249,8 @@ void Lane::destroy() {
     }
     if (lens) (void)hipHostFree(lens);
     lens = nullptr;
+    for (hipEvent_t e : free_events) (void)hipEventDestroy(e);
+    free_events.clear();
     if (ctx_ev) (void)hipEventDestroy(ctx_ev);
     ctx_ev = nullptr;
     // the idle scratch serves the context's later products (the stream is
@@ -379,6 +387,20 @@ void Lane::loop() {
 }
 
 void Lane::run_iteration() {
+    // products that finish in this iteration write their trimmed length to a
+    // pinned word from the device (one small kernel after the carry)
+    std::vector<Req *> finishing;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (Req *q : active)
+            if (q->p.stage == q->p.nstages - 1) {
+                finishing.push_back(q);
+                if (lens_free.empty()) continue;  // length found later by a device read
+                q->len_slot = lens_free.back();
+                lens_free.pop_back();
+                q->p.len_host = lens + q->len_slot;
+            }
+    }
     Iteration it;
     for (Req *q : active) add_stage(it, q->p);
     finalize_iteration(it);
@@ -390,58 +412,63 @@ void Lane::run_iteration() {
     (void)hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, stream);
     launch_iteration(it, (const TItem *)sl.dev, (const CItem *)((unsigned char *)sl.dev + tb), c->tw(), stream);
     (void)hipEventRecord(sl.ev, stream);
+    if (!finishing.empty()) {
+        // one event marks every product that finishes in this iteration
+        Done d;
+        if (!free_events.empty()) {
+            d.ev = free_events.back();
+            free_events.pop_back();
+        } else {
+            (void)hipEventCreateWithFlags(&d.ev, hipEventDisableTiming);
+        }
+        (void)hipEventRecord(d.ev, stream);
+        for (Req *q : finishing) free_scratch(q);  // later work on this lane is stream-ordered after it
+        d.reqs = std::move(finishing);
+        draining.push_back(std::move(d));
+    }
     std::vector<Req *> still;
+    for (Req *q : active)
+        if (q->p.stage < q->p.nstages) still.push_back(q);
+    active.swap(still);
     std::lock_guard<std::mutex> g(s->mu);
     s->iterations++;
     s->lane_iterations[this == &s->lanes[0] ? 0 : 1]++;
     s->launches += it.groups.size();
-    for (Req *q : active) {
-        if (q->p.stage >= q->p.nstages) {
-            if (!lens_free.empty()) {
-                q->len_slot = lens_free.back();
-                lens_free.pop_back();
-                (void)hipMemcpyAsync(lens + q->len_slot, q->p.dlen, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-            }
-            (void)hipEventCreateWithFlags(&q->last, hipEventDisableTiming);
-            (void)hipEventRecord(q->last, stream);
-            free_scratch(q);  // later work on this lane is stream-ordered after this product
-            draining.push_back(q);
-        } else {
-            still.push_back(q);
-        }
-    }
-    active.swap(still);
 }
 
-// Retire products whose last stage has finished.  With nothing left to
-// launch, wait for the oldest one rather than spin.
+// Retire the iterations whose products have finished, oldest first (their
+// events complete in stream order).  With nothing left to launch, wait for
+// the oldest one rather than spin.
 void Lane::retire() {
-    if (active.empty() && !draining.empty()) (void)hipEventSynchronize(draining.front()->last);
-    std::vector<Req *> keep;
+    if (active.empty() && !draining.empty()) (void)hipEventSynchronize(draining.front().ev);
     bool freed = false;
-    for (Req *q : draining) {
-        if (hipEventQuery(q->last) != hipSuccess) { keep.push_back(q); continue; }
-        (void)hipEventDestroy(q->last);
-        q->last = nullptr;
-        if (q->uploaded) (void)hipEventDestroy(q->uploaded);
-        q->uploaded = nullptr;
-        s->free_operands(q);
+    while (!draining.empty() && hipEventQuery(draining.front().ev) == hipSuccess) {
+        Done d = std::move(draining.front());
+        draining.pop_front();
+        free_events.push_back(d.ev);
+        for (Req *q : d.reqs) {
+            if (q->uploaded) (void)hipEventDestroy(q->uploaded);
+            q->uploaded = nullptr;
+            s->free_operands(q);
+        }
         {
             std::lock_guard<std::mutex> g(s->mu);
-            if (q->len_slot >= 0) {
-                q->len = lens[q->len_slot];
-                lens_free.push_back(q->len_slot);
-                q->len_slot = -1;
+            const auto now = Clock::now();
+            for (Req *q : d.reqs) {
+                if (q->len_slot >= 0) {
+                    q->len = lens[q->len_slot];
+                    lens_free.push_back(q->len_slot);
+                    q->len_slot = -1;
+                }
+                s->bytes_in_flight -= q->bytes;
+                s->n_admitted--;
+                q->t_done = now;
+                q->done = true;
+                s->completed++;
             }
-            s->bytes_in_flight -= q->bytes;
-            s->n_admitted--;
-            q->t_done = Clock::now();
-            q->done = true;
-            s->completed++;
         }
         freed = true;
     }
-    draining.swap(keep);
     if (freed) {
         s->cv_done.notify_all();
         s->wake_lanes();
