@@ -114,7 +114,7 @@ struct LevelStat {
 struct Options {
     uint64_t digits = 0;
     std::string out;
-    double gpu_min_limbs = 2048;  // a node merges on the GPU when both children's P reach this
+    double gpu_min_limbs = 1024;  // a node merges on the GPU when both children's P reach this
     int threads = 0;
     bool host_merge = false;      // merge the upper tree on the host too (comparison run)
     bool fa_gmp = false;          // final assembly and radix conversion by GMP (comparison run)
