@@ -41,6 +41,7 @@
 
 #include "../../include/gnt.h"
 #include "gmp_abi.hpp"
+#include "par_mpz.hpp"
 
 namespace {
 
@@ -122,6 +123,7 @@ struct Options {
     int device = 0;
     std::string plans;            // plan cache from the autotuner
     std::string algorithm = "chudnovsky";
+    double par_min_limbs = 1 << 17;  // host bignum steps multi-threaded from this many limbs
     bool merge_levels = false;    // merge a whole tree level at a time after all host nodes
                                   // (comparison run); default: merge nodes as their children finish
     bool shutdown = false;        // free the context's device memory before exiting
@@ -387,15 +389,20 @@ struct Driver {
 template <class F>
 void parallel_for(size_t n, int threads, F f) {
     std::atomic<size_t> next{0};
+    const int workers = (int)std::max<size_t>(1, std::min<size_t>((size_t)threads, n));
+    const int share = std::max(1, par::threads_here() / workers);  // helper threads per worker
     auto w = [&] {
+        const int saved = par::tl_threads;
+        par::tl_threads = share;
         for (;;) {
             size_t i = next.fetch_add(1);
-            if (i >= n) return;
+            if (i >= n) break;
             f(i);
         }
+        par::tl_threads = saved;
     };
     std::vector<std::thread> ts;
-    for (int t = 1; t < threads && (size_t)t < n; t++) ts.emplace_back(w);
+    for (int t = 1; t < workers; t++) ts.emplace_back(w);
     w();
     for (auto &t : ts) t.join();
 }
@@ -408,6 +415,8 @@ struct Mul {
     int threads = 1;
     size_t gpu_products = 0, host_products = 0;
     double gpu_limbs = 0;
+    double t_gpu = 0;     // seconds inside GPU products (host buffers: copies included)
+    double t_gpu_fa = 0;  // of which in the final assembly
 
     static size_t nl(const gmp_mpz *x) { return (size_t)(x->size < 0 ? -x->size : x->size); }
     bool on_gpu(size_t na, size_t nb) const { return ctx && (double)std::min(na, nb) >= min_limbs; }
@@ -424,8 +433,10 @@ struct Mul {
         Mpz t;
         __gmpz_realloc2(&t, (unsigned long)(na + nb) * 64);
         size_t len = 0;
+        const auto t0 = Clock::now();
         if (a == b) check(gnt_sqr(ctx, a->d, na, t.z.d, na + nb, &len), "square");
         else check(gnt_mul(ctx, a->d, na, b->d, nb, t.z.d, na + nb, &len), "multiply");
+        t_gpu += since(t0);
         t.z.size = neg ? -(int)len : (int)len;
         __gmpz_swap(r, &t);
         gpu_products++;
@@ -452,7 +463,9 @@ struct Mul {
         std::thread host([&] { parallel_for(h.size(), std::max(1, threads - 1), [&](size_t k) {
                                    __gmpz_mul(jobs[h[k]].r, jobs[h[k]].a, jobs[h[k]].b);
                                }); });
+        const auto t0 = Clock::now();
         if (!gj.empty()) check(gnt_mul_batch(ctx, gj.data(), gj.size()), "batched multiply");
+        t_gpu += since(t0);
         host.join();
         for (size_t k = 0; k < g.size(); k++) jobs[g[k]].r->size = (int)gj[k].out_len;
         gpu_products += g.size();
@@ -464,8 +477,7 @@ unsigned long bits_of(const gmp_mpz *x) { return x->size ? (unsigned long)__gmpz
 
 // r = x * 2^s (s may be negative: truncation toward zero)
 void shift(gmp_mpz *r, const gmp_mpz *x, long s) {
-    if (s >= 0) __gmpz_mul_2exp(r, x, (unsigned long)s);
-    else __gmpz_tdiv_q_2exp(r, x, (unsigned long)(-s));
+    par::shift(r, x, s);  // multi-threaded on large numbers
 }
 
 // Newton precisions from k down to a start the hardware double gives exactly
@@ -499,16 +511,15 @@ void reciprocal(Mul &M, const gmp_mpz *D, unsigned long k, gmp_mpz *R) {
         const unsigned long q = ps[i];  // next precision
         shift(&dt, D, (long)(q + g) - (long)n);
         M.mul(&t, &dt, R);                              // ~ 2^(q+g+p)
-        __gmpz_set_ui(&e, 1);
-        __gmpz_mul_2exp(&e, &e, q + g + p);
-        __gmpz_sub(&e, &e, &t);                         // ~ err 2^(q+g), signed
+        par::pow2(&e, q + g + p);
+        par::sub(&e, &e, &t);                           // ~ err 2^(q+g), signed
         const long eb = (long)bits_of(&e);
         const long drop = std::max(0L, eb - (long)(q - p + 48));
         shift(&e, &e, -drop);
         M.mul(&c, R, &e);
         shift(&c, &c, drop - (long)(2 * p + g));        // r e / 2^(2p+g)
         shift(R, R, (long)(q - p));
-        __gmpz_add(R, R, &c);
+        par::add(R, R, &c);
         p = q;
     }
 }
@@ -522,17 +533,16 @@ void inv_sqrt(Mul &M, unsigned long c, unsigned long k, gmp_mpz *Y) {
     for (size_t i = ps.size() - 1; i-- > 0;) {
         const unsigned long q = ps[i];
         M.mul(&s, Y, Y);                                // ~ 2^(2p) / c
-        __gmpz_mul_ui(&s, &s, c);
-        __gmpz_set_ui(&e, 1);
-        __gmpz_mul_2exp(&e, &e, 2 * p);
-        __gmpz_sub(&e, &e, &s);                         // ~ err 2^p, signed
+        par::mul_ui(&s, &s, c);
+        par::pow2(&e, 2 * p);
+        par::sub(&e, &e, &s);                           // ~ err 2^p, signed
         const long eb = (long)bits_of(&e);
         const long drop = std::max(0L, eb - (long)(q - p + 48));
         shift(&e, &e, -drop);
         M.mul(&corr, Y, &e);
         shift(&corr, &corr, drop - (long)(3 * p + 1 - q));
         shift(Y, Y, (long)(q - p));
-        __gmpz_add(Y, Y, &corr);
+        par::add(Y, Y, &corr);
         p = q;
     }
 }
@@ -542,8 +552,8 @@ void inv_sqrt(Mul &M, unsigned long c, unsigned long k, gmp_mpz *Y) {
 void assemble_pi(Mul &M, const Mpz &Q, const Mpz &Tabs, unsigned long prec, gmp_mpz *pi_fix) {
     const unsigned long w = prec + 64;  // working precision
     Mpz den, q;
-    __gmpz_mul_ui(&den, &Q, CH_A);
-    __gmpz_sub(&den, &den, &Tabs);
+    par::mul_ui(&den, &Q, CH_A);
+    par::sub(&den, &den, &Tabs);
     __gmpz_set(&q, &Q);
     const long s = (long)bits_of(&den) - (long)(w + 64);
     if (s > 0) {  // Q / den needs w + 64 bits of each
@@ -556,9 +566,9 @@ void assemble_pi(Mul &M, const Mpz &Q, const Mpz &Tabs, unsigned long prec, gmp_
     inv_sqrt(M, 10005, k, &Y);   // ~ 2^k / sqrt(10005)
     M.mul(&qf, &q, &R);
     shift(&qf, &qf, (long)w - (long)(n + k));  // ~ Q/den 2^w
-    __gmpz_mul_ui(&Y, &Y, 10005);              // ~ sqrt(10005) 2^k
+    par::mul_ui(&Y, &Y, 10005);                // ~ sqrt(10005) 2^k
     M.mul(pi_fix, &Y, &qf);
-    __gmpz_mul_ui(pi_fix, pi_fix, 426880);
+    par::mul_ui(pi_fix, pi_fix, 426880);
     shift(pi_fix, pi_fix, -(long)(k + w - prec));
 }
 
@@ -605,11 +615,10 @@ struct Decimal {
     // x in [E 10^m, 2^F - E 10^m]: the digits cannot move under the error E.
     bool decided(const gmp_mpz *rem, unsigned long F, unsigned err, const gmp_mpz *p10m) {
         Mpz b, top;
-        __gmpz_mul_ui(&b, p10m, err + 1);
+        par::mul_ui(&b, p10m, err + 1);
         if (__gmpz_cmp(rem, &b) < 0) return false;
-        __gmpz_set_ui(&top, 1);
-        __gmpz_mul_2exp(&top, &top, F);
-        __gmpz_sub(&top, &top, &b);
+        par::pow2(&top, F);
+        par::sub(&top, &top, &b);
         return __gmpz_cmp(rem, &top) <= 0;
     }
 
@@ -669,7 +678,7 @@ struct Decimal {
                 hi.err = x.err + 1;
                 // low digits: fraction part of f 10^h
                 Mpz rem;
-                __gmpz_tdiv_r_2exp(&rem, &g[k], x.F);
+                par::trunc(&rem, &g[k], x.F);
                 if (!decided(&rem, x.F, x.err, p10(h))) die("decimal conversion undecided", "");
                 lo.F = x.F - bh;
                 shift(&lo.f, &rem, -(long)bh);
@@ -701,14 +710,14 @@ std::string g_decimal_stats = "[]";  // last conversion's per-level timings (JSO
 // D characters of pi from pi_fix ~ pi 2^prec (error bound err units).
 std::string digits_of(Mul &M, const Mpz &pi_fix, uint64_t digits, uint64_t prec, int threads, unsigned err) {
     Mpz ip;
-    __gmpz_tdiv_q_2exp(&ip, &pi_fix, prec);
+    par::shift(&ip, &pi_fix, -(long)prec);
     if (ip.limbs() != 1 || ip.z.d[0] != 3) die("integer part is not 3", "");
     std::string out(digits, '0');
     out[0] = '3';
     if (digits > 1) {
         Decimal dec(M, threads);
         Decimal::Frac root;
-        __gmpz_tdiv_r_2exp(&root.f, &pi_fix, prec);
+        par::trunc(&root.f, &pi_fix, prec);
         root.F = prec;
         root.n = digits - 1;
         root.at = 1;
@@ -725,6 +734,7 @@ std::string pi_digits_gpu(Mul &M, const Mpz &Q, const Mpz &Tabs, uint64_t digits
     Mpz pi_fix;
     assemble_pi(M, Q, Tabs, (unsigned long)prec, &pi_fix);
     *t_fa = since(t0);
+    M.t_gpu_fa = M.t_gpu;
     t0 = Clock::now();
     std::string out = digits_of(M, pi_fix, digits, prec, threads, 4);
     *t_dec = since(t0);
@@ -756,16 +766,15 @@ void sqrt_big(Mul &M, const gmp_mpz *x, unsigned long k, gmp_mpz *out) {
         shift(&vt, x, (long)(q + g) - (long)e);   // ~ v 2^(q+g)
         M.mul(&sq, &y, &y);                        // ~ 2^(2p) / v
         M.mul(&t, &vt, &sq);                       // ~ 2^(q+g+2p)
-        __gmpz_set_ui(&er, 1);
-        __gmpz_mul_2exp(&er, &er, q + g + 2 * p);
-        __gmpz_sub(&er, &er, &t);                  // ~ err 2^(q+g+p), signed
+        par::pow2(&er, q + g + 2 * p);
+        par::sub(&er, &er, &t);                    // ~ err 2^(q+g+p), signed
         const long eb = (long)bits_of(&er);
         const long drop = std::max(0L, eb - (long)(q - p + 48));
         shift(&er, &er, -drop);
         M.mul(&corr, &y, &er);
         shift(&corr, &corr, drop - (long)(g + 3 * p + 1));
         shift(&y, &y, (long)(q - p));
-        __gmpz_add(&y, &y, &corr);
+        par::add(&y, &y, &corr);
         p = q;
     }
     // sqrt(x) = x y_k / 2^(e/2 + k), with x cut to k + 64 bits
@@ -791,17 +800,17 @@ void pi_gauss_legendre(Mul &M, unsigned long prec, gmp_mpz *pi_fix, int *iters_o
     __gmpz_mul_2exp(&t, &t, W - 2);
     const int iters = (int)std::ceil(std::log2((double)prec)) + 2;
     for (int i = 0; i < iters; i++) {
-        __gmpz_add(&an, &a, &b);
-        __gmpz_tdiv_q_2exp(&an, &an, 1);
+        par::add(&an, &a, &b);
+        shift(&an, &an, -1);
         M.mul(&ab, &a, &b);                // scale 2^(2W)
         sqrt_big(M, &ab, k, &b);           // b' = sqrt(a b), scale 2^W
-        __gmpz_sub(&dd, &a, &an);
+        par::sub(&dd, &a, &an);
         M.mul(&dd, &dd, &dd);
         shift(&dd, &dd, (long)i - (long)W);  // p (a - a')^2 with p = 2^i
-        __gmpz_sub(&t, &t, &dd);
+        par::sub(&t, &t, &dd);
         __gmpz_swap(&a, &an);
     }
-    __gmpz_add(&an, &a, &b);
+    par::add(&an, &a, &b);
     M.mul(&ab, &an, &an);                  // (a+b)^2, scale 2^(2W)
     shift(&ab, &ab, -(long)W);
     Mpz R;
@@ -866,7 +875,7 @@ void usage() {
     std::fprintf(stderr,
                  "usage: pi_compute DIGITS OUT_FILE [--algorithm chudnovsky|gauss-legendre] [--gpu-min-limbs L]\n"
                  "                  [--threads T] [--host-merge] [--merge dataflow|levels] [--shutdown]\n"
-                 "                     [--fa-gmp] [--fa-min-limbs L] [--device D] [--plans CACHE.json]\n");
+                 "                     [--fa-gmp] [--fa-min-limbs L] [--par-min-limbs L] [--device D] [--plans CACHE.json]\n");
     std::exit(2);
 }
 
@@ -944,6 +953,7 @@ int main(int argc, char **argv) {
         else if (a == "--plans" && i + 1 < argc) o.plans = argv[++i];
         else if (a == "--algorithm" && i + 1 < argc) o.algorithm = argv[++i];
         else if (a == "--shutdown") o.shutdown = true;
+        else if (a == "--par-min-limbs" && i + 1 < argc) o.par_min_limbs = std::atof(argv[++i]);
         else if (a == "--merge" && i + 1 < argc) {
             const std::string m = argv[++i];
             if (m == "levels") o.merge_levels = true;
@@ -955,6 +965,8 @@ int main(int argc, char **argv) {
         unsigned hc = std::thread::hardware_concurrency();
         o.threads = (int)std::min(16u, hc ? hc : 1u);
     }
+    par::cfg.threads = o.threads;
+    par::cfg.min_limbs = (size_t)std::max(1.0, o.par_min_limbs);
     auto t_start = Clock::now();
     FILE *f = std::fopen(o.out.c_str(), "wb");
     if (!f) die("cannot open", o.out.c_str());
@@ -1079,13 +1091,15 @@ int main(int argc, char **argv) {
                 "\"t_serve_stop\": %.3f, \"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, "
                 "\"t_shutdown\": %.3f, \"t_total\": %.3f, "
                 "\"serve_iterations\": %llu, \"serve_launches\": %llu, \"fa\": \"%s\", \"fa_gpu_products\": %zu, "
-                "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"decimal_levels\": %s, \"levels\": %s}\n",
+                "\"fa_host_products\": %zu, \"fa_gpu_operand_limbs\": %.0f, \"fa_gpu_s\": %.3f, \"dec_gpu_s\": %.3f, "
+                "\"decimal_levels\": %s, "
+                "\"levels\": %s}\n",
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
                 !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"),
                 dataflow ? "dataflow" : "levels", t_levels, o.threads, o.gpu_min_limbs,
                 o.fa_min_limbs, d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
                 t_download, t_serve_stop, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
-                M.host_products, M.gpu_limbs, g_decimal_stats.c_str(), lv.c_str());
+                M.host_products, M.gpu_limbs, M.t_gpu_fa, M.t_gpu - M.t_gpu_fa, g_decimal_stats.c_str(), lv.c_str());
     return finish(o);
 }

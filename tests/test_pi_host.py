@@ -60,6 +60,22 @@ def test_host_merged_tree(tmp_path, digits, limbs, fa):
     assert hashlib.sha256(data).hexdigest() == gold[str(digits)]["sha256"]
 
 
+@pytest.mark.parametrize("digits,threads", [(100000, "4"), (1000000, "7")])
+def test_multithreaded_bignum_steps(tmp_path, digits, threads):
+    """The final assembly's and the decimal conversion's shifts, sums,
+    differences and word multiplies on several threads from 64 limbs up
+    (pi_amd/csrc/par_mpz.hpp; the default switches at 131072 limbs), with
+    host products: the same digits."""
+    gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]
+    data, st = _run(tmp_path, digits, "--host-merge", "--gpu-min-limbs", "64", "--fa-min-limbs", "1e18",
+                    "--par-min-limbs", "64", "--threads", threads)
+    assert st["fa"] == "host"
+    assert hashlib.sha256(data).hexdigest() == gold[str(digits)]["sha256"]
+    data, st = _run(tmp_path, digits, "--algorithm", "gauss-legendre", "--fa-min-limbs", "1e18",
+                    "--par-min-limbs", "64", "--threads", threads)
+    assert hashlib.sha256(data).hexdigest() == gold[str(digits)]["sha256"]
+
+
 @pytest.mark.parametrize("digits", [2, 1000, 10000, 100000])
 def test_gauss_legendre_host(tmp_path, digits):
     """The AGM cross-check (gauss_legendre.rs:61-166) with host products."""
