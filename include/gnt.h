@@ -39,7 +39,9 @@ typedef struct gnt_config {
     const char *plan_cache_path;/* autotuner plan cache (JSON); NULL = built-in plans */
     size_t serve_split_log_n;   /* serving: products with transforms up to 2^this run on the
                                    small-request lane, larger ones on the large lane; 0 = 22 */
-    size_t reserved[3];
+    size_t serve_large_inflight;/* serving: products the large lane advances at once (admitted
+                                   in arrival order); 0 = 1, SIZE_MAX = no limit */
+    size_t reserved[2];
 } gnt_config;
 
 /* ---- lifetime ---- */

@@ -56,6 +56,7 @@ struct Ctx {
     Stager up{true}, down{false};  // host <-> device copies of caller memory
     size_t hbm_cap = 0;
     int serve_split_log_n = 22;  // serving: largest transform (log2) on the small-request lane
+    size_t serve_large_inflight = 1;  // serving: products the large lane advances at once
     bool timing = false;
     double phase_ms[5] = {0, 0, 0, 0, 0};
     uint64_t products = 0;

@@ -106,6 +106,8 @@ struct Lane {
     uint32_t *lens = nullptr;
     std::vector<int> lens_free;  // guarded by Server::mu
     hipEvent_t ctx_ev = nullptr; // orders admissions after the context stream's work
+    size_t max_inflight = 0;     // products advanced at once (0: any number)
+    size_t pending_admit = 0;    // admitted in this pass, not yet in `active`
 
     int init(Server *srv, Ctx *ctx);
     void destroy();
@@ -264,6 +266,7 @@ void Lane::destroy() {
 // anything).
 bool Lane::admissible() const {
     if (waiting.empty()) return false;
+    if (max_inflight && active.size() + pending_admit >= max_inflight) return false;
     return s->bytes_in_flight == 0 || s->bytes_in_flight + waiting.front()->bytes <= c->hbm_cap;
 }
 
@@ -356,9 +359,11 @@ void Lane::loop() {
             // admission in arrival order while the bytes in flight fit the cap
             // (bytes_in_flight counts everything admitted by either lane and
             // not retired, including requests admitted earlier in this pass)
+            pending_admit = 0;
             while (admissible()) {
                 Req *q = waiting.front();
                 waiting.pop_front();
+                pending_admit++;
                 s->bytes_in_flight += q->bytes;
                 s->peak_bytes = std::max(s->peak_bytes, s->bytes_in_flight);
                 s->peak_active = std::max(s->peak_active, ++s->n_admitted);
@@ -380,6 +385,7 @@ void Lane::loop() {
                 if (rc) fail_request(q, rc);
                 else active.push_back(q);
             }
+            pending_admit = 0;  // now counted in `active`
             if (!active.empty()) run_iteration();
         }
         retire();
@@ -486,6 +492,7 @@ int gnt_serve_start(gnt_ctx *ctx) {
     if (!c) return gnt::set_error(GNT_E_ARG, "null context");
     if (c->server) return GNT_OK;
     gnt::Server *s = new gnt::Server(c, c->serve_split_log_n);
+    s->lanes[1].max_inflight = c->serve_large_inflight;
     for (gnt::Lane &l : s->lanes)
         if (l.init(s, c) != GNT_OK) {
             for (gnt::Lane &m : s->lanes) m.destroy();

@@ -49,7 +49,7 @@ class GntError(RuntimeError):
 
 class GntConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_cap_bytes", SZ), ("plan_cache_path", ctypes.c_char_p),
-                ("serve_split_log_n", SZ), ("reserved", SZ * 3)]
+                ("serve_split_log_n", SZ), ("serve_large_inflight", SZ), ("reserved", SZ * 2)]
 
 
 class GntReq(ctypes.Structure):
@@ -167,11 +167,11 @@ class Context:
     """One device context (gnt_ctx)."""
 
     def __init__(self, device: int = 0, hbm_cap_bytes: int = 0, plan_cache: Optional[str] = None,
-                 serve_split_log_n: int = 0):
+                 serve_split_log_n: int = 0, serve_large_inflight: int = 0):
         L = load()
         cfg = GntConfig(device=device, hbm_cap_bytes=hbm_cap_bytes,
                         plan_cache_path=plan_cache.encode() if plan_cache else None,
-                        serve_split_log_n=serve_split_log_n)
+                        serve_split_log_n=serve_split_log_n, serve_large_inflight=serve_large_inflight)
         h = ctypes.c_void_p()
         _check(L.gnt_init(ctypes.byref(h), ctypes.byref(cfg)))
         self.h = h
