@@ -223,7 +223,7 @@ static TItem titem(const Prod &p, int pass, uint64_t *x, const uint64_t *limbs, 
     return t;
 }
 
-void add_stage(Iteration &it, Prod &p) {
+static void add_one_stage(Iteration &it, Prod &p) {
     const int P = p.pl.npass;
     const int s = p.stage++;
     const bool sq = p.sqr();
@@ -314,6 +314,18 @@ void add_stage(Iteration &it, Prod &p) {
         it.want_lens = true;
     }
     it.citems.push_back(c);
+}
+
+// The last transform stage (or a sum's first carry step) takes the three
+// carry steps with it: an iteration launches its transform groups before its
+// carry groups, and those in step order, on one stream.
+int next_stage_span(const Prod &p) {
+    const int carry0 = p.nstages - 3;  // first carry step
+    return (p.stage == carry0 - 1 || p.stage == carry0) ? p.nstages - p.stage : 1;
+}
+
+void add_stage(Iteration &it, Prod &p) {
+    for (int k = next_stage_span(p); k > 0; k--) add_one_stage(it, p);
 }
 
 void finalize_iteration(Iteration &it) {

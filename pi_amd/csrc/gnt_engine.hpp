@@ -120,8 +120,11 @@ struct Iteration {
     bool want_lens = false;  // some last carry step has a len_host: copy after the ripple
 };
 
-// Append the stage p.stage of product p to `it` and advance p.stage.
+// Append the next stage of product p to `it` and advance p.stage: one stage,
+// or the last transform stage with the three carry steps after it.
 void add_stage(Iteration &it, Prod &p);
+// Stages the next add_stage of p takes (its last call finishes the product).
+int next_stage_span(const Prod &p);
 // Sort the items into launch groups (kind, radix) and fill tile offsets.
 void finalize_iteration(Iteration &it);
 // Launch one iteration whose items were copied to dev_t / dev_c.

@@ -399,7 +399,7 @@ void Lane::run_iteration() {
     {
         std::lock_guard<std::mutex> g(s->mu);
         for (Req *q : active)
-            if (q->p.stage == q->p.nstages - 1) {
+            if (q->p.stage + next_stage_span(q->p) >= q->p.nstages) {
                 finishing.push_back(q);
                 if (lens_free.empty()) continue;  // length found later by a device read
                 q->len_slot = lens_free.back();

@@ -995,6 +995,9 @@ int main(int argc, char **argv) {
         gnt_config cfg{};
         cfg.device = o.device;
         cfg.plan_cache_path = o.plans.empty() ? nullptr : o.plans.c_str();
+        // the merges want throughput, not the tail latency that one large
+        // product at a time buys a stream of independent requests
+        cfg.serve_large_inflight = SIZE_MAX;
         check(gnt_init(&d.ctx, &cfg), "init");
         if (gpu_merge) check(gnt_serve_start(d.ctx), "serve start");
         t_init = since(t0);
