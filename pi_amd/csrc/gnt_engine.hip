@@ -223,6 +223,15 @@ static TItem titem(const Prod &p, int pass, uint64_t *x, const uint64_t *limbs, 
     return t;
 }
 
+// Forward pass `pass` of operand A or B (packed from its limbs at pass 0).
+static void add_fwd(Iteration &it, const Prod &p, int pass, bool b) {
+    TItem t = b ? titem(p, pass, p.xb, pass == 0 ? p.db : nullptr, p.nb)
+                : titem(p, pass, p.xa, pass == 0 ? p.da : nullptr, p.na);
+    t.pad = K_FWD;
+    t.T = p.tabs.fwd[pass];
+    it.titems.push_back(t);
+}
+
 static void add_one_stage(Iteration &it, Prod &p) {
     const int P = p.pl.npass;
     const int s = p.stage++;
@@ -254,16 +263,8 @@ static void add_one_stage(Iteration &it, Prod &p) {
         return;
     }
     if (s < nfwd) {
-        TItem a = titem(p, s, p.xa, s == 0 ? p.da : nullptr, p.na);
-        a.pad = K_FWD;
-        a.T = p.tabs.fwd[s];
-        it.titems.push_back(a);
-        if (!sq && s < P - 1) {
-            TItem b = titem(p, s, p.xb, s == 0 ? p.db : nullptr, p.nb);
-            b.pad = K_FWD;
-            b.T = p.tabs.fwd[s];
-            it.titems.push_back(b);
-        }
+        add_fwd(it, p, s, false);
+        if (!sq && s < P - 1) add_fwd(it, p, s, true);
         return;
     }
     if (s == nfwd && ab) {
@@ -324,7 +325,27 @@ int next_stage_span(const Prod &p) {
     return (p.stage == carry0 - 1 || p.stage == carry0) ? p.nstages - p.stage : 1;
 }
 
+// A b_late multiply (two or more passes): forward stages of A while B is
+// missing, B's forward passes once it is there (one per iteration, beside
+// A's), then the usual stages; B's last pass and the fused stage may share an
+// iteration (forward groups launch first).
+static int fwd_stages(const Prod &p) { return p.pl.npass - 1; }
+
+bool waits_for_b(const Prod &p) {
+    return p.b_late && !p.b_ready && p.stage >= fwd_stages(p);
+}
+
 void add_stage(Iteration &it, Prod &p) {
+    if (p.b_late) {
+        const int nf = fwd_stages(p);
+        if (p.b_ready && p.b_next < nf) add_fwd(it, p, p.b_next++, true);
+        if (p.stage < nf) {
+            add_fwd(it, p, p.stage++, false);
+            return;
+        }
+        if (!p.b_ready || p.b_next < nf) return;  // B not yet caught up
+        p.b_late = false;
+    }
     for (int k = next_stage_span(p); k > 0; k--) add_one_stage(it, p);
 }
 

@@ -95,6 +95,12 @@ struct Prod {
     int nstages = 0;
     int op = OP_MUL;  // OP_ADD / OP_SUB: carry stages only, no transform
     uint32_t *len_host = nullptr;  // pinned word that receives the trimmed length (or null)
+    // A multiply whose B reaches the device after A (serving): A's forward
+    // passes run ahead, B's follow once b_ready (b_next of them done), and the
+    // fused stage waits for both.
+    bool b_late = false;
+    bool b_ready = true;
+    int b_next = 0;
     bool sqr() const { return db == nullptr; }
     bool linear() const { return op == OP_ADD || op == OP_SUB; }
     size_t scratch_bytes() const;            // xa + xb + lh + cs
@@ -125,6 +131,8 @@ struct Iteration {
 void add_stage(Iteration &it, Prod &p);
 // Stages the next add_stage of p takes (its last call finishes the product).
 int next_stage_span(const Prod &p);
+// True when p (b_late) has nothing to launch until B is on the device.
+bool waits_for_b(const Prod &p);
 // Sort the items into launch groups (kind, radix) and fill tile offsets.
 void finalize_iteration(Iteration &it);
 // Launch one iteration whose items were copied to dev_t / dev_c.

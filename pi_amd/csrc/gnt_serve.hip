@@ -59,6 +59,11 @@ struct Req {
     int64_t len = -1;                        // trimmed result length once known
     size_t up_total = 0, up_done = 0;        // host operand bytes to upload, uploaded so far
     hipEvent_t uploaded = nullptr;           // recorded on the copy stream after the last slice
+    // A multiply whose B is uploaded after the request is admitted: its lane
+    // runs A's forward passes meanwhile (Prod::b_late).
+    bool b_split = false;
+    bool b_arrived = false;                  // B's last slice issued (Server::mu)
+    hipEvent_t uploaded_b = nullptr;         // recorded after B's last slice
     void *lane = nullptr;                    // the lane that runs it
 };
 
@@ -72,6 +77,10 @@ struct Slot {  // descriptor staging for one iteration in flight
 }  // namespace
 
 namespace gnt {
+
+// Host B operands from this size on are uploaded after their request is
+// admitted, while A's forward passes run.
+constexpr size_t B_SPLIT_MIN_BYTES = size_t(1) << 20;
 
 struct Server;
 
@@ -112,6 +121,7 @@ struct Lane {
     int init(Server *srv, Ctx *ctx);
     void destroy();
     bool admissible() const;     // Server::mu held
+    bool can_launch() const;     // Server::mu held: some admitted product has work to launch
     int admit(Req *q);
     void free_scratch(Req *q);
     void fail_request(Req *q, int code);
@@ -212,20 +222,34 @@ void Server::upload_loop() {
             len = std::min(UPLOAD_SLICE, q->p.nb * 8 - o);
             e = c->up.copy((char *)q->own_b + o, (const char *)q->r.b + o, len, c->copy_stream);
         }
-        const bool finished = done + len == q->up_total;
-        if (e == hipSuccess && finished) {
-            e = hipEventCreateWithFlags(&q->uploaded, hipEventDisableTiming);
-            if (e == hipSuccess) e = hipEventRecord(q->uploaded, c->copy_stream);
+        bool finished = done + len == q->up_total;
+        // a split request goes to its lane once A is on the device; from then
+        // on its lane owns the operands, so an error only ends B's upload and
+        // is reported when the product completes
+        const bool a_now = q->b_split && a_bytes && done < a_bytes && done + len == a_bytes;
+        const bool handed = q->b_split && done >= a_bytes;
+        hipEvent_t *ev = handed ? &q->uploaded_b : &q->uploaded;
+        if (e == hipSuccess && (finished || a_now)) {
+            e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventRecord(*ev, c->copy_stream);
+        }
+        if (e != hipSuccess && handed) {
+            if (*ev) (void)hipEventDestroy(*ev);
+            *ev = nullptr;
+            q->status = GNT_E_DEVICE;
+            e = hipSuccess;
+            finished = true;
         }
         Lane *lane = (Lane *)q->lane;
         {
             std::lock_guard<std::mutex> g(mu);
             q->up_done = done + len;
             if (e != hipSuccess || finished) uploads.erase(std::find(uploads.begin(), uploads.end(), q));
-            if (e == hipSuccess && finished) lane->waiting.push_back(q);
+            if (e == hipSuccess && (a_now || (finished && !q->b_split))) lane->waiting.push_back(q);
+            if (e == hipSuccess && finished && q->b_split) q->b_arrived = true;
         }
         if (e != hipSuccess) fail_upload(q, GNT_E_DEVICE);
-        else if (finished) lane->cv_work.notify_one();
+        else if (finished || a_now) lane->cv_work.notify_one();
     }
 }
 
@@ -268,6 +292,12 @@ bool Lane::admissible() const {
     if (waiting.empty()) return false;
     if (max_inflight && active.size() + pending_admit >= max_inflight) return false;
     return s->bytes_in_flight == 0 || s->bytes_in_flight + waiting.front()->bytes <= c->hbm_cap;
+}
+
+bool Lane::can_launch() const {
+    for (const Req *q : active)
+        if (!waits_for_b(q->p) || q->b_arrived) return true;
+    return false;
 }
 
 // Buffers of an admitted request (lane thread only).  Host operands were
@@ -327,11 +357,26 @@ Slot &Lane::slot_for(size_t bytes) {
 void Lane::fail_request(Req *q, int code) {
     (void)hipStreamSynchronize(stream);
     free_scratch(q);
-    if (q->uploaded) (void)hipEventDestroy(q->uploaded);
-    q->uploaded = nullptr;
+    // a late B may still be uploading into the operand buffers
+    for (;;) {
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (!q->b_split || q->b_arrived) break;
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
+    }
+    for (hipEvent_t *e : {&q->uploaded, &q->uploaded_b}) {
+        if (*e) {
+            (void)hipEventSynchronize(*e);
+            (void)hipEventDestroy(*e);
+        }
+        *e = nullptr;
+    }
     s->free_operands(q);
     c->pool.release(q->p.dout, q->p.nl * 8);
     c->pool.release(q->p.dlen, 4);
+    q->p.dout = nullptr;
+    q->p.dlen = nullptr;
     {
         std::lock_guard<std::mutex> g(s->mu);
         s->bytes_in_flight -= q->bytes;
@@ -354,7 +399,7 @@ void Lane::loop() {
         std::vector<Req *> admitted;
         {
             std::unique_lock<std::mutex> g(s->mu);
-            cv_work.wait(g, [&] { return s->stop || admissible() || !active.empty() || !draining.empty(); });
+            cv_work.wait(g, [&] { return s->stop || admissible() || can_launch() || !draining.empty(); });
             if (s->stop && waiting.empty() && active.empty() && draining.empty()) return;
             // admission in arrival order while the bytes in flight fit the cap
             // (bytes_in_flight counts everything admitted by either lane and
@@ -399,6 +444,12 @@ void Lane::run_iteration() {
     {
         std::lock_guard<std::mutex> g(s->mu);
         for (Req *q : active)
+            if (q->p.b_late && !q->p.b_ready && q->b_arrived) {
+                // B's last slices may still be on the copy stream
+                if (q->uploaded_b) (void)hipStreamWaitEvent(stream, q->uploaded_b, 0);
+                q->p.b_ready = true;
+            }
+        for (Req *q : active)
             if (q->p.stage + next_stage_span(q->p) >= q->p.nstages) {
                 finishing.push_back(q);
                 if (lens_free.empty()) continue;  // length found later by a device read
@@ -409,6 +460,7 @@ void Lane::run_iteration() {
     }
     Iteration it;
     for (Req *q : active) add_stage(it, q->p);
+    if (it.titems.empty() && it.citems.empty()) return;  // every product waits for its B
     finalize_iteration(it);
     const size_t tb = (it.titems.size() * sizeof(TItem) + 255) & ~(size_t)255;
     const size_t bytes = tb + it.citems.size() * sizeof(CItem) + 256;
@@ -446,15 +498,22 @@ void Lane::run_iteration() {
 // events complete in stream order).  With nothing left to launch, wait for
 // the oldest one rather than spin.
 void Lane::retire() {
-    if (active.empty() && !draining.empty()) (void)hipEventSynchronize(draining.front().ev);
+    bool idle;  // nothing to launch before an iteration finishes or a B arrives
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        idle = !can_launch();
+    }
+    if (idle && !draining.empty()) (void)hipEventSynchronize(draining.front().ev);
     bool freed = false;
     while (!draining.empty() && hipEventQuery(draining.front().ev) == hipSuccess) {
         Done d = std::move(draining.front());
         draining.pop_front();
         free_events.push_back(d.ev);
         for (Req *q : d.reqs) {
-            if (q->uploaded) (void)hipEventDestroy(q->uploaded);
-            q->uploaded = nullptr;
+            for (hipEvent_t *e : {&q->uploaded, &q->uploaded_b}) {
+                if (*e) (void)hipEventDestroy(*e);
+                *e = nullptr;
+            }
             s->free_operands(q);
         }
         {
@@ -581,17 +640,23 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
         return gnt::set_error(GNT_E_NOMEM, "no device memory for the operands");
     }
     q->up_total = (q->own_a ? p.na * 8 : 0) + (q->own_b ? p.nb * 8 : 0);
+    // a multiply of two or more passes whose host B is large starts on A while
+    // B is still uploading (A first: the uploader moves a, then b)
+    q->b_split = req->op == GNT_OP_MUL && q->own_b && p.pl.npass >= 2 && p.nb * 8 >= gnt::B_SPLIT_MIN_BYTES;
+    p.b_late = q->b_split;
+    p.b_ready = !q->b_split;
     gnt::Lane &lane = s->lane_for(p);
     q->lane = &lane;
+    const bool to_lane = !q->up_total || (q->b_split && !q->own_a);  // A needs no upload
     {
         std::lock_guard<std::mutex> g(s->mu);
         q->ticket = s->next_ticket++;
         s->tickets[q->ticket] = q;
         if (q->up_total) s->uploads.push_back(q);
-        else lane.waiting.push_back(q);
+        if (to_lane) lane.waiting.push_back(q);
     }
     if (q->up_total) s->cv_up.notify_one();
-    else lane.cv_work.notify_one();
+    if (to_lane) lane.cv_work.notify_one();
     *ticket = q->ticket;
     return GNT_OK;
 }
@@ -624,7 +689,12 @@ int gnt_wait(gnt_ctx *ctx, uint64_t ticket, int64_t timeout_us, gnt_buf **result
     s->tickets.erase(it);
     const int st = q->status;
     if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(q->t_done - q->t_submit).count();
-    if (st == GNT_OK) *result = gnt::buf_wrap(c, q->p.dout, q->p.nl, q->p.dlen, q->len);
+    if (st == GNT_OK) {
+        *result = gnt::buf_wrap(c, q->p.dout, q->p.nl, q->p.dlen, q->len);
+    } else {  // a request that ran to the end on a failed upload returns its result buffer
+        c->pool.release(q->p.dout, q->p.nl * 8);
+        c->pool.release(q->p.dlen, 4);
+    }
     delete q;
     if (st != GNT_OK) return gnt::set_error(st, "request failed");
     return GNT_OK;
