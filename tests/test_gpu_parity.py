@@ -440,6 +440,40 @@ def test_serving_two_lanes(orc):
             assert np.array_equal(got, orc.gmp_sqr(a) if b is None else orc.gmp_mul(a, b))
 
 
+@pytest.mark.parametrize("inflight", [0, 3, (1 << 64) - 1])
+def test_serving_late_b_operand(orc, inflight):
+    """Multiplies whose host B (2 MiB and more) is still uploading when A's
+    forward passes start (host A and host B, and a device A with host B),
+    several at once, with the large lane advancing 1 (default), 3 or any
+    number of products: every product exact and the same as alone."""
+    from pi_amd import gnt
+    c = gnt.Context(0, serve_split_log_n=20, serve_large_inflight=inflight)
+    srv = gnt.Server(c)
+    try:
+        n = 1 << 18
+        ops = [(orc.splitmix64(500 + i, n - 7 * i), orc.splitmix64(600 + i, n + 5 * i)) for i in range(4)]
+        ts = [srv.submit(a, b) for a, b in ops]
+        dev_a = c.upload(ops[0][0])
+        t_dev = srv.submit(b=ops[1][1], a_buf=dev_a)
+        outs = []
+        for t in ts:
+            buf, _ = srv.wait(t)
+            outs.append(buf.download())
+            buf.release()
+        buf, _ = srv.wait(t_dev)
+        got_dev = buf.download()
+        buf.release()
+        dev_a.release()
+        st = srv.stats()
+        assert st["large_lane_iterations"] > 0, st
+    finally:
+        srv.stop()
+        c.close()
+    for (a, b), got in zip(ops, outs):
+        assert np.array_equal(got, orc.gmp_mul(a, b))
+    assert np.array_equal(got_dev, orc.gmp_mul(ops[0][0], ops[1][1]))
+
+
 # ---- serving loop: sums and differences (binary splitting's t = x +- y) ----
 
 def _rand_int(rng, n):
