@@ -84,11 +84,12 @@ constexpr size_t B_SPLIT_MIN_BYTES = size_t(1) << 20;
 
 struct Server;
 
-// One scheduler thread over one stream.  It admits its own requests in
-// arrival order while the server's bytes in flight stay under the cap,
-// advances every admitted product one stage per iteration, and retires the
-// finished ones.  Scratch comes from the lane's own pool, whose reuse follows
-// the lane's stream order.
+// One scheduler thread over one stream.  It admits its own requests in the
+// order their operands reach the device (up to max_inflight products at once)
+// while the server's bytes in flight stay under the cap, advances every
+// admitted product one stage per iteration, and retires the finished ones an
+// iteration at a time.  Scratch comes from the lane's own pool, whose reuse
+// follows the lane's stream order.
 struct Lane {
     Server *s = nullptr;
     Ctx *c = nullptr;
@@ -300,10 +301,10 @@ bool Lane::can_launch() const {
     return false;
 }
 
-// Buffers of an admitted request (lane thread only).  Host operands were
-// uploaded at submission (copy stream), so only the result and scratch are
-// allocated: the result from the context's pool (it outlives the request),
-// scratch from the lane's.
+// Buffers of an admitted request (lane thread only).  Host operands have
+// device buffers from submission (filled by the uploader on the copy stream),
+// so only the result and scratch are allocated: the result from the context's
+// pool (it outlives the request), scratch from the lane's.
 int Lane::admit(Req *q) {
     gnt::Prod &p = q->p;
     const bool sq = q->r.op == GNT_OP_SQR;
