@@ -639,6 +639,7 @@ def bench_chudnovsky(args):
         "data": "the Chudnovsky series (no input data)",
         "config": {"workload": f"pi to {args.digits} digits", "n_terms": st["n_terms"], "prec_bits": st["prec_bits"],
                    "gpu_min_limbs": st["gpu_min_limbs"], "fa_min_limbs": st.get("fa_min_limbs"),
+                   "dec_min_limbs": st.get("dec_min_limbs"),
                    "host_threads": st["threads"],
                    "plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in"},
         "digits_sha256": h.hexdigest(),

@@ -120,6 +120,8 @@ struct Options {
     bool host_merge = false;      // merge the upper tree on the host too (comparison run)
     bool fa_gmp = false;          // final assembly and radix conversion by GMP (comparison run)
     double fa_min_limbs = 2048;   // final-assembly / conversion products on the GPU from this size
+    double dec_min_limbs = 512;   // decimal conversion's batched products on the GPU from this size
+                                  // (batches cross as packed transfers: they pay from smaller sizes)
     int device = 0;
     std::string plans;            // plan cache from the autotuner
     std::string algorithm = "chudnovsky";
@@ -411,7 +413,8 @@ void parallel_for(size_t n, int threads, F f) {
 // both operands reach min_limbs limbs (libgnt, host buffers), otherwise GMP.
 struct Mul {
     gnt_ctx *ctx = nullptr;
-    double min_limbs = 1024;
+    double min_limbs = 1024;        // single products (host buffers, one transfer each way)
+    double batch_min_limbs = 1024;  // batched products (packed transfers, shared launches)
     int threads = 1;
     size_t gpu_products = 0, host_products = 0;
     double gpu_limbs = 0;
@@ -451,7 +454,8 @@ struct Mul {
     // threads do the small ones.
     void batch(std::vector<Job> &jobs) {
         std::vector<size_t> g, h;
-        for (size_t i = 0; i < jobs.size(); i++) (on_gpu(nl(jobs[i].a), nl(jobs[i].b)) ? g : h).push_back(i);
+        for (size_t i = 0; i < jobs.size(); i++)
+            (ctx && (double)std::min(nl(jobs[i].a), nl(jobs[i].b)) >= batch_min_limbs ? g : h).push_back(i);
         std::vector<gnt_job> gj(g.size());
         for (size_t k = 0; k < g.size(); k++) {
             Job &j = jobs[g[k]];
@@ -875,7 +879,8 @@ void usage() {
     std::fprintf(stderr,
                  "usage: pi_compute DIGITS OUT_FILE [--algorithm chudnovsky|gauss-legendre] [--gpu-min-limbs L]\n"
                  "                  [--threads T] [--host-merge] [--merge dataflow|levels] [--shutdown]\n"
-                 "                     [--fa-gmp] [--fa-min-limbs L] [--par-min-limbs L] [--device D] [--plans CACHE.json]\n");
+                 "                     [--fa-gmp] [--fa-min-limbs L] [--dec-min-limbs L] [--par-min-limbs L] [--device D]\n"
+                 "                     [--plans CACHE.json]\n");
     std::exit(2);
 }
 
@@ -913,6 +918,7 @@ int run_gauss_legendre(const Options &o, uint64_t prec, FILE *f, Clock::time_poi
     Mul M;
     M.ctx = ctx;
     M.min_limbs = o.fa_min_limbs;
+    M.batch_min_limbs = std::max(o.dec_min_limbs, 1.0);
     M.threads = o.threads;
     t0 = Clock::now();
     Mpz pi_fix;
@@ -949,6 +955,7 @@ int main(int argc, char **argv) {
         else if (a == "--host-merge") o.host_merge = true;
         else if (a == "--fa-gmp") o.fa_gmp = true;
         else if (a == "--fa-min-limbs" && i + 1 < argc) o.fa_min_limbs = std::atof(argv[++i]);
+        else if (a == "--dec-min-limbs" && i + 1 < argc) o.dec_min_limbs = std::atof(argv[++i]);
         else if (a == "--device" && i + 1 < argc) o.device = std::atoi(argv[++i]);
         else if (a == "--plans" && i + 1 < argc) o.plans = argv[++i];
         else if (a == "--algorithm" && i + 1 < argc) o.algorithm = argv[++i];
@@ -1056,6 +1063,7 @@ int main(int argc, char **argv) {
     Mul M;
     M.ctx = gpu_fa ? d.ctx : nullptr;
     M.min_limbs = o.fa_min_limbs;
+    M.batch_min_limbs = std::max(o.dec_min_limbs, 1.0);
     M.threads = o.threads;
     std::string s = o.fa_gmp ? final_assembly_gmp(Q, Tabs, o.digits, prec, &t_fa, &t_dec)
                              : pi_digits_gpu(M, Q, Tabs, o.digits, prec, o.threads, &t_fa, &t_dec);
@@ -1089,7 +1097,7 @@ int main(int argc, char **argv) {
     lv += "]";
     std::printf("{\"digits\": %llu, \"n_terms\": %llu, \"prec_bits\": %llu, \"merge\": \"%s\", \"merge_order\": \"%s\", "
                 "\"merge_busy_s\": %.3f, \"threads\": %d, "
-                "\"gpu_min_limbs\": %.0f, \"fa_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
+                "\"gpu_min_limbs\": %.0f, \"fa_min_limbs\": %.0f, \"dec_min_limbs\": %.0f, \"host_nodes\": %zu, \"merge_nodes\": %zu, \"t_init\": %.3f, "
                 "\"t_host_bs\": %.3f, \"t_upload\": %.3f, \"t_merge\": %.3f, \"t_download\": %.3f, "
                 "\"t_serve_stop\": %.3f, \"t_final_assembly\": %.3f, \"t_decimal\": %.3f, \"t_write\": %.3f, "
                 "\"t_shutdown\": %.3f, \"t_total\": %.3f, "
@@ -1100,7 +1108,7 @@ int main(int argc, char **argv) {
                 (unsigned long long)o.digits, (unsigned long long)n_terms, (unsigned long long)prec,
                 !any_merge ? "none" : (d.merges_on_device() ? "gpu" : "host"),
                 dataflow ? "dataflow" : "levels", t_levels, o.threads, o.gpu_min_limbs,
-                o.fa_min_limbs, d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
+                o.fa_min_limbs, o.dec_min_limbs, d.host_nodes.size(), d.nodes.size() - d.host_nodes.size(), t_init, t_host, t_upload, t_merge,
                 t_download, t_serve_stop, t_fa, t_dec, t_write, t_shutdown, t_total, (unsigned long long)sstats[0],
                 (unsigned long long)sstats[1], o.fa_gmp ? "gmp" : (gpu_fa ? "gpu" : "host"), M.gpu_products,
                 M.host_products, M.gpu_limbs, M.t_gpu_fa, M.t_gpu - M.t_gpu_fa, g_decimal_stats.c_str(), lv.c_str());
