@@ -184,6 +184,10 @@ struct Server {
 
 void Server::fail_upload(Req *q, int code) {
     (void)hipStreamSynchronize(c->copy_stream);
+    for (hipEvent_t *e : {&q->uploaded, &q->uploaded_b}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
     free_operands(q);
     {
         std::lock_guard<std::mutex> g(mu);
