@@ -99,7 +99,15 @@ GL_HD G make_geom(const PassDesc &d, size_t tile_index) {
 // consecutive lanes on consecutive columns), so it is plain row-major and a
 // task's R addresses are one base plus constant offsets.  The contiguous last
 // pass also walks down columns (its rows are adjacent in memory), so its
-// columns are XOR-swizzled by the row: both walks then hit distinct banks.
+// columns are XOR-swizzled by the row.  With rows of 32 columns or more both
+// walks then hit distinct banks.  With 8 columns or fewer a row covers only
+// part of the 64 banks and a column permutation cannot separate rows that
+// share them: the down-column walks and the last column-DFT sub-steps
+// conflict.  A layout that also permutes the low row bits by higher ones
+// removes every conflict, but its addresses are no longer one base plus
+// constants, so the compiler loses the paired LDS accesses and adds address
+// arithmetic.  On a 2^24-limb product that cost more than the conflicts did,
+// so the plain swizzle stays.
 template <class G>
 GL_HD int tix(const G &, int n, int c) {
     if constexpr (G::strided) return (n << G::log_c) + c;
