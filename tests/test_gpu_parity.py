@@ -184,8 +184,8 @@ def _fingerprints_ok(orc, c, a, b):
 
 @pytest.mark.parametrize("kind", ["uniform", "allones", "sparse"])
 def test_2p24_distributions(ctx, orc, kind):
-    """2^24 x 2^24 (N = 2^27): GMP when present, residue fingerprints and the
-    low limb always; the closed form for all-ones."""
+    """2^24 x 2^24 (N = 2^27) and the square of a: GMP when present, residue
+    fingerprints and the low limb always; the closed form for all-ones."""
     n = 1 << 24
     log(f"2^24 {kind}")
     a, b = _dist(orc, kind, n, 5), _dist(orc, kind, n, 6)
@@ -200,6 +200,10 @@ def test_2p24_distributions(ctx, orc, kind):
         assert np.array_equal(c, orc.gmp_mul(a, b))
     s = ctx.sqr(a)
     assert _fingerprints_ok(orc, s, a, a)
+    if kind == "allones":
+        assert np.array_equal(s, c)  # a == b: the square is the closed form above
+    elif orc.gmp() is not None:
+        assert np.array_equal(s, orc.gmp_sqr(a))
 
 
 def test_transform_lengths_up_to_the_cap(ctx, orc):
