@@ -77,15 +77,143 @@ def operand_dist(dist: str, seed: int, n: int) -> np.ndarray:
 # with rocprofv3 --pmc SQ_INSTS_VALU, keyed like HBM_TRAFFIC.  The secondary
 # bound: 1024 SIMDs issue one wave64 instruction each per 4 cycles at 2.4 GHz
 # for everything but plain 32-bit add/sub/xor/mov (2 cycles).
-VALU_INSTS = {
-    ("mul", 27, (8, 9, 10)): 1057226752,
-    ("sqr", 27, (8, 9, 10)): 753926144,
-}
+# Both tables hold counter readings of the kernels built from the sources
+# whose digest is PROFILED_SOURCES (kernel_sources_digest()); with any other
+# kernel source the line reports them as null rather than stale figures.
+PROFILED_SOURCES = ""
+VALU_INSTS = {}
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4  # wave64 instructions per second at 4 cycles each
 
-HBM_TRAFFIC = {
-    ("mul", 27, (8, 9, 10)): {"fused": 3226732305, "product": 14934773694},
-}
+HBM_TRAFFIC = {}
+
+KERNEL_SOURCES = ("gl_field.hpp", "gnt_plan.hpp", "gnt_pass.hpp", "gnt_carry.hpp", "gnt_kernels.hip",
+                  "gnt_kernels.hpp", "gnt_api.hip")
+
+
+def kernel_sources_digest() -> str:
+    """First 16 hex digits of the SHA-256 over the sources that define the
+    single-product launches (field arithmetic, plans, tile bodies, carry,
+    kernels, their host launch sequence)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "pi_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+# Three primes below 2^31 (2^31 - 1 and the next two): a wrong product passes
+# all three residue checks with probability about 2^-93.
+FP_PRIMES = (2147483647, 2147483629, 2147483587)
+_POW32 = {}  # q -> [2^(32k) mod q for k < len]
+
+
+def _pow32(q: int, n: int) -> np.ndarray:
+    """2^(32k) mod q for k < n (extended by doubling: w[m:2m] = w[:m] * 2^(32m))."""
+    w = _POW32.get(q)
+    if w is None or w.size < n:
+        m = 1 if w is None else w.size
+        if w is None:
+            w = np.ones(1, dtype=np.uint64)
+        while m < n:
+            step = np.uint64(pow(2, 32 * m, q))
+            w = np.concatenate([w, (w * step) % np.uint64(q)])  # < 2^62: exact
+            m *= 2
+        _POW32[q] = w
+    return w[:n]
+
+
+_CHUNK = 1 << 15
+
+
+_SEG = 1 << 20
+_POOL = None
+
+
+def _segment_residue(h32, w) -> int:
+    """sum h_k w_k mod q over one segment (q is folded in by the caller): w_k
+    is split at bit 16 so that h_k * (w_k mod 2^16) < 2^48 and
+    h_k * (w_k >> 16) < 2^47, and each partial sum runs over at most 2^15
+    terms (< 2^63)."""
+    h = h32.astype(np.uint64)
+    pad = (-h.size) % _CHUNK
+    if pad:
+        h = np.concatenate([h, np.zeros(pad, dtype=np.uint64)])
+        w = np.concatenate([w, np.zeros(pad, dtype=np.uint64)])
+    lo = (h * (w & np.uint64(0xFFFF))).reshape(-1, _CHUNK).sum(axis=1, dtype=np.uint64)
+    hi = (h * (w >> np.uint64(16))).reshape(-1, _CHUNK).sum(axis=1, dtype=np.uint64)
+    return sum(int(v) for v in lo) + (sum(int(v) for v in hi) << 16)
+
+
+def residues(x, primes=FP_PRIMES):
+    """The integer with little-endian limbs x, mod each prime: its 32-bit
+    halves h_k weighted by w_k = 2^(32k) mod q, in segments on host threads."""
+    global _POOL
+    from concurrent.futures import ThreadPoolExecutor
+    if _POOL is None:
+        _POOL = ThreadPoolExecutor(max_workers=host_threads())
+    h = np.ascontiguousarray(x, dtype="<u8").view("<u4")
+    out = []
+    for q in primes:
+        w = _pow32(q, h.size)
+        parts = _POOL.map(lambda k: _segment_residue(h[k:k + _SEG], w[k:k + _SEG]), range(0, h.size, _SEG))
+        out.append(sum(parts) % q)
+    return tuple(out)
+
+
+def residues_many(arrays, primes=FP_PRIMES):
+    """residues() of many arrays, one array per host thread."""
+    residues(np.zeros(1, dtype=np.uint64))  # starts the pool
+    big = max((x.size for x in arrays), default=0)
+    for q in primes:
+        _pow32(q, 2 * big)
+
+    def one(x):
+        h = np.ascontiguousarray(x, dtype="<u8").view("<u4")
+        return tuple(sum(_segment_residue(h[k:k + _SEG], _pow32(q, h.size)[k:k + _SEG])
+                         for k in range(0, h.size, _SEG)) % q for q in primes)
+
+    return list(_POOL.map(one, arrays))
+
+
+def residues_match(ra, rb, rc, primes=FP_PRIMES) -> bool:
+    return all(z == x * y % q for x, y, z, q in zip(ra, rb, rc, primes))
+
+
+def product_ok(c, a, b) -> bool:
+    """c == a * b by residues mod three primes (independent of the
+    transform), plus the low limb and the length bounds."""
+    ra, rb, rc = residues(a), residues(b), residues(c)
+    ok = all(z == x * y % q for x, y, z, q in zip(ra, rb, rc, FP_PRIMES))
+    ok &= c.size in (a.size + b.size, a.size + b.size - 1)
+    return bool(ok and int(c[0]) == (int(a[0]) * int(b[0])) % (1 << 64))
+
+
+_GMP = None
+
+
+def gmp_product(a, b):
+    """GMP's mpn_mul (libgmp's public API), or None where libgmp is absent."""
+    global _GMP
+    import ctypes
+    if _GMP is None:
+        try:
+            _GMP = ctypes.CDLL("libgmp.so.10")
+        except OSError:
+            _GMP = False
+    if not _GMP:
+        return None
+    if a.size < b.size:
+        a, b = b, a
+    a, b = np.ascontiguousarray(a, dtype=np.uint64), np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.zeros(a.size + b.size, dtype=np.uint64)
+    P = ctypes.c_void_p
+    _GMP.__gmpn_mul(P(out.ctypes.data), P(a.ctypes.data), ctypes.c_long(a.size), P(b.ctypes.data),
+                    ctypes.c_long(b.size))
+    n = out.size
+    while n and out[n - 1] == 0:
+        n -= 1
+    return out[:n]
 
 
 def valu_block(insts, ms):
@@ -107,36 +235,39 @@ def b_sqr(n: int, log_n: int) -> float:
     return 24.0 * n + 48.0 * (1 << log_n)
 
 
-def cpu_baseline(log_size: int, op: str, threads: int) -> dict:
-    """The oracle (CPU restatement of ntt.rs::mul_mag_ntt) on a bounded sample."""
+def cpu_baseline(log_size: int, op: str, threads: int, timed: int = 3, budget_s: float = 40.0) -> dict:
+    """The oracle (CPU restatement of ntt.rs::mul_mag_ntt, with the host knobs
+    config_gen.rs:56-99 picks for `threads` cores) on the metric's own
+    operands: one warm-up product, then the median of `timed` products (fewer
+    if the time budget runs out first)."""
     from oracle import oracle as orc
 
     orc.set_threads(threads)
     orc.apply_host_knobs(threads)
+    four_step = orc.get_four_step_threshold()   # the threshold the timed runs use
     n = 1 << log_size
     a, b = operand(3, n), operand(4, n)
     if op == "sqr":
         b = a
-    orc.mul_mag_ntt(a[:4096], b[:4096])  # warm the pool/twiddle caches for small sizes
-    times = []
-    t_budget = 25.0
     t_start = time.perf_counter()
-    reps = 0
-    while reps < 3 and (time.perf_counter() - t_start) < t_budget:
+    orc.mul_mag_ntt(a, b)                        # warm-up: pool buffers and twiddle tables
+    times = []
+    while len(times) < timed and (time.perf_counter() - t_start) < budget_s:
         t0 = time.perf_counter()
         orc.mul_mag_ntt(a, b)
         times.append(time.perf_counter() - t0)
-        reps += 1
-    t = float(np.median(times[1:] if len(times) > 1 else times))
-    # one thread on a quarter-size sample, and GMP's mpn_mul (one thread) beside it
-    n1 = n >> 2
+    t = float(np.median(times))
+    # one thread on a 2^(log_size-4) sample, and GMP's mpn_mul (one thread) beside it
+    n1 = n >> 4
     a1, b1 = a[:n1].copy(), (a if op == "sqr" else b)[:n1].copy()
     orc.set_threads(1)
     orc.apply_host_knobs(1)
+    orc.mul_mag_ntt(a1, b1)
     t0 = time.perf_counter()
     orc.mul_mag_ntt(a1, b1)
     t1 = time.perf_counter() - t0
     orc.set_threads(threads)
+    orc.apply_host_knobs(threads)
     gmp = None
     if orc.gmp() is not None:
         t0 = time.perf_counter()
@@ -147,10 +278,12 @@ def cpu_baseline(log_size: int, op: str, threads: int) -> dict:
         "unit": "Mlimbs/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{reps} x {op} 2^{log_size} x 2^{log_size} limbs (median of runs after the first), "
-                  f"four-step threshold {orc.get_four_step_threshold()}",
+        "sample": f"{op} 2^{log_size} x 2^{log_size} limbs (the metric's operands): 1 warm-up, median of "
+                  f"{len(times)} timed ({', '.join(f'{x:.2f}' for x in times)} s); four-step threshold "
+                  f"{four_step} (host knobs for {threads} cores)",
         "cpu_model": cpu_model(),
-        "one_thread": {"value": round(2 * n1 / t1 / 1e6, 3), "sample": f"1 x {op} 2^{log_size - 2} limbs"},
+        "one_thread": {"value": round(2 * n1 / t1 / 1e6, 3),
+                       "sample": f"1 x {op} 2^{log_size - 4} limbs after 1 warm-up"},
         "gmp_mpn_1thread": {"value": gmp, "sample": f"1 x {op} 2^{log_size} limbs"} if gmp else None,
     }
 
@@ -172,7 +305,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log-size", type=int, default=24, help="log2 limbs per operand")
     ap.add_argument("--op", choices=["mul", "sqr"], default="mul")
-    ap.add_argument("--cpu-log-size", type=int, default=22, help="log2 limbs of the CPU-baseline sample")
+    ap.add_argument("--cpu-log-size", type=int, default=24, help="log2 limbs of the CPU-baseline sample")
+    ap.add_argument("--cpu-timed", type=int, default=3, help="timed CPU-baseline products after the warm-up")
+    ap.add_argument("--no-batched", action="store_true", help="single: skip the 1024-pair batched run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky", "c2", "copies"], default="single",
                     help="single: the metric (one 2^24 product per step); batched: 1024 uneven pairs per step; "
@@ -258,16 +393,19 @@ def main():
     fused_ms = phases[1] / prods
     fused_bytes = (16 if args.op == "sqr" else 24) * (1 << log_n)
     fused_gbs = fused_bytes / (fused_ms * 1e-3) / 1e9
-    measured = HBM_TRAFFIC.get((args.op, log_n, tuple(radices[:3])))
+    digest = kernel_sources_digest()
+    fresh = digest == PROFILED_SOURCES
+    key = (args.op, log_n, tuple(radices))
+    measured = HBM_TRAFFIC.get(key) if fresh else None
     fused_traffic = measured["fused"] if measured else None
     product_traffic = measured["product"] if measured else None
 
-    # a check that the timed path computes the right thing (low limb + length)
+    # the timed path's result, whole: residues mod three primes, low limb, length
     r = ctx.sqr_buf(da) if args.op == "sqr" else ctx.mul_buf(da, db)
     out = r.download()
     r.release()
-    bb = a if args.op == "sqr" else b
-    ok = out.size in (2 * n, 2 * n - 1) and int(out[0]) == (int(a[0]) * int(bb[0])) % (1 << 64)
+    ok = product_ok(out, a, a if args.op == "sqr" else b)
+    del out
 
     line = {
         "metric": "Goldilocks-NTT multiply Mlimbs/s at 2^24 limbs; batched p50/p99 latency",
@@ -285,7 +423,7 @@ def main():
         "config": {
             "workload": f"single GPU {args.op} of 2^{args.log_size} x 2^{args.log_size} limbs",
             "limbs_a": n, "limbs_b": n, "log_n": log_n, "radices_log2": radices,
-            "inputs": "resident in HBM", "check_low_limb_and_length": ok,
+            "inputs": "resident in HBM", "result_exact_by_residues": ok,
             "plans": os.path.relpath(args.plans, ROOT) if args.plans else "built-in",
         },
         "roofline": {
@@ -299,7 +437,10 @@ def main():
             "traffic": fused_traffic,
             "algorithmic_bytes_per_launch": fused_bytes,
             "ms_per_launch": round(fused_ms, 4),
-            "valu": valu_block(VALU_INSTS.get((args.op, log_n, tuple(radices[:3]))), fused_ms),
+            "valu": valu_block(VALU_INSTS.get(key) if fresh else None, fused_ms),
+            "counters_from": {"kernel_sources": digest, "profiled_sources": PROFILED_SOURCES,
+                              "current": fresh, "profiles": "profiles/mul_2p24_hbm_traffic.md, "
+                                                            "profiles/transform_valu_2p24.md"},
             "whole_product": {
                 "achieved": round(achieved, 2), "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "bytes_model": "B_mul = 16(na+nb)+80N" if args.op == "mul" else "B_sqr = 24n+48N",
@@ -311,6 +452,10 @@ def main():
         },
     }
     ctx.close()
+    if not args.no_batched:
+        # configs' batched case beside the single product: 1024 uneven pairs
+        # as one captured plan, against one launch sequence per product
+        line["batched"] = batched_run(args, graph_steps=max(5, args.steps // 2), serial_steps=3)
     if not args.no_latency:
         # the metric's second half: p50/p99 latency of mixed-size requests
         # through the continuously batching serving loop
@@ -324,7 +469,8 @@ def main():
                                             "to result in HBM",
                            "run": pt}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(min(args.cpu_log_size, args.log_size), args.op, host_threads())
+        line["cpu_baseline"] = cpu_baseline(min(args.cpu_log_size, args.log_size), args.op, host_threads(),
+                                            timed=args.cpu_timed)
     print(json.dumps(line), flush=True)
 
 
@@ -339,43 +485,73 @@ def batched_pairs(n_pairs: int, seed: int = 7):
     return pairs
 
 
-def bench_batched(args):
-    """Config: batched uneven products.  Batched plan (graph replay, and direct
-    level launches) against one launch sequence per product, same buffers."""
-    from pi_amd import gnt
+def batched_modes(ctx, pairs, modes, steps: dict, warmup: int):
+    """Time each (label, graph, serial) mode of one batch plan over `pairs`
+    and check its results: every pair by residues mod three primes (with the
+    low limb and length), and every 128th pair limb for limb against GMP."""
+    limbs = sum(a.size + b.size for a, b in pairs)
+    ra = residues_many([a for a, _ in pairs])
+    rb = residues_many([b for _, b in pairs])
+    res = {}
+    for label, graph, serial in modes:
+        plan = ctx.batch_plan(pairs, graph=graph)
+        for _ in range(warmup):
+            plan.run(serial=serial)
+        ctx.sync()
+        k = steps[label]
+        t0 = time.perf_counter()
+        for _ in range(k):
+            plan.run(serial=serial)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / k
+        info = plan.info()
+        outs = plan.results()
+        plan.close()
+        rc = residues_many(outs)
+        ok = all(residues_match(x, y, z) and c.size in (a.size + b.size, a.size + b.size - 1)
+                 and int(c[0]) == (int(a[0]) * int(b[0])) % (1 << 64)
+                 for (a, b), c, x, y, z in zip(pairs, outs, ra, rb, rc))
+        gmp_ok = None
+        for i in range(0, len(pairs), 128):
+            want = gmp_product(*pairs[i])
+            if want is None:
+                break
+            gmp_ok = (gmp_ok is not False) and bool(np.array_equal(outs[i], want))
+        del outs
+        res[label] = {"ms": round(dt * 1e3, 3), "Mlimbs_per_s": round(limbs / dt / 1e6, 1), "runs": k,
+                      "launches_per_run": info["launches_per_run"] if not serial else None,
+                      "all_pairs_exact_by_residues": bool(ok), "sampled_pairs_equal_gmp": gmp_ok}
+    return res
 
+
+def batched_run(args, graph_steps: int, serial_steps: int) -> dict:
+    """The batched key of the default line: 1024 uneven pairs (n = round(2^U[8,20])
+    limbs, seed 7) as one captured plan and one launch sequence per product."""
     pairs = batched_pairs(args.pairs)
     limbs = sum(a.size + b.size for a, b in pairs)
     ctx = make_ctx(args)
-    res = {}
-    for label, graph, serial in (("batched_graph", True, False), ("batched_launches", False, False),
-                                 ("per_product", False, True)):
-        plan = ctx.batch_plan(pairs, graph=graph)
-        for _ in range(args.warmup):
-            plan.run(serial=serial)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            plan.run(serial=serial)
-        ctx.sync()
-        dt = (time.perf_counter() - t0) / args.steps
-        info = plan.info()
-        outs = plan.results()
-        # spot check: every product's low limb and length, fingerprints on a sample
-        ok = True
-        for i, ((a, b), c) in enumerate(zip(pairs, outs)):
-            ok &= c.size in (a.size + b.size, a.size + b.size - 1)
-            ok &= int(c[0]) == (int(a[0]) * int(b[0])) % (1 << 64)
-        q = (1 << 61) - 1
-        for i in range(0, len(pairs), 97):
-            a, b = pairs[i]
-            A = int.from_bytes(a.astype("<u8").tobytes(), "little") % q
-            B = int.from_bytes(b.astype("<u8").tobytes(), "little") % q
-            C = int.from_bytes(outs[i].astype("<u8").tobytes(), "little") % q
-            ok &= C == A * B % q
-        res[label] = {"ms": round(dt * 1e3, 3), "Mlimbs_per_s": round(limbs / dt / 1e6, 1),
-                      "launches_per_run": info["launches_per_run"] if not serial else None, "checks_ok": bool(ok)}
-        plan.close()
+    res = batched_modes(ctx, pairs, (("batched_graph", True, False), ("per_product", False, True)),
+                        {"batched_graph": graph_steps, "per_product": serial_steps}, warmup=2)
+    ctx.close()
+    return {"workload": f"{len(pairs)} pairs, n = round(2^U[8,20]) limbs per operand (seed 7), uniform limbs, "
+                        "operands resident in HBM",
+            "total_operand_limbs": limbs, "unit": "Mlimbs/s",
+            "value": res["batched_graph"]["Mlimbs_per_s"],
+            "batched_over_per_product": round(res["batched_graph"]["Mlimbs_per_s"]
+                                              / res["per_product"]["Mlimbs_per_s"], 3),
+            "modes": res}
+
+
+def bench_batched(args):
+    """Config: batched uneven products.  Batched plan (graph replay, and direct
+    level launches) against one launch sequence per product, same buffers."""
+    pairs = batched_pairs(args.pairs)
+    ctx = make_ctx(args)
+    res = batched_modes(ctx, pairs, (("batched_graph", True, False), ("batched_launches", False, False),
+                                     ("per_product", False, True)),
+                        {"batched_graph": args.steps, "batched_launches": args.steps, "per_product": args.steps},
+                        warmup=args.warmup)
+    limbs = sum(a.size + b.size for a, b in pairs)
     line = {
         "metric": "batched uneven products Mlimbs/s",
         "value": res["batched_graph"]["Mlimbs_per_s"], "unit": "Mlimbs/s", "n_gpus": 1,

@@ -73,6 +73,16 @@ int gnt_sqr(gnt_ctx *ctx, const uint64_t *a, size_t na, uint64_t *out, size_t ou
  * select the reference defaults (8192, 32).  out_cap >= na+nb+1. */
 int gnt_mul_mag(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out,
                 size_t out_cap, size_t *out_len, size_t ntt_threshold, size_t karatsuba_threshold);
+/* mul_mag_dispatch (integer.rs:460-486) with the transform cap passed in, as
+ * the reference passes ntt_max_coeffs so that its tests can force the
+ * Karatsuba-above-NTT branch on small inputs (integer.rs:1727-1754): the GPU
+ * product is taken only when 4(na+nb) <= ntt_max_coeffs (0 or anything above
+ * the library cap 2^32 means the cap).  Karatsuba's three sub-products run
+ * concurrently while the smaller operand has >= 512 limbs (rayon::join in
+ * integer.rs:565-593).  karatsuba_threshold 1 is refused (it never ends). */
+int gnt_mul_mag_dispatch(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out,
+                         size_t out_cap, size_t *out_len, size_t ntt_max_coeffs, size_t ntt_threshold,
+                         size_t karatsuba_threshold);
 
 /* ---- device-resident operands (no reference counterpart: the reference has
  * no device; these keep operands and results in HBM between dependent

@@ -4,7 +4,7 @@
 //
 // Every limb j has a value y_j (a limb sum L_j + H_{j-1}, or a + b) and a
 // generate bit g_j (that sum overflowed); it propagates an incoming carry
-// when y_j = 2^64 - 1 (p_j).  g_j and p_j are never both set.  Limb j of the
+// when y_j = 2^64 - 1 and g_j = 0 (p_j), so g_j and p_j are never both set.  Limb j of the
 // result is y_j + c_j with c_j = g_{j-1} | (p_{j-1} & c_{j-1}).
 //
 // A carry tile is CARRY_TILE consecutive limbs, worked by one workgroup of
@@ -75,8 +75,11 @@ __device__ __forceinline__ void carry_tile(const Src &src, uint64_t n_limbs, uin
         y[r] = v;
         gb |= g << r;
         // limbs past the end propagate, so a tile is all-ones exactly when
-        // its limbs inside the result are
-        pb |= (uint32_t)(j >= n_limbs || v == ~0ull) << r;
+        // its limbs inside the result are.  A generating limb never
+        // propagates (its carry out is 1 whatever comes in): a difference's
+        // limb 0 can be 2^64 - 1 with the +1 of the two's complement's carry,
+        // and marking both would hand lane 1 a carry twice.
+        pb |= (uint32_t)(j >= n_limbs || (v == ~0ull && !g)) << r;
     }
     // Rows with no carry into the wave: per lane, the carry it receives (m0)
     // and whether every lane below it propagates (pre); per row, its carry

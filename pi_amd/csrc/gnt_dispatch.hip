@@ -8,7 +8,9 @@
 // (integer.rs:433-444, 552-608).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/gnt.h"
@@ -97,12 +99,28 @@ void add_at(std::vector<u64> &dest, const std::vector<u64> &src, size_t off) {
     }
 }
 
+// Karatsuba's three sub-products run concurrently (z0 on the calling thread,
+// z2 and z1 on two more) while the smaller operand has at least
+// PAR_KARATSUBA limbs, as the reference forks them with rayon::join above
+// parallel_karatsuba_threshold (integer.rs:565-593, config.rs:100), and at
+// most PAR_DEPTH levels deep (3^PAR_DEPTH threads at most).  Concurrent GPU
+// leaves are batched together by the serving loop when it runs; otherwise they
+// take turns on the device while their host work (uploads, sums, schoolbook
+// leaves) overlaps.
+constexpr size_t PAR_KARATSUBA = 512;
+constexpr int PAR_DEPTH = 3;
+
 struct Dispatch {
     gnt_ctx *ctx;
     size_t max_coeffs;
     size_t ntt_thr;
     size_t kara_thr;
-    int err = GNT_OK;
+    std::atomic<int> err{GNT_OK};
+
+    void fail(int rc) {
+        int ok = GNT_OK;
+        err.compare_exchange_strong(ok, rc);
+    }
 
     std::vector<u64> gpu(const u64 *a, size_t na, const u64 *b, size_t nb) {
         std::vector<u64> out(na + nb);
@@ -110,23 +128,34 @@ struct Dispatch {
         int rc = (a == b && na == nb) ? gnt_sqr(ctx, a, na, out.data(), out.size(), &len)
                                       : gnt_mul(ctx, a, na, b, nb, out.data(), out.size(), &len);
         if (rc) {
-            err = rc;
+            fail(rc);
             return {};
         }
         out.resize(len);
         return out;
     }
 
-    std::vector<u64> karatsuba(const u64 *a, size_t na, const u64 *b, size_t nb) {
-        size_t n = na > nb ? na : nb;
-        size_t m = n / 2;
-        size_t la = m < na ? m : na, lb = m < nb ? m : nb;
-        std::vector<u64> z0 = mul(a, la, b, lb);
-        std::vector<u64> z2 = mul(a + la, na - la, b + lb, nb - lb);
-        std::vector<u64> as = add_mag(a, la, a + la, na - la);
-        std::vector<u64> bs = add_mag(b, lb, b + lb, nb - lb);
-        std::vector<u64> z1p = mul(as.data(), as.size(), bs.data(), bs.size());
-        if (err) return {};
+    // integer.rs:552-608
+    std::vector<u64> karatsuba(const u64 *a, size_t na, const u64 *b, size_t nb, int depth) {
+        const size_t n = na > nb ? na : nb;
+        const size_t m = n / 2;
+        const size_t la = m < na ? m : na, lb = m < nb ? m : nb;
+        const std::vector<u64> as = add_mag(a, la, a + la, na - la);
+        const std::vector<u64> bs = add_mag(b, lb, b + lb, nb - lb);
+        std::vector<u64> z0, z2, z1p;
+        const size_t smaller = na < nb ? na : nb;
+        if (smaller >= PAR_KARATSUBA && depth < PAR_DEPTH) {
+            std::thread t2([&] { z2 = mul(a + la, na - la, b + lb, nb - lb, depth + 1); });
+            std::thread t1([&] { z1p = mul(as.data(), as.size(), bs.data(), bs.size(), depth + 1); });
+            z0 = mul(a, la, b, lb, depth + 1);
+            t2.join();
+            t1.join();
+        } else {
+            z0 = mul(a, la, b, lb, depth + 1);
+            z2 = mul(a + la, na - la, b + lb, nb - lb, depth + 1);
+            z1p = mul(as.data(), as.size(), bs.data(), bs.size(), depth + 1);
+        }
+        if (err.load()) return {};
         std::vector<u64> z02 = add_mag(z0.data(), z0.size(), z2.data(), z2.size());
         std::vector<u64> z1 = sub_mag(z1p, z02);
         std::vector<u64> out(na + nb + 1, 0);
@@ -138,13 +167,13 @@ struct Dispatch {
     }
 
     // integer.rs:460-486
-    std::vector<u64> mul(const u64 *a, size_t na, const u64 *b, size_t nb) {
-        if (na == 0 || nb == 0 || err) return {};
+    std::vector<u64> mul(const u64 *a, size_t na, const u64 *b, size_t nb, int depth = 0) {
+        if (na == 0 || nb == 0 || err.load()) return {};
         size_t smaller = na < nb ? na : nb;
-        bool fits = (na + nb) * 4 <= max_coeffs;
+        bool fits = (na + nb) <= max_coeffs / 4;  // 4(na+nb) <= max_coeffs, without overflow
         if (fits && smaller >= ntt_thr) return gpu(a, na, b, nb);
         if (smaller < kara_thr) return schoolbook(a, na, b, nb);
-        return karatsuba(a, na, b, nb);
+        return karatsuba(a, na, b, nb, depth);
     }
 };
 
@@ -152,17 +181,28 @@ struct Dispatch {
 
 }  // namespace gnt
 
-extern "C" int gnt_mul_mag(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
-                           uint64_t *out, size_t out_cap, size_t *out_len, size_t ntt_threshold,
-                           size_t karatsuba_threshold) {
+extern "C" int gnt_mul_mag_dispatch(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                                    uint64_t *out, size_t out_cap, size_t *out_len, size_t ntt_max_coeffs,
+                                    size_t ntt_threshold, size_t karatsuba_threshold) {
     if (!ctx || !out_len) return gnt::set_error(GNT_E_ARG, "null argument");
     if ((na && !a) || (nb && !b)) return gnt::set_error(GNT_E_ARG, "null operand");
-    gnt::Dispatch d{ctx, (size_t)1 << gnt::MAX_LOG_N, ntt_threshold ? ntt_threshold : 8192,
-                    karatsuba_threshold ? karatsuba_threshold : 32};
+    const size_t cap = (size_t)1 << gnt::MAX_LOG_N;
+    if (karatsuba_threshold == 1) return gnt::set_error(GNT_E_ARG, "karatsuba_threshold must be 0 or >= 2");
+    gnt::Dispatch d;
+    d.ctx = ctx;
+    d.max_coeffs = ntt_max_coeffs && ntt_max_coeffs < cap ? ntt_max_coeffs : cap;
+    d.ntt_thr = ntt_threshold ? ntt_threshold : 8192;
+    d.kara_thr = karatsuba_threshold ? karatsuba_threshold : 32;
     std::vector<u64> r = d.mul(a, na, b, nb);
-    if (d.err) return d.err;
+    if (int rc = d.err.load()) return rc;
     if (r.size() > out_cap) return gnt::set_error(GNT_E_ARG, "out_cap too small");
     if (!r.empty()) std::memcpy(out, r.data(), r.size() * 8);
     *out_len = r.size();
     return GNT_OK;
+}
+
+extern "C" int gnt_mul_mag(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                           uint64_t *out, size_t out_cap, size_t *out_len, size_t ntt_threshold,
+                           size_t karatsuba_threshold) {
+    return gnt_mul_mag_dispatch(ctx, a, na, b, nb, out, out_cap, out_len, 0, ntt_threshold, karatsuba_threshold);
 }

@@ -115,14 +115,17 @@ GL_HD int tix(const G &, int n, int c) {
 }
 
 // Global index of tile element (row n, tile column c).  Transforms hold at
-// most 2^MAX_LOG_N elements, so indices fit in 32 bits.
+// most 2^MAX_LOG_N = 2^32 elements, so indices fit in 32 bits.  A pass whose
+// block is the whole 2^32-element array has block index b = 0; its shift count
+// is taken mod 32 (what the hardware shift does anyway) so that the 32 is not
+// an undefined shift.
 GL_HD u32 tile_addr(size_t cg0, int n, int c, int log_m, int log_s) {
     u32 cg = (u32)cg0 + (u32)c;
     u32 b = cg >> log_s;
     u32 j = cg & ((1u << log_s) - 1);
-    return (b << log_m) + ((u32)n << log_s) + j;
+    return (b << (log_m & 31)) + ((u32)n << log_s) + j;
 }
-static_assert(MAX_LOG_N <= 31, "32-bit element indices");
+static_assert(MAX_LOG_N <= 32, "32-bit element indices");
 
 // Tile element index -> (row, column): columns fastest when columns are
 // adjacent in memory (strided passes), rows fastest in the contiguous last pass.

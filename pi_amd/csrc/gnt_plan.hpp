@@ -15,11 +15,13 @@
 
 #include <stdint.h>
 
+#include <atomic>
+
 namespace gnt {
 
 constexpr int TILE_LOG = 13;         // 8192 u64 = 64 KiB of LDS per workgroup
 constexpr int MAX_PASSES = 4;
-constexpr int MAX_LOG_N = 30;        // transform length cap of this library
+constexpr int MAX_LOG_N = 32;        // transform length cap: 4(na+nb) <= 2^32, as ntt_fits (integer.rs:476-478)
 constexpr int ROOT_LOG = 32;         // twiddle tables hold powers of a 2^32-th root
 constexpr int TW_SPLIT = 16;         // two-level table: 2^16 low x 2^16 high
 
@@ -70,10 +72,11 @@ static inline Plan make_plan_split(int log_n, int np, const int *lr) {
     return pl;
 }
 
-// Built-in rule: one pass up to a tile, else 2 or 3 passes with the smaller
-// radices first (strided passes then load longer contiguous column runs).
+// Built-in rule: one pass up to a tile, else 2, 3 or (above 2^30) 4 passes
+// with the smaller radices first (strided passes then load longer contiguous
+// column runs).
 static inline void default_split(int log_n, int *np_out, int *lr) {
-    int np = log_n <= TILE_LOG ? 1 : (log_n <= 20 ? 2 : 3);
+    int np = log_n <= TILE_LOG ? 1 : (log_n <= 20 ? 2 : (log_n <= 30 ? 3 : 4));
     int rem = log_n;
     for (int p = 0; p < np; p++) {
         lr[p] = rem / (np - p);
@@ -96,20 +99,33 @@ static inline bool split_valid(int log_n, int np, const int *lr) {
 }
 
 // Per-length choices installed by the autotuner or a plan cache (host side;
-// npass 0 keeps the built-in rule).
-struct PlanChoice {
-    int npass;
-    int log_r[MAX_PASSES];
-};
-inline PlanChoice g_plan_choice[MAX_LOG_N + 1] = {};
+// 0 keeps the built-in rule).  Each choice is one word (pass count in bits
+// 0-3, radix p in bits 4+4p..7+4p), stored and loaded atomically, so a thread
+// planning a product while another installs a plan never pairs the new pass
+// count with the old radices.
+inline std::atomic<uint32_t> g_plan_word[MAX_LOG_N + 1] = {};
+
+static inline void set_plan_choice(int log_n, int np, const int *lr) {
+    uint32_t w = (uint32_t)np;
+    for (int p = 0; p < np; p++) w |= (uint32_t)lr[p] << (4 + 4 * p);
+    g_plan_word[log_n].store(w, std::memory_order_release);
+}
+// The installed split of log_n into lr[], or 0 passes (built-in rule).
+static inline int get_plan_choice(int log_n, int *lr) {
+    if (log_n < 0 || log_n > MAX_LOG_N) return 0;
+    const uint32_t w = g_plan_word[log_n].load(std::memory_order_acquire);
+    const int np = (int)(w & 15u);
+    for (int p = 0; p < np; p++) lr[p] = (int)((w >> (4 + 4 * p)) & 15u);
+    return np;
+}
 
 // A strided pass whose block size M is at most 2^g_table_max_log takes its
 // twiddles from a precomputed table of M entries per direction (one multiply
 // per element); larger blocks step their twiddles (two multiplies), since
 // their table would be as large as the data.  Host side; -1 disables tables.
-inline int g_table_max_log = 22;
+inline std::atomic<int> g_table_max_log{22};
 static inline bool pass_has_table(const PassDesc &d) {
-    return d.log_m > d.log_r && d.log_m <= g_table_max_log;
+    return d.log_m > d.log_r && d.log_m <= g_table_max_log.load(std::memory_order_relaxed);
 }
 
 // Twiddle tables of a plan's passes (device pointers; null where a pass steps
@@ -121,13 +137,8 @@ struct PassTables {
 
 static inline Plan make_plan(int log_n) {
     int lr[MAX_PASSES] = {0, 0, 0, 0};
-    int np = 0;
-    if (log_n >= 0 && log_n <= MAX_LOG_N && g_plan_choice[log_n].npass) {
-        np = g_plan_choice[log_n].npass;
-        for (int p = 0; p < np; p++) lr[p] = g_plan_choice[log_n].log_r[p];
-    } else {
-        default_split(log_n, &np, lr);
-    }
+    int np = get_plan_choice(log_n, lr);
+    if (!np) default_split(log_n, &np, lr);
     return make_plan_split(log_n, np, lr);
 }
 

@@ -587,7 +587,14 @@ int gnt_serve_stop(gnt_ctx *ctx) {
     s->wake_lanes();
     for (gnt::Lane &l : s->lanes) l.th.join();
     for (gnt::Lane &l : s->lanes) l.destroy();
-    for (auto &kv : s->tickets) delete kv.second;  // unclaimed results stay owned by their gnt_buf
+    // results never claimed by gnt_wait have no gnt_buf: their buffers go back
+    // to the pool here (the lanes have drained, so every request is done)
+    for (auto &kv : s->tickets) {
+        Req *q = kv.second;
+        if (q->p.dout) c->pool.release(q->p.dout, q->p.nl * 8);
+        if (q->p.dlen) c->pool.release(q->p.dlen, 4);
+        delete q;
+    }
     delete s;
     c->server = nullptr;
     return GNT_OK;

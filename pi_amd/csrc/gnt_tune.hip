@@ -30,7 +30,7 @@ using u64 = uint64_t;
 namespace gnt {
 namespace {
 
-std::mutex g_plan_mu;  // writers of g_plan_choice
+std::mutex g_plan_mu;  // serialises writers of the plan words and the table limit
 
 void compositions(int log_n, int np, std::vector<int> &cur, std::vector<std::vector<int>> &out) {
     if ((int)cur.size() == np) {
@@ -61,10 +61,7 @@ std::vector<std::vector<int>> candidates(int log_n) {
 
 void install(int log_n, int np, const int *lr) {
     std::lock_guard<std::mutex> g(g_plan_mu);
-    PlanChoice c{};
-    c.npass = np;
-    for (int p = 0; p < np; p++) c.log_r[p] = lr[p];
-    g_plan_choice[log_n] = c;
+    set_plan_choice(log_n, np, lr);
 }
 
 std::string key(const std::vector<int> &v) {
@@ -83,8 +80,8 @@ extern "C" {
 int gnt_set_pass_tables(int max_log_m, int *previous) {
     if (max_log_m < -1 || max_log_m > gnt::MAX_LOG_N) return gnt::set_error(GNT_E_ARG, "max_log_m out of range");
     std::lock_guard<std::mutex> g(gnt::g_plan_mu);
-    if (previous) *previous = gnt::g_table_max_log;
-    gnt::g_table_max_log = max_log_m;
+    if (previous) *previous = gnt::g_table_max_log.load();
+    gnt::g_table_max_log.store(max_log_m);
     return GNT_OK;
 }
 
@@ -104,7 +101,7 @@ int gnt_set_plan(int log_n, int npass, const int *log_r) {
     if (log_n < 1 || log_n > gnt::MAX_LOG_N) return gnt::set_error(GNT_E_ARG, "log_n out of range");
     if (npass == 0) {  // back to the built-in rule
         std::lock_guard<std::mutex> g(gnt::g_plan_mu);
-        gnt::g_plan_choice[log_n] = gnt::PlanChoice{};
+        gnt::set_plan_choice(log_n, 0, nullptr);
         return GNT_OK;
     }
     if (!log_r || !gnt::split_valid(log_n, npass, log_r))

@@ -31,7 +31,7 @@ GNT_E_STATE = -6
 
 # Every entry point include/gnt.h declares (checked by the ABI test).
 EXPORTS = (
-    "gnt_version", "gnt_last_error", "gnt_init", "gnt_shutdown", "gnt_mul", "gnt_sqr", "gnt_mul_mag",
+    "gnt_version", "gnt_last_error", "gnt_init", "gnt_shutdown", "gnt_mul", "gnt_sqr", "gnt_mul_mag", "gnt_mul_mag_dispatch",
     "gnt_buf_upload", "gnt_buf_download", "gnt_buf_len", "gnt_buf_release", "gnt_mul_buf", "gnt_sqr_buf",
     "gnt_sync", "gnt_mul_batch", "gnt_mul_batch_serial", "gnt_plan_info", "gnt_set_timing", "gnt_get_timing",
     "gnt_batch_create", "gnt_batch_upload", "gnt_batch_run", "gnt_batch_download", "gnt_batch_info",
@@ -80,6 +80,7 @@ def load() -> ctypes.CDLL:
     L.gnt_mul.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PU64, SZ, PSZ]
     L.gnt_sqr.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PSZ]
     L.gnt_mul_mag.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PU64, SZ, PSZ, SZ, SZ]
+    L.gnt_mul_mag_dispatch.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PU64, SZ, PSZ, SZ, SZ, SZ]
     L.gnt_buf_upload.argtypes = [ctypes.c_void_p, PU64, SZ, ctypes.POINTER(ctypes.c_void_p)]
     L.gnt_buf_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, PU64, SZ, PSZ]
     L.gnt_buf_len.argtypes = [ctypes.c_void_p, ctypes.c_void_p, PSZ]
@@ -203,13 +204,16 @@ class Context:
         _check(load().gnt_sqr(self.h, _p(x), x.size, _p(out), out.size, ctypes.byref(n)))
         return out[:n.value].copy()
 
-    def mul_mag(self, a, b, ntt_threshold: int = 0, karatsuba_threshold: int = 0) -> np.ndarray:
-        """mul_mag / mul_mag_dispatch (integer.rs:445-486) with the GPU leaf."""
+    def mul_mag(self, a, b, ntt_threshold: int = 0, karatsuba_threshold: int = 0,
+                ntt_max_coeffs: int = 0) -> np.ndarray:
+        """mul_mag / mul_mag_dispatch (integer.rs:445-486) with the GPU leaf;
+        ntt_max_coeffs (0: the library cap 2^32) is the dispatch's transform
+        cap, which the reference's tests lower to force Karatsuba above it."""
         x, y = _u64(a), _u64(b)
         out = np.zeros(x.size + y.size + 1, dtype=np.uint64)
         n = SZ()
-        _check(load().gnt_mul_mag(self.h, _p(x), x.size, _p(y), y.size, _p(out), out.size, ctypes.byref(n),
-                                  ntt_threshold, karatsuba_threshold))
+        _check(load().gnt_mul_mag_dispatch(self.h, _p(x), x.size, _p(y), y.size, _p(out), out.size, ctypes.byref(n),
+                                           ntt_max_coeffs, ntt_threshold, karatsuba_threshold))
         return out[:n.value].copy()
 
     # ---- device-resident ----

@@ -175,14 +175,11 @@ u64 emu_root(int log_n) {
 // Install a pass split for one length in this emulation (npass 0: built-in rule).
 int emu_set_plan(int log_n, int npass, const int *log_r) {
     if (npass == 0) {
-        g_plan_choice[log_n] = PlanChoice{};
+        set_plan_choice(log_n, 0, nullptr);
         return 0;
     }
     if (!split_valid(log_n, npass, log_r)) return -1;
-    PlanChoice c{};
-    c.npass = npass;
-    for (int p = 0; p < npass; p++) c.log_r[p] = log_r[p];
-    g_plan_choice[log_n] = c;
+    set_plan_choice(log_n, npass, log_r);
     return 0;
 }
 
@@ -193,7 +190,7 @@ u64 emu_pass_table_entry(u32 i, int log_m, int log_r, int inv) {
 }
 
 // Largest pass block size (log) that takes its twiddles from a table; -1: none.
-void emu_set_table_max_log(int max_log) { g_table_max_log = max_log; }
+void emu_set_table_max_log(int max_log) { g_table_max_log.store(max_log); }
 
 void emu_plan(int log_n, int *npass, int *log_r, int *log_c) {
     Plan pl = make_plan(log_n);

@@ -47,8 +47,11 @@ def test_plan_info_and_errors_without_gpu():
 
     log_n, radices = gnt.plan_info(1 << 24, 1 << 24)
     assert log_n == 27 and sum(radices) == 27
+    # the cap is the reference's: 4(na+nb) <= 2^32 (integer.rs:476-478)
+    assert gnt.plan_info(1 << 29, 1 << 29) == (32, [8, 8, 8, 8])
+    assert gnt.plan_info((1 << 28) - 7, 1 << 28) == (31, [7, 8, 8, 8])
     with pytest.raises(gnt.GntError) as e:
-        gnt.plan_info(1 << 29, 1 << 29)
+        gnt.plan_info((1 << 29) + 1, 1 << 29)
     assert e.value.code == gnt.GNT_E_TOO_LARGE
     lib = gnt.load()
     assert lib.gnt_version().startswith(b"gnt")

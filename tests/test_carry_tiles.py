@@ -35,7 +35,7 @@ def _limb_values(mode, n_limbs, A, B, j):
     if mode == SUB and j == 0:
         s = (s + 1) & MASK
         g |= int(s == 0)
-    return s, g, s == MASK
+    return s, g, s == MASK and not g
 
 
 def row_carries(G, P, cin):
@@ -186,10 +186,15 @@ def _cases():
         ("sub", X, X),
         ("sub", X, rnd.getrandbits(64 * 40)),
         ("sub", 7, 5),
+        # limb 0 of a - b is 2^64 - 1 with a carry out (the two's complement's
+        # +1): generating, so it must not also propagate
+        ("sub", (1 << 128) - 1, 1 << 64),
+        ("sub", MASK | (0xDEADBEEF << 64), 0x1234 << 64),
+        ("sub", MASK | (X << 64), (X - 1) << 64),
     ]
 
 
-@pytest.mark.parametrize("i", range(9))
+@pytest.mark.parametrize("i", range(12))
 def test_add_sub_tiles(i):
     op, a, b = _cases()[i]
     got, length = run_add(a, b, op == "sub")

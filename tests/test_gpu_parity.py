@@ -96,7 +96,23 @@ def test_unbalanced(ctx, orc, la, lb):
 
 
 def test_every_transform_length(ctx, orc):
-    """One product per transform length 2^3 .. 2^24 (every plan shape)."""
+    """One product per transform length 2^3 .. 2^24 under the tuned plans that
+    bench.py and pi_compute load (their splits asserted)."""
+    from pi_amd import gnt
+    from conftest import tuned_plans
+    for log_n, sp in tuned_plans().items():
+        assert gnt.plan(log_n) == (len(sp), sp), log_n
+    _every_length(ctx, orc)
+
+
+def test_every_transform_length_builtin_plans(ctx, orc, builtin_plans):
+    """The same with the library's built-in splits (no plan cache)."""
+    from pi_amd import gnt
+    assert gnt.plan(27) == (3, [9, 9, 9])
+    _every_length(ctx, orc)
+
+
+def _every_length(ctx, orc):
     for log_n in range(3, 25):
         # na + nb = 2^(log_n-2) gives n_out = 2^log_n - 1
         tot = 1 << (log_n - 2) if log_n >= 3 else 2
@@ -186,7 +202,11 @@ def _fingerprints_ok(orc, c, a, b):
 def test_2p24_distributions(ctx, orc, kind):
     """2^24 x 2^24 (N = 2^27) and the square of a: GMP when present, residue
     fingerprints and the low limb always; the closed form for all-ones."""
+    from pi_amd import gnt
+    from conftest import tuned_plans
     n = 1 << 24
+    # the benched configuration: the tuned split of N = 2^27
+    assert gnt.plan(27) == (3, tuned_plans()[27]) == (3, [8, 9, 10])
     log(f"2^24 {kind}")
     a, b = _dist(orc, kind, n, 5), _dist(orc, kind, n, 6)
     c = ctx.mul_mag_ntt(a, b)
@@ -206,13 +226,20 @@ def test_2p24_distributions(ctx, orc, kind):
         assert np.array_equal(s, orc.gmp_sqr(a))
 
 
+def _ones_product_ok(c, m, n):
+    """(B^m - 1)(B^n - 1) = B^(m+n) - B^m - B^n + 1 for m >= n."""
+    return (c.size == m + n and c[0] == 1 and not c[1:n].any()
+            and (c[n:m] == np.uint64(2**64 - 1)).all() and c[m] == np.uint64(2**64 - 2)
+            and (c[m + 1:] == np.uint64(2**64 - 1)).all())
+
+
 def test_transform_lengths_up_to_the_cap(ctx, orc):
     """One product and one square per transform length 2^25 .. 2^30, each with
-    4(na+nb) - 1 just under the power of two; 2^30 is the cap
-    (4(na+nb) <= 2^30).  Length, low limb and residue fingerprints (GMP at these
-    sizes would take minutes); at the cap also the all-ones closed form, and one
-    limb more is refused with GNT_E_TOO_LARGE, as ntt_fits refuses it
-    (integer.rs:476-478)."""
+    4(na+nb) - 1 just under the power of two: length, low limb and residue
+    fingerprints (GMP at these sizes would take minutes).  Then 2^31 and the cap
+    2^32 (4(na+nb) <= 2^32, as ntt_fits, integer.rs:476-478), four-pass plans,
+    by the all-ones closed form, and one limb more is refused with
+    GNT_E_TOO_LARGE."""
     from pi_amd import gnt
     for log_n in range(25, 31):
         tot = 1 << (log_n - 2)
@@ -228,19 +255,18 @@ def test_transform_lengths_up_to_the_cap(ctx, orc):
         s = ctx.sqr(a[: tot // 2])
         assert _fingerprints_ok(orc, s, a[: tot // 2], a[: tot // 2]), log_n
         del s, a, b
-    # all-ones at the cap: (B^m - 1)(B^n - 1) = B^(m+n) - B^m - B^n + 1
-    m, n = (1 << 27) + 5, (1 << 27) - 5
-    log(f"all-ones {m} x {n}")
-    ones = np.full(m, np.uint64(2**64 - 1))
-    c = ctx.mul_mag_ntt(ones, ones[:n])
-    assert c.size == m + n
-    assert c[0] == 1 and not c[1:n].any()
-    assert c[n] == np.uint64(2**64 - 1) and (c[n:m] == np.uint64(2**64 - 1)).all()
-    assert c[m] == np.uint64(2**64 - 2) and (c[m + 1:] == np.uint64(2**64 - 1)).all()
-    del c
+    for log_n in (31, 32):
+        m, n = (1 << (log_n - 3)) + 5, (1 << (log_n - 3)) - 5
+        assert gnt.plan_info(m, n)[0] == log_n and len(gnt.plan_info(m, n)[1]) == 4
+        log(f"all-ones {m} x {n} (log_n={log_n})")
+        ones = np.full(m, np.uint64(2**64 - 1))
+        c = ctx.mul_mag_ntt(ones, ones[:n])
+        assert _ones_product_ok(c, m, n), log_n
+        del c
     with pytest.raises(gnt.GntError) as e:
         ctx.mul_mag_ntt(ones, ones[: n + 1])
     assert e.value.code == gnt.GNT_E_TOO_LARGE
+    del ones
 
 
 # ---- device-resident operands, batches, dispatch ----
@@ -279,6 +305,37 @@ def test_mul_mag_dispatch(ctx, orc, la, lb):
     check(orc, ctx.mul_mag(a, b), a, b)
     # a low NTT threshold routes the Karatsuba leaves to the GPU too
     check(orc, ctx.mul_mag(a, b, ntt_threshold=64), a, b)
+
+
+def test_karatsuba_above_the_transform_cap(ctx, orc):
+    """The reference's karatsuba_above_ntt_routes_correctly_on_oversize
+    (integer.rs:1727-1754) with its inputs: a cap of 100 coefficients forces
+    Karatsuba on 200-limb operands, a large cap takes the transform; both equal
+    schoolbook.  Then GPU leaves under an injected cap: 2^17 x 2^17 limbs with
+    4(na+nb) <= 2^17 per leaf split into Karatsuba levels whose concurrent
+    sub-products (>= 512 limbs) are GPU transforms, unbalanced operands, and the
+    same with the serving loop batching the concurrent leaves."""
+    from pi_amd import gnt
+    M = 1 << 64
+    a = np.array([(0xDEADBEEFCAFE0001 * (i + 1)) % M for i in range(200)], dtype=np.uint64)
+    b = np.array([(0xFACEF00D12345678 * (i + 7)) % M for i in range(200)], dtype=np.uint64)
+    want = orc.schoolbook(a, b)
+    assert np.array_equal(ctx.mul_mag(a, b, ntt_threshold=50, karatsuba_threshold=32, ntt_max_coeffs=100), want)
+    assert np.array_equal(ctx.mul_mag(a, b, ntt_threshold=50, karatsuba_threshold=32, ntt_max_coeffs=1 << 32), want)
+    cases = [(1 << 17, 1 << 17, 1 << 17), ((1 << 17) + 999, 70001, 1 << 16), (3 << 15, 5000, 1 << 15)]
+    for la, lb, cap in cases:
+        x, y = orc.splitmix64(la + 5, la), orc.splitmix64(lb + 6, lb)
+        log(f"karatsuba above cap {cap}: {la} x {lb}")
+        check(orc, ctx.mul_mag(x, y, ntt_threshold=1024, ntt_max_coeffs=cap), x, y)
+    c2 = gnt.Context(0)
+    srv = gnt.Server(c2)
+    try:
+        x, y = orc.splitmix64(77, 1 << 17), orc.splitmix64(78, (1 << 17) - 3)
+        check(orc, c2.mul_mag(x, y, ntt_threshold=1024, ntt_max_coeffs=1 << 16), x, y)
+        assert srv.stats()["completed"] >= 9          # the leaves went through the loop
+    finally:
+        srv.stop()
+        c2.close()
 
 
 # ---- batch plans: level-grouped launches, hipGraph capture and replay ----
@@ -511,6 +568,11 @@ def test_serving_add_sub(ctx, orc):
         ("sub", X, X),                                      # zero
         ("add", 5, 7),
         ("sub", 7, 5),
+        # limb 0 of a - b is 2^64 - 1 and carries out (the two's complement's
+        # +1): it generates and must not propagate as well
+        ("sub", (1 << 128) - 1, 1 << 64),
+        ("sub", (2**64 - 1) | (0xDEADBEEF << 64), 0x1234 << 64),
+        ("sub", (2**64 - 1) | (X << 64), (X - 1) << 64),
     ]
     srv = gnt.Server(ctx)
     try:

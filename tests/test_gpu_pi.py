@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PI = os.path.join(ROOT, "pi_amd", "pi_compute")
 GOLD = os.path.join(ROOT, "tests", "golden")
+PLAN_CACHE = os.path.join(ROOT, "pi_amd", "plans_gfx950.json")
 
 
 def _digest(path):
@@ -28,7 +29,9 @@ def _digest(path):
 def _run(digits, tmp_path, *extra):
     out = tmp_path / f"pi_{digits}.txt"
     print(f"pi_compute {digits} {' '.join(extra)}", flush=True)
-    r = subprocess.run([PI, str(digits), str(out), *extra], capture_output=True, text=True, timeout=300)
+    # the tuned plan cache, as the measured runs load it
+    r = subprocess.run([PI, str(digits), str(out), "--plans", PLAN_CACHE, *extra], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stderr
     stats = json.loads(r.stdout.strip().splitlines()[-1])
     return out, stats
@@ -64,3 +67,15 @@ def test_gauss_legendre_gpu(tmp_path, digits, fa_min):
     out, st = _run(digits, tmp_path, "--algorithm", "gauss-legendre", "--fa-min-limbs", fa_min)
     assert st["products"] == "gpu" and st["gpu_products"] > 0
     assert _digest(out) == gold[str(digits)]["sha256"]
+
+
+@pytest.mark.parametrize("algorithm", ["chudnovsky", "gauss-legendre"])
+def test_pi_1e8(tmp_path, algorithm):
+    """10^8 digits (7,051,375 series terms for Chudnovsky; 27 AGM iterations at
+    full precision for Gauss-Legendre) with the default thresholds, byte for
+    byte against the pinned SHA-256."""
+    gold = json.load(open(os.path.join(GOLD, "pi_digits.json")))["digits"]["100000000"]
+    out, st = _run(100000000, tmp_path, "--algorithm", algorithm)
+    assert os.path.getsize(out) == gold["bytes"]
+    assert _digest(out) == gold["sha256"]
+    print(json.dumps({k: st[k] for k in st if k in ("wall_s", "total_s", "merge", "products")}), flush=True)

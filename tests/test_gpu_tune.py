@@ -6,6 +6,8 @@ import json
 import numpy as np
 import pytest
 
+from conftest import PLAN_CACHE, tuned_plans
+
 pytestmark = pytest.mark.gpu
 
 
@@ -33,7 +35,25 @@ def test_every_split_exact(ctx, orc, log_n):
             assert np.array_equal(ctx.mul_mag_ntt(a, b), want), sp
             assert np.array_equal(ctx.sqr(a), want_sq), sp
     finally:
-        gnt.set_plan(log_n)
+        gnt.load_plans(PLAN_CACHE)
+
+
+@pytest.mark.parametrize("log_n", range(11, 29))
+def test_tuned_lengths_exact(ctx, orc, log_n):
+    """Every length of the committed plan cache under its tuned split, as
+    bench.py and pi_compute run it: one product with 4(na+nb) - 1 just under
+    2^log_n (unbalanced) and one square, exact against GMP."""
+    from pi_amd import gnt
+    sp = tuned_plans()[log_n]
+    assert gnt.plan(log_n) == (len(sp), sp)
+    tot = 1 << (log_n - 2)
+    na = tot // 2 + tot // 8
+    a, b = orc.splitmix64(900 + log_n, na), orc.splitmix64(950 + log_n, tot - na)
+    assert gnt.plan_info(na, tot - na) == (log_n, sp)
+    print(f"log_n {log_n} split {sp} na {na} nb {tot - na}", flush=True)
+    assert np.array_equal(ctx.mul_mag_ntt(a, b), orc.gmp_mul(a, b)), sp
+    s = a[: tot // 2]
+    assert np.array_equal(ctx.sqr(s), orc.gmp_sqr(s)), sp
 
 
 def test_tune_writes_loadable_cache(ctx, orc, tmp_path):
@@ -59,8 +79,7 @@ def test_tune_writes_loadable_cache(ctx, orc, tmp_path):
         finally:
             c2.close()
     finally:
-        for n in range(12, 19):
-            gnt.set_plan(n)
+        gnt.load_plans(PLAN_CACHE)
 
 
 @pytest.mark.parametrize("max_log", [-1, 14, 30])

@@ -250,7 +250,7 @@ def test_karatsuba_and_dispatch(orc):
     want = orc.to_int(a) * orc.to_int(b)
     assert orc.to_int(orc.mul_mag_dispatch(a, b, 1 << 14, 8192, 32)) == want
     assert orc.to_int(orc.mul_mag_dispatch(a, b, 1 << 32, 8192, 32)) == want
-    # guard formula: combined limbs above 2^30 do not fit one transform
+    # guard formula: combined limbs above 2^30 (4(na+nb) > 2^32) do not fit one transform
     lim = 1 << 32
     assert (lim // 4 + 1) * 4 > lim and (lim // 4 - 1) * 4 <= lim
 
