@@ -55,8 +55,8 @@ int gnt_shutdown(gnt_ctx *ctx);
  * gnt_mul replaces ntt::mul_mag_ntt (ntt.rs:806-865): out receives |a|*|b|
  * trimmed of most-significant zero limbs (ntt.rs:712-715, 781-783);
  * out_cap >= na+nb.  na == 0 or nb == 0 gives *out_len = 0 (ntt.rs:807-809).
- * GNT_E_TOO_LARGE when 4*(na+nb) > 2^30 coefficients (the reference's cap is
- * 2^32, integer.rs:476-478; the caller splits with Karatsuba either way).
+ * GNT_E_TOO_LARGE when 4*(na+nb) > 2^32 coefficients, the reference's cap
+ * (integer.rs:476-478); the caller splits with Karatsuba (gnt_mul_mag does).
  * Thread-safe; while the serving loop runs (gnt_serve_start), calls from any
  * thread are submitted to it, so concurrent callers are batched together. */
 int gnt_mul(gnt_ctx *ctx, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out,
@@ -96,6 +96,24 @@ int gnt_mul_buf(gnt_ctx *ctx, gnt_buf *a, gnt_buf *b, gnt_buf **out);
 int gnt_sqr_buf(gnt_ctx *ctx, gnt_buf *a, gnt_buf **out);
 /* Wait until all work queued on the context's stream has finished. */
 int gnt_sync(gnt_ctx *ctx);
+
+/* gnt_mul_buf's product for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Operands d_a (na limbs) and d_b (nb limbs;
+ * NULL squares d_a), the output d_out (out_cap >= na + nb limbs) and the word
+ * d_len that receives the trimmed length are caller-owned device memory.  The
+ * product's scratch (coefficient arrays, limb sums, carry flags) is a static
+ * device array of GNT_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends, so products are limited to 4(na+nb) <= 2^GNT_NOALLOC_MAX_LOG_N
+ * (GNT_E_TOO_LARGE above).  Stream-ordered on `stream` (a hipStream_t; NULL =
+ * the context's stream); returns once the work is enqueued.  No hipMalloc,
+ * hipFree or other allocating call; every strided pass steps its twiddles
+ * (no twiddle tables).  All calls on one GPU share its scratch array: calls
+ * whose work overlaps on the device race. */
+#define GNT_NOALLOC_MAX_LOG_N 22
+#define GNT_NOALLOC_SCRATCH_BYTES 83887104 /* 2 x 32 MiB coefficients + 16 MiB limb sums + 1 KiB flags */
+int gnt_mul_noalloc(gnt_ctx *ctx, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, uint64_t *d_out,
+                    size_t out_cap, uint32_t *d_len, void *stream);
 
 /* ---- batched products (within a binary-splitting node the reference runs its
  * four products one after another, chudnovsky.rs:411-413; independent

@@ -238,6 +238,53 @@ bool plan_for(size_t na, size_t nb, Plan &pl) {
     return true;
 }
 
+// Device scratch of one product: two coefficient arrays (one for a square),
+// the limb sums and the carry flags.
+ProductScratch product_scratch_layout(size_t na, size_t nb, bool sqr, int log_n) {
+    ProductScratch s;
+    s.N = (size_t)1 << log_n;
+    s.n_limbs = na + nb;
+    s.nt = carry_tiles(s.n_limbs);
+    s.sqr = sqr;
+    return s;
+}
+
+// The launch sequence of one product on `st`, all buffers given: forward
+// passes, fused last pass + pointwise + first inverse pass, remaining inverse
+// passes, carry.  ev (if not null) receives 5 phase events.
+int launch_product(const Plan &pl, const PassTables &tabs, const Tw &tw, const u64 *da, size_t na, const u64 *db,
+                   size_t nb, u64 *dout, u32 *dlen, const ProductScratch &s, u64 *xa, u64 *xb, u64 *lh,
+                   unsigned char *gp, hipStream_t st, hipEvent_t *ev) {
+    const bool sqr = s.sqr;
+    const size_t n_out = 4 * (na + nb) - 1;
+    u64 *Ls = lh, *Hs = lh + s.n_limbs;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    const int P = pl.npass;
+    const bool ab = !sqr && P > 1;  // both last forward passes in the fused kernel
+    int rc = 0;
+    if (sqr) {
+        if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, tw, tabs, st);
+    } else if (ab) {
+        rc = launch_forward_pair(pl, P - 1, xa, da, na, xb, db, nb, tw, tabs, st);
+    } else {
+        rc = launch_forward(pl, P, xa, da, na, tw, tabs, st);
+    }
+    if (ev) (void)hipEventRecord(ev[1], st);
+    if (!rc && ab) {
+        rc = launch_last_fused_ab(pl, xa, xb, tw, tabs, st);
+    } else if (!rc) {
+        const u64 *bsrc = sqr ? xa : xb;
+        const u64 *blimbs = sqr ? da : db;
+        rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, tw, tabs, Ls, Hs, n_out, s.n_limbs, st);
+    }
+    if (ev) (void)hipEventRecord(ev[2], st);
+    if (!rc) rc = launch_inverse_rest(pl, xa, tw, tabs, Ls, Hs, n_out, s.n_limbs, st);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    if (!rc) rc = launch_carry(Ls, Hs, dout, s.n_limbs, gp, gp + s.nt, dlen, st);
+    if (ev) (void)hipEventRecord(ev[4], st);
+    return rc;
+}
+
 // Enqueue one product of device limb arrays on ctx->stream.  b == nullptr
 // squares a.  out holds na+nb limbs; *len receives the trimmed length.
 int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64 *dout, u32 *dlen) {
@@ -247,54 +294,26 @@ int Ctx::enqueue_product(const u64 *da, size_t na, const u64 *db, size_t nb, u64
     if (!plan_for(na, nb, pl)) return set_error(GNT_E_TOO_LARGE, "product exceeds the transform cap");
     PassTables tabs;
     pass_tables(pl, tabs);
-    const size_t N = (size_t)1 << pl.log_n;
-    const size_t n_out = 4 * (na + nb) - 1;
-    const size_t n_limbs = na + nb;
-    const size_t nt = carry_tiles(n_limbs);
-    u64 *xa = (u64 *)pool.acquire(N * sizeof(u64));
-    u64 *xb = sqr ? nullptr : (u64 *)pool.acquire(N * sizeof(u64));
-    u64 *lh = (u64 *)pool.acquire(2 * n_limbs * sizeof(u64));
-    unsigned char *gp = (unsigned char *)pool.acquire(2 * nt);
+    const ProductScratch s = product_scratch_layout(na, nb, sqr, pl.log_n);
+    u64 *xa = (u64 *)pool.acquire(s.N * sizeof(u64));
+    u64 *xb = sqr ? nullptr : (u64 *)pool.acquire(s.N * sizeof(u64));
+    u64 *lh = (u64 *)pool.acquire(2 * s.n_limbs * sizeof(u64));
+    unsigned char *gp = (unsigned char *)pool.acquire(2 * s.nt);
     auto give_back = [&]() {
-        pool.release(xa, N * sizeof(u64));
-        pool.release(xb, N * sizeof(u64));
-        pool.release(lh, 2 * n_limbs * sizeof(u64));
-        pool.release(gp, 2 * nt);
+        pool.release(xa, s.N * sizeof(u64));
+        pool.release(xb, s.N * sizeof(u64));
+        pool.release(lh, 2 * s.n_limbs * sizeof(u64));
+        pool.release(gp, 2 * s.nt);
     };
     if (!xa || (!sqr && !xb) || !lh || !gp) {
         give_back();
         return set_error(GNT_E_NOMEM, "device memory exhausted");
     }
-    u64 *Ls = lh, *Hs = lh + n_limbs;
     hipEvent_t ev[5];
-    if (timing) {
+    if (timing)
         for (auto &e : ev) (void)hipEventCreate(&e);
-        (void)hipEventRecord(ev[0], stream);
-    }
-    const int P = pl.npass;
-    const bool ab = !sqr && P > 1;  // both last forward passes in the fused kernel
-    int rc = 0;
-    if (sqr) {
-        if (P > 1) rc = launch_forward(pl, P - 1, xa, da, na, tw(), tabs, stream);
-    } else if (ab) {
-        rc = launch_forward_pair(pl, P - 1, xa, da, na, xb, db, nb, tw(), tabs, stream);
-    } else {
-        rc = launch_forward(pl, P, xa, da, na, tw(), tabs, stream);
-    }
-    if (timing) (void)hipEventRecord(ev[1], stream);
-    if (!rc && ab) {
-        rc = launch_last_fused_ab(pl, xa, xb, tw(), tabs, stream);
-    } else if (!rc) {
-        const u64 *bsrc = sqr ? xa : xb;
-        const u64 *blimbs = sqr ? da : db;
-        rc = launch_last_fused(pl, bsrc, blimbs, sqr ? na : nb, xa, sqr, tw(), tabs, Ls, Hs, n_out, n_limbs,
-                               stream);
-    }
-    if (timing) (void)hipEventRecord(ev[2], stream);
-    if (!rc) rc = launch_inverse_rest(pl, xa, tw(), tabs, Ls, Hs, n_out, n_limbs, stream);
-    if (timing) (void)hipEventRecord(ev[3], stream);
-    if (!rc) rc = launch_carry(Ls, Hs, dout, n_limbs, gp, gp + nt, dlen, stream);
-    if (timing) (void)hipEventRecord(ev[4], stream);
+    const int rc = launch_product(pl, tabs, tw(), da, na, db, nb, dout, dlen, s, xa, xb, lh, gp, stream,
+                                  timing ? ev : nullptr);
     give_back();
     if (timing) {
         (void)hipEventSynchronize(ev[4]);
@@ -375,6 +394,9 @@ int gnt_init(gnt_ctx **out, const gnt_config *cfg) {
         return fail(GNT_E_DEVICE, "stream creation failed");
     }
     int rc = c->init_tables();
+    // the no-allocation entry point's scratch address on this device; without
+    // it only gnt_mul_noalloc fails (with GNT_E_DEVICE)
+    if (!rc) (void)gnt::noalloc_setup(c);
     if (!rc && cfg && cfg->plan_cache_path && *cfg->plan_cache_path) rc = gnt_load_plans(cfg->plan_cache_path);
     if (rc) {
         (void)hipStreamDestroy(c->stream);

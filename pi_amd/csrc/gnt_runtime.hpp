@@ -61,6 +61,7 @@ struct Ctx {
     double phase_ms[5] = {0, 0, 0, 0, 0};
     uint64_t products = 0;
     void *server = nullptr;  // serving loop (gnt_serve.hip) while running
+    uint64_t *noalloc_scratch = nullptr;  // this device's gnt_mul_noalloc scratch array (device address)
 
     int init_tables();
     // Twiddle tables of pl's strided passes that have one (pass_has_table),
@@ -75,6 +76,20 @@ struct Ctx {
 };
 
 bool plan_for(size_t na, size_t nb, Plan &pl);
+
+// Device scratch of one product (sizes in elements): coefficient arrays xa,
+// xb (N u64 each; xb unused for a square), limb sums (2 n_limbs u64), carry
+// flags (2 nt bytes).
+struct ProductScratch {
+    size_t N = 0, n_limbs = 0, nt = 0;
+    bool sqr = false;
+};
+ProductScratch product_scratch_layout(size_t na, size_t nb, bool sqr, int log_n);
+int launch_product(const Plan &pl, const PassTables &tabs, const Tw &tw, const uint64_t *da, size_t na,
+                   const uint64_t *db, size_t nb, uint64_t *dout, uint32_t *dlen, const ProductScratch &s,
+                   uint64_t *xa, uint64_t *xb, uint64_t *lh, unsigned char *gp, hipStream_t st, hipEvent_t *ev);
 int run_batch(Ctx *c, gnt_job *jobs, size_t n);
+// Looks up this device's gnt_mul_noalloc scratch address (device current).
+int noalloc_setup(Ctx *c);
 
 }  // namespace gnt

@@ -31,7 +31,8 @@ GNT_E_STATE = -6
 
 # Every entry point include/gnt.h declares (checked by the ABI test).
 EXPORTS = (
-    "gnt_version", "gnt_last_error", "gnt_init", "gnt_shutdown", "gnt_mul", "gnt_sqr", "gnt_mul_mag", "gnt_mul_mag_dispatch",
+    "gnt_version", "gnt_last_error", "gnt_init", "gnt_shutdown", "gnt_mul", "gnt_sqr", "gnt_mul_mag",
+    "gnt_mul_mag_dispatch", "gnt_mul_noalloc",
     "gnt_buf_upload", "gnt_buf_download", "gnt_buf_len", "gnt_buf_release", "gnt_mul_buf", "gnt_sqr_buf",
     "gnt_sync", "gnt_mul_batch", "gnt_mul_batch_serial", "gnt_plan_info", "gnt_set_timing", "gnt_get_timing",
     "gnt_batch_create", "gnt_batch_upload", "gnt_batch_run", "gnt_batch_download", "gnt_batch_info",
@@ -81,6 +82,8 @@ def load() -> ctypes.CDLL:
     L.gnt_sqr.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PSZ]
     L.gnt_mul_mag.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PU64, SZ, PSZ, SZ, SZ]
     L.gnt_mul_mag_dispatch.argtypes = [ctypes.c_void_p, PU64, SZ, PU64, SZ, PU64, SZ, PSZ, SZ, SZ, SZ]
+    L.gnt_mul_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, SZ, ctypes.c_void_p, SZ, ctypes.c_void_p, SZ,
+                                  ctypes.c_void_p, ctypes.c_void_p]
     L.gnt_buf_upload.argtypes = [ctypes.c_void_p, PU64, SZ, ctypes.POINTER(ctypes.c_void_p)]
     L.gnt_buf_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, PU64, SZ, PSZ]
     L.gnt_buf_len.argtypes = [ctypes.c_void_p, ctypes.c_void_p, PSZ]
