@@ -31,11 +31,23 @@ constexpr int TWAVES = TBLOCK / 64 * 2 / 4;  // waves per SIMD at two tiles per 
 using X = DevExec<TBLOCK>;
 #define GNT_TBOUNDS __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(TWAVES, TWAVES)))
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup w to XCD
+// w mod 8), each XCD with its own L2.  A packing pass reads 2^(log_c - 2)
+// limbs per tile row, half a 128-byte line when log_c = 5, and the other half
+// belongs to the neighbouring tile: within every 16 workgroups this mapping
+// gives tiles 2m and 2m+1 to workgroups w and w + 8, which run on the same XCD
+// at about the same time, so the line is fetched from HBM once.
+__device__ __forceinline__ unsigned xcd_paired_tile(unsigned w, unsigned tiles) {
+    if (tiles & 15u) return w;
+    return (w & ~15u) | ((w & 7u) << 1) | ((w >> 3) & 1u);
+}
+
 template <class G>
 __global__ GNT_TBOUNDS void k_fwd_pass(u64 *__restrict__ x, const u64 *__restrict__ limbs, size_t n_limbs,
                                        PassDesc d, Tw tw, const u64 *__restrict__ T) {
     extern __shared__ u64 tile[];
-    body_fwd<X, G>(tile, make_geom<G>(d, blockIdx.x), x, limbs, n_limbs, tw, T);
+    const unsigned t = limbs ? xcd_paired_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    body_fwd<X, G>(tile, make_geom<G>(d, t), x, limbs, n_limbs, tw, T);
 }
 
 // The same forward pass over two operands in one launch: blocks [0, tiles)
@@ -46,7 +58,8 @@ __global__ GNT_TBOUNDS void k_fwd_pass_pair(u64 *__restrict__ xa, const u64 *__r
                                             unsigned tiles, PassDesc d, Tw tw, const u64 *__restrict__ T) {
     extern __shared__ u64 tile[];
     const bool b = blockIdx.x >= tiles;
-    const unsigned t = b ? blockIdx.x - tiles : blockIdx.x;
+    unsigned t = b ? blockIdx.x - tiles : blockIdx.x;
+    if (la) t = xcd_paired_tile(t, tiles);  // tiles is a multiple of 16 or the mapping is the identity
     if (b) body_fwd<X, G>(tile, make_geom<G>(d, t), xb, lb, nb, tw, T);
     else body_fwd<X, G>(tile, make_geom<G>(d, t), xa, la, na, tw, T);
 }

@@ -70,7 +70,15 @@ struct Sub {
     static constexpr int log(int i) { return LOGR / count + (i < LOGR % count ? 1 : 0); }
     // block size (log) seen by sub-step i inside the R-point column DFT
     static constexpr int log_m(int i) { return i == 0 ? LOGR : log_m(i - 1) - log(i - 1); }
+    // The shift stage: the one sub-step (not the last) whose block has at most
+    // 64 elements, so its twiddles omega_{2^LM}^e = 2^(e 192/2^LM) are powers
+    // of two; -1 when there is none.
+    static constexpr int shift_stage(int i = 0) {
+        return i >= count - 1 ? -1 : (log_m(i) <= 6 ? i : shift_stage(i + 1));
+    }
 };
+
+constexpr int bitrev_c(int x, int bits) { return bits == 0 ? 0 : ((x & 1) << (bits - 1)) | bitrev_c(x >> 1, bits - 1); }
 
 // Tile shape, fixed at compile time: R = 2^LOGR rows, 2^LOGC columns;
 // STRIDED: columns are adjacent in memory (a pass with column stride > 1);
@@ -83,6 +91,28 @@ struct Geom {
     static constexpr int tile_n = 1 << (LOGR_ + LOGC_);
     int log_m, log_s;  // pass block size, column stride (logs)
     size_t cg0;        // first column of the tile
+};
+
+// Shift stage of a tile shape's column DFT (Sub<LOGR>::shift_stage).  Its
+// twiddle omega_{2^LM}^(jj * k) depends on the task's offset jj inside its
+// block of 2^LM rows and on the register k; when every wave's tasks share one
+// jj the twiddles are shifts by compile-time amounts in each of 2^LT branches
+// taken by whole waves (a general multiply from a table otherwise).  Tasks of
+// that sub-step are therefore numbered column fastest, then block, then jj,
+// so that a wave's 64 tasks cover 2^log_c columns of 2^(6 - log_c) blocks.
+// Strided passes with at least 4 columns per tile only: measured on a 2^24-limb
+// product, it takes 5% of the VALU instructions out of those launches; in the
+// contiguous last pass (rows of 4 or 8 columns, XOR-swizzled) the same change
+// raised LDS bank conflicts by 60% and gained nothing, and a row swizzle that
+// removes the conflicts cost more address arithmetic than the shifts saved.
+template <class G>
+struct ShiftTw {
+    static constexpr int I = Sub<G::LOGR>::shift_stage();
+    static constexpr int LM = I >= 0 ? Sub<G::LOGR>::log_m(I) : 0;
+    static constexpr int A = I >= 0 ? Sub<G::LOGR>::log(I) : 0;
+    static constexpr int LT = LM - A;
+    static constexpr int LOG_NBLK = G::LOGR - LM;
+    static constexpr bool on = I >= 0 && G::strided && G::log_c >= 2 && G::log_c + LOG_NBLK >= 6;
 };
 
 template <class G>
@@ -142,6 +172,54 @@ GL_HD u64 coeff_from_limbs(const u64 *__restrict__ limbs, size_t n_limbs, size_t
     return (limbs[l] >> (16 * (i & 3))) & 0xFFFFull;
 }
 
+// A value every lane of the wave holds alike, moved to a scalar register so
+// that branches on it are taken by the whole wave.
+GL_HD int wave_uniform_int(int x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(x);
+#else
+    return x;
+#endif
+}
+
+// v[q] *= 2^(sign * SH * J * bitrev_A(q)) for q = 1 .. 2^A - 1 with J fixed at
+// compile time: shifts, negations and one-reduction multiplies only.
+template <int SH, int A, int J, bool INV>
+GL_HD void shift_twiddles(u64 *v) {
+#pragma unroll
+    for (int q = 1; q < (1 << A); q++) {
+        const int e = (SH * J * bitrev_c(q, A)) % 192;
+        v[q] = gl::mul_pow2(v[q], INV ? (192 - e) % 192 : e);
+    }
+}
+
+// The same with J = jj, a wave-uniform value below NJ, one branch per value.
+template <int SH, int A, int NJ, bool INV, int J = 0>
+GL_HD void shift_twiddles_by(u64 *v, int jj) {
+    if constexpr (J < NJ) {
+        if (jj == J) shift_twiddles<SH, A, J, INV>(v);
+        else shift_twiddles_by<SH, A, NJ, INV, J + 1>(v, jj);
+    }
+}
+
+// Task L of sub-step I: column c, offset jj inside its block of 2^LM rows,
+// block blk.  Column fastest, then jj, then block; for the shift stage of a
+// shape with ShiftTw on, column, then block, then jj (whole waves share jj).
+template <class G, int I>
+GL_HD void task_coords(int L, int &c, int &jj, int &blk) {
+    constexpr int LM = Sub<G::LOGR>::log_m(I);
+    constexpr int LT = LM - Sub<G::LOGR>::log(I);
+    c = L & ((1 << G::log_c) - 1);
+    const int rest = L >> G::log_c;
+    if constexpr (ShiftTw<G>::on && I == ShiftTw<G>::I) {
+        blk = rest & ((1 << ShiftTw<G>::LOG_NBLK) - 1);
+        jj = rest >> ShiftTw<G>::LOG_NBLK;
+    } else {
+        jj = rest & ((1 << LT) - 1);
+        blk = rest >> LT;
+    }
+}
+
 // Sub-step I of the forward R-point column DFT for task L (DIF): an r-point
 // DFT over rows base + u*t of column c, then the twiddle omega_{2^LM}^(jj*k1)
 // (none after the last sub-step; the pass twiddle is applied on the way out).
@@ -153,16 +231,16 @@ GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
     constexpr int LM = Sub<LOGR>::log_m(I);
     constexpr int LT = LM - A;
     constexpr bool LAST = (I == Sub<LOGR>::count - 1);
-    const int c = L & ((1 << G::log_c) - 1);
-    const int rest = L >> G::log_c;
-    const int jj = rest & ((1 << LT) - 1);
-    const int blk = rest >> LT;
+    int c, jj, blk;
+    task_coords<G, I>(L, c, jj, blk);
     const int base = (blk << LM) + jj;
     u64 v[R];
 #pragma unroll
     for (int u = 0; u < R; u++) v[u] = tile[tix(g, base + (u << LT), c)];
     gl::dft_dif<R>(v);
-    if (!LAST) {
+    if constexpr (ShiftTw<G>::on && I == ShiftTw<G>::I) {
+        shift_twiddles_by<192 / (1 << LM), A, (1 << LT), false>(v, wave_uniform_int(jj));
+    } else if constexpr (!LAST) {
         const u64 *w = tw.wsm + (1 << LM);
 #pragma unroll
         for (int q = 1; q < R; q++) v[q] = gl::mul(v[q], w[(u32)jj * gl::bitrev((u32)q, A)]);
@@ -181,15 +259,15 @@ GL_HD void inv_task(u64 *tile, const G &g, int L, const Tw &tw) {
     constexpr int LM = Sub<LOGR>::log_m(I);
     constexpr int LT = LM - A;
     constexpr bool LAST = (I == Sub<LOGR>::count - 1);
-    const int c = L & ((1 << G::log_c) - 1);
-    const int rest = L >> G::log_c;
-    const int jj = rest & ((1 << LT) - 1);
-    const int blk = rest >> LT;
+    int c, jj, blk;
+    task_coords<G, I>(L, c, jj, blk);
     const int base = (blk << LM) + jj;
     u64 v[R];
 #pragma unroll
     for (int q = 0; q < R; q++) v[q] = tile[tix(g, base + (q << LT), c)];
-    if (!LAST) {
+    if constexpr (ShiftTw<G>::on && I == ShiftTw<G>::I) {
+        shift_twiddles_by<192 / (1 << LM), A, (1 << LT), true>(v, wave_uniform_int(jj));
+    } else if constexpr (!LAST) {
         const u64 *w = tw.wsm + (1 << LM);
 #pragma unroll
         for (int q = 1; q < R; q++) {
