@@ -216,8 +216,13 @@ int gnt_set_pass_tables(int max_log_m, int *previous);
  * no reference counterpart (the reference's multiply never leaves the host). */
 int gnt_set_staging(int threads, size_t chunk_bytes, size_t min_bytes);
 /* Time every valid split of each length in [min_log_n, max_log_n] (4..28) on
- * the device (median of reps products), install the fastest, and write the
- * choices and timings to cache_path (JSON) when it is not NULL. */
+ * the device (median of reps products each, a multiply and a square), check
+ * every candidate's product (fingerprints mod 2^64-1 and 2^61-1 and the low
+ * limb; GNT_E_DEVICE if one is wrong), install the split whose multiply plus
+ * square is fastest, and write the choices and both timings to cache_path
+ * (JSON) when it is not NULL.  Tile shapes are fixed at 2^13 elements (two
+ * 64 KiB tiles per CU; 2^12-element tiles at four per CU measured slower), so
+ * a split is the whole plan. */
 int gnt_tune(gnt_ctx *ctx, int min_log_n, int max_log_n, int reps, const char *cache_path);
 
 /* ---- introspection ---- */

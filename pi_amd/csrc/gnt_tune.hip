@@ -64,6 +64,41 @@ void install(int log_n, int np, const int *lr) {
     set_plan_choice(log_n, np, lr);
 }
 
+// Fingerprints of the integer with little-endian limbs x: x mod (2^64 - 1)
+// (2^64 = 1 there: the limb sum with end-around carry) and x mod (2^61 - 1)
+// (2^64 = 8 there; Horner from the top limb).
+struct Fp {
+    u64 m64, m61;
+    bool operator==(const Fp &o) const { return m64 == o.m64 && m61 == o.m61; }
+};
+constexpr u64 M61 = (1ull << 61) - 1;
+u64 mod61(unsigned __int128 v) {
+    u64 r = (u64)(v & M61) + (u64)((v >> 61) & M61) + (u64)(v >> 122);
+    while (r >= M61) r -= M61;
+    return r;
+}
+Fp fingerprint(const u64 *x, size_t n) {
+    unsigned __int128 s = 0;
+    u64 h = 0;
+    for (size_t i = 0; i < n; i++) s += x[i];
+    for (size_t i = n; i-- > 0;) h = mod61((unsigned __int128)h * 8 + x[i]);
+    u64 m = (u64)s + (u64)(s >> 64);
+    if (m < (u64)s) m++;             // end-around carry
+    if (m == ~0ull) m = 0;           // 2^64 - 1 = 0
+    return Fp{m, h};
+}
+// c == a * b by the two fingerprints and the low limb.
+bool product_consistent(const u64 *a, size_t na, const u64 *b, size_t nb, const u64 *c, size_t nc) {
+    const Fp fa = fingerprint(a, na), fb = fingerprint(b, nb), fc = fingerprint(c, nc);
+    unsigned __int128 p = (unsigned __int128)fa.m64 * fb.m64;
+    u64 m = (u64)p + (u64)(p >> 64);
+    if (m < (u64)p) m++;
+    if (m == ~0ull) m = 0;
+    const u64 h = mod61((unsigned __int128)fa.m61 * fb.m61);
+    const u64 low = nc ? c[0] : 0;
+    return m == fc.m64 && h == fc.m61 && low == a[0] * b[0];
+}
+
 std::string key(const std::vector<int> &v) {
     std::string s;
     for (size_t i = 0; i < v.size(); i++) s += (i ? "," : "") + std::to_string(v[i]);
@@ -156,54 +191,68 @@ int gnt_tune(gnt_ctx *ctx, int min_log_n, int max_log_n, int reps, const char *c
     if (reps < 1) reps = 3;
     if (hipSetDevice(c->device) != hipSuccess) return gnt::set_error(GNT_E_DEVICE, "hipSetDevice failed");
     std::lock_guard<std::mutex> g(c->mu);
-    // operands for the largest length: na = nb = 2^(log_n-3) fills the transform exactly
+    // operands for the largest length: na = nb = 2^(log_n-3) fills the transform
+    // exactly; two different SplitMix64 streams, so the multiply is no square
     const size_t nmax = (size_t)1 << (max_log_n - 3);
-    std::vector<u64> host(nmax);
+    std::vector<u64> ha(nmax), hb(nmax), hout(2 * nmax);
     u64 st = 0x243F6A8885A308D3ull;
-    for (auto &v : host) {
+    for (size_t i = 0; i < 2 * nmax; i++) {
         st += 0x9E3779B97F4A7C15ull;
         u64 z = st;
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        v = z ^ (z >> 31);
+        (i < nmax ? ha[i] : hb[i - nmax]) = z ^ (z >> 31);
     }
     u64 *da = nullptr, *db = nullptr, *dout = nullptr;
     uint32_t *dlen = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = GNT_OK;
-    std::string plans, times;
+    std::string plans, times, times_sq;
     if (hipMalloc(&da, nmax * 8) || hipMalloc(&db, nmax * 8) || hipMalloc(&dout, 2 * nmax * 8) ||
-        hipMalloc(&dlen, 4) || hipMemcpy(da, host.data(), nmax * 8, hipMemcpyHostToDevice) ||
-        hipMemcpy(db, host.data(), nmax * 8, hipMemcpyHostToDevice) || hipEventCreate(&e0) || hipEventCreate(&e1))
+        hipMalloc(&dlen, 4) || hipMemcpy(da, ha.data(), nmax * 8, hipMemcpyHostToDevice) ||
+        hipMemcpy(db, hb.data(), nmax * 8, hipMemcpyHostToDevice) || hipEventCreate(&e0) || hipEventCreate(&e1))
         rc = gnt::set_error(GNT_E_DEVICE, "tuner allocation failed");
+    // median device time (us) of reps products after one warm-up, and the
+    // product itself checked (fingerprints mod 2^64-1 and 2^61-1, low limb)
+    auto time_product = [&](size_t n, bool sqr, float &med) -> int {
+        std::vector<float> us;
+        for (int r = 0; r <= reps; r++) {
+            if (hipEventRecord(e0, c->stream) || c->enqueue_product(da, n, sqr ? nullptr : db, n, dout, dlen) ||
+                hipEventRecord(e1, c->stream) || hipEventSynchronize(e1))
+                return gnt::set_error(GNT_E_DEVICE, "tuner product failed");
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r > 0) us.push_back(ms * 1000.0f);
+        }
+        std::sort(us.begin(), us.end());
+        med = us[us.size() / 2];
+        if (hipMemcpy(hout.data(), dout, 2 * n * 8, hipMemcpyDeviceToHost))
+            return gnt::set_error(GNT_E_DEVICE, "tuner download failed");
+        const u64 *b = sqr ? ha.data() : hb.data();
+        if (!gnt::product_consistent(ha.data(), n, b, n, hout.data(), 2 * n))
+            return gnt::set_error(GNT_E_DEVICE, "tuner: a candidate split gave a wrong product");
+        return GNT_OK;
+    };
     for (int log_n = min_log_n; !rc && log_n <= max_log_n; log_n++) {
         const size_t n = (size_t)1 << (log_n - 3);
         const auto cands = gnt::candidates(log_n);
         std::vector<int> best;
         float best_us = 0;
-        std::string row;
+        std::string row, row_sq;
         for (const auto &cand : cands) {
             gnt::install(log_n, (int)cand.size(), cand.data());
-            std::vector<float> us;
-            for (int r = 0; r <= reps && !rc; r++) {  // first run warms up
-                if (hipEventRecord(e0, c->stream) || c->enqueue_product(da, n, db, n, dout, dlen) ||
-                    hipEventRecord(e1, c->stream) || hipEventSynchronize(e1)) {
-                    rc = gnt::set_error(GNT_E_DEVICE, "tuner product failed");
-                    break;
-                }
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (r > 0) us.push_back(ms * 1000.0f);
-            }
-            if (rc) break;
-            std::sort(us.begin(), us.end());
-            const float med = us[us.size() / 2];
+            float mul_us = 0, sqr_us = 0;
+            if ((rc = time_product(n, false, mul_us)) || (rc = time_product(n, true, sqr_us))) break;
             char b[64];
-            std::snprintf(b, sizeof b, "%s\"%s\": %.2f", row.empty() ? "" : ", ", gnt::key(cand).c_str(), med);
+            std::snprintf(b, sizeof b, "%s\"%s\": %.2f", row.empty() ? "" : ", ", gnt::key(cand).c_str(), mul_us);
             row += b;
-            if (best.empty() || med < best_us) {
+            std::snprintf(b, sizeof b, "%s\"%s\": %.2f", row_sq.empty() ? "" : ", ", gnt::key(cand).c_str(), sqr_us);
+            row_sq += b;
+            // one split per length serves multiplies and squares: the
+            // fastest for a multiply and a square together
+            if (best.empty() || mul_us + sqr_us < best_us) {
                 best = cand;
-                best_us = med;
+                best_us = mul_us + sqr_us;
             }
         }
         if (rc) break;
@@ -211,6 +260,7 @@ int gnt_tune(gnt_ctx *ctx, int min_log_n, int max_log_n, int reps, const char *c
         std::string l = std::to_string(log_n);
         plans += (plans.empty() ? "" : ",\n  ") + ("\"" + l + "\": [" + gnt::key(best) + "]");
         times += (times.empty() ? "" : ",\n  ") + ("\"" + l + "\": {" + row + "}");
+        times_sq += (times_sq.empty() ? "" : ",\n  ") + ("\"" + l + "\": {" + row_sq + "}");
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -224,8 +274,8 @@ int gnt_tune(gnt_ctx *ctx, int min_log_n, int max_log_n, int reps, const char *c
         std::string dev = hipGetDeviceProperties(&prop, c->device) == hipSuccess ? prop.gcnArchName : "unknown";
         FILE *f = std::fopen(cache_path, "wb");
         if (!f) return gnt::set_error(GNT_E_ARG, "cannot write plan cache");
-        std::fprintf(f, "{\"device\": \"%s\",\n \"plans\": {\n  %s\n },\n \"us\": {\n  %s\n }\n}\n", dev.c_str(),
-                     plans.c_str(), times.c_str());
+        std::fprintf(f, "{\"device\": \"%s\",\n \"plans\": {\n  %s\n },\n \"us\": {\n  %s\n },\n \"us_sqr\": {\n  %s\n }\n}\n",
+                     dev.c_str(), plans.c_str(), times.c_str(), times_sq.c_str());
         std::fclose(f);
     }
     return GNT_OK;
