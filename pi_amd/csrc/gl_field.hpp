@@ -160,11 +160,12 @@ GL_HD u32 bitrev(u32 x, int bits) {
 // ---------------------------------------------------------------------------
 
 // Forward DIF: natural-order input, output register q holds frequency
-// bitrev_R(q).  Sum_i v[i] * omega_R^(i*k).
-template <int R>
+// bitrev_R(q).  Sum_i v[i] * omega_R^(i*k).  M0 < R starts at the level of
+// blocks of M0 (the levels above done by the caller).
+template <int R, int M0 = R>
 GL_HD void dft_dif(u64 *v) {
 #pragma unroll
-    for (int m = R; m >= 2; m >>= 1) {
+    for (int m = M0; m >= 2; m >>= 1) {
         const int h = m >> 1;
 #pragma unroll
         for (int b = 0; b < R; b += m) {
@@ -176,6 +177,15 @@ GL_HD void dft_dif(u64 *v) {
             }
         }
     }
+}
+
+// dft_dif<R> of v whose upper half v[R/2 ..] is zero: the first level's
+// butterflies reduce to v[j + R/2] = v[j] * omega_R^j (v[R/2 ..] not read).
+template <int R>
+GL_HD void dft_dif_upper_zero(u64 *v) {
+#pragma unroll
+    for (int j = 0; j < R / 2; j++) v[j + R / 2] = mul_pow2(v[j], (192 / R) * j);
+    dft_dif<R, R / 2>(v);
 }
 
 // Inverse DIT (unscaled): register q holds frequency bitrev_R(q); output is

@@ -223,7 +223,9 @@ GL_HD void task_coords(int L, int &c, int &jj, int &blk) {
 // Sub-step I of the forward R-point column DFT for task L (DIF): an r-point
 // DFT over rows base + u*t of column c, then the twiddle omega_{2^LM}^(jj*k1)
 // (none after the last sub-step; the pass twiddle is applied on the way out).
-template <class G, int I>
+// UPPER_ZERO (sub-step 0 only): the tile's rows R/2 .. R-1 are zero and not
+// read (a packing pass whose operand fills at most half the transform).
+template <class G, int I, bool UPPER_ZERO = false>
 GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
     constexpr int LOGR = G::LOGR;
     constexpr int A = Sub<LOGR>::log(I);
@@ -231,13 +233,20 @@ GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
     constexpr int LM = Sub<LOGR>::log_m(I);
     constexpr int LT = LM - A;
     constexpr bool LAST = (I == Sub<LOGR>::count - 1);
+    static_assert(!UPPER_ZERO || (I == 0 && R >= 2), "upper-zero input: first sub-step only");
     int c, jj, blk;
     task_coords<G, I>(L, c, jj, blk);
     const int base = (blk << LM) + jj;
     u64 v[R];
+    if constexpr (UPPER_ZERO) {
 #pragma unroll
-    for (int u = 0; u < R; u++) v[u] = tile[tix(g, base + (u << LT), c)];
-    gl::dft_dif<R>(v);
+        for (int u = 0; u < R / 2; u++) v[u] = tile[tix(g, base + (u << LT), c)];
+        gl::dft_dif_upper_zero<R>(v);
+    } else {
+#pragma unroll
+        for (int u = 0; u < R; u++) v[u] = tile[tix(g, base + (u << LT), c)];
+        gl::dft_dif<R>(v);
+    }
     if constexpr (ShiftTw<G>::on && I == ShiftTw<G>::I) {
         shift_twiddles_by<192 / (1 << LM), A, (1 << LT), false>(v, wave_uniform_int(jj));
     } else if constexpr (!LAST) {
@@ -393,14 +402,32 @@ GL_HD void tile_load_step(u64 *tile, const G &g, const u64 *__restrict__ x, cons
     X::sync();
 }
 
-template <class X, class G, int I>
+template <class X, class G, int I, bool UPPER_ZERO = false>
 GL_HD void tile_fwd(u64 *tile, const G &g, const Tw &tw) {
     if constexpr (I < Sub<G::LOGR>::count) {
         const int ntask = G::tile_n >> Sub<G::LOGR>::log(I);
-        GNT_FOR(X, L, ntask) fwd_task<G, I>(tile, g, L, tw);
+        GNT_FOR(X, L, ntask) fwd_task<G, I, UPPER_ZERO>(tile, g, L, tw);
         X::sync();
         tile_fwd<X, G, I + 1>(tile, g, tw);
     }
+}
+
+// Pack a tile of a strided first pass (block N, so element a = n 2^log_s + j
+// is coefficient a) from the operand's limbs, 16 bits per coefficient
+// (ntt.rs:640-665): one task per limb, which loads it once and writes its four
+// coefficients to the row; rows at or beyond `rows` are left unwritten.
+template <class X, class G>
+GL_HD void tile_pack(u64 *tile, const G &g, const u64 *__restrict__ limbs, size_t n_limbs, int rows) {
+    static_assert(G::strided && G::log_c >= 2, "a row holds whole limbs");
+    constexpr int QL = G::log_c - 2;  // limbs per row (log)
+    GNT_FOR(X, k, rows << QL) {
+        const int n = k >> QL, q = (k & ((1 << QL) - 1)) << 2;
+        const size_t l = tile_addr(g.cg0, n, q, g.log_m, g.log_s) >> 2;
+        const u64 v = l < n_limbs ? limbs[l] : 0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) tile[tix(g, n, q + t)] = (v >> (16 * t)) & 0xFFFFull;
+    }
+    X::sync();
 }
 
 // Inverse sub-steps I, I-1, ..., 0.
@@ -477,8 +504,23 @@ GL_HD void tile_limb_sums(const u64 *tile, const G &g, size_t n_coeffs, size_t n
 // stepped (T null).
 template <class X, class G>
 GL_HD void body_fwd(u64 *tile, const G &g, u64 *x, const u64 *limbs, size_t n_limbs, const Tw &tw, const u64 *T) {
-    tile_load<X, G>(tile, g, x, limbs, n_limbs);
-    tile_fwd<X, G, 0>(tile, g, tw);
+    if constexpr (G::strided && G::log_c >= 2) {
+        if (limbs) {
+            // first pass of a strided plan (block N): rows R/2 .. R-1 hold
+            // coefficients from N/2 on, zero when the operand fills at most
+            // half the transform (every balanced product)
+            const bool upper_zero = 4 * (u64)n_limbs <= ((u64)1 << (g.log_m - 1));
+            tile_pack<X, G>(tile, g, limbs, n_limbs, upper_zero ? (1 << G::LOGR) / 2 : 1 << G::LOGR);
+            if (upper_zero) tile_fwd<X, G, 0, true>(tile, g, tw);
+            else tile_fwd<X, G, 0>(tile, g, tw);
+        } else {
+            tile_load<X, G>(tile, g, x, nullptr, 0);
+            tile_fwd<X, G, 0>(tile, g, tw);
+        }
+    } else {
+        tile_load<X, G>(tile, g, x, limbs, n_limbs);
+        tile_fwd<X, G, 0>(tile, g, tw);
+    }
     if constexpr (G::strided) {
         if (T) tile_store_tw<X, G>(tile, g, x, T, g.log_m);
         else tile_store_step<X, G>(tile, g, x, tw);
