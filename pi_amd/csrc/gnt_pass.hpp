@@ -64,10 +64,18 @@ GL_HD u64 pass_table_entry(const Tw &tw, u32 i, int log_m, int log_r, bool inv, 
 
 // Sub-steps of an R-point column DFT: radix <= 8 each (8 values in registers
 // keeps a thread within 64 VGPRs), as even as possible, larger ones first.
-template <int LOGR>
+template <int LOGR, bool L16 = false>
 struct Sub {
-    static constexpr int count = (LOGR + 2) / 3;
-    static constexpr int log(int i) { return LOGR / count + (i < LOGR % count ? 1 : 0); }
+    // L16 with LOGR = 10 or 13: the column DFT ends in one radix-16 sub-step
+    // instead of two radix-4 ones, which removes the twiddle stage between
+    // them; the last sub-step has no twiddle after it, so its 16 values fit
+    // the register budget where the kernel holds little else (with LOGR = 7
+    // the fused multiply kernel spills two registers, so 7 keeps 3+2+2).
+    static constexpr bool last16 = L16 && LOGR >= 10 && LOGR % 3 == 1;
+    static constexpr int count = last16 ? (LOGR - 1) / 3 : (LOGR + 2) / 3;
+    static constexpr int log(int i) {
+        return last16 ? (i == count - 1 ? 4 : 3) : LOGR / count + (i < LOGR % count ? 1 : 0);
+    }
     // block size (log) seen by sub-step i inside the R-point column DFT
     static constexpr int log_m(int i) { return i == 0 ? LOGR : log_m(i - 1) - log(i - 1); }
     // The shift stage: the one sub-step (not the last) whose block has at most
@@ -83,8 +91,9 @@ constexpr int bitrev_c(int x, int bits) { return bits == 0 ? 0 : ((x & 1) << (bi
 // Tile shape, fixed at compile time: R = 2^LOGR rows, 2^LOGC columns;
 // STRIDED: columns are adjacent in memory (a pass with column stride > 1);
 // otherwise the pass is the contiguous last one (rows adjacent).
-template <int LOGR_, int LOGC_, bool STRIDED_>
+template <int LOGR_, int LOGC_, bool STRIDED_, bool L16_ = false>
 struct Geom {
+    using S = Sub<LOGR_, L16_>;  // sub-steps of the column DFT
     static constexpr int LOGR = LOGR_;
     static constexpr int log_c = LOGC_;
     static constexpr bool strided = STRIDED_;
@@ -93,7 +102,7 @@ struct Geom {
     size_t cg0;        // first column of the tile
 };
 
-// Shift stage of a tile shape's column DFT (Sub<LOGR>::shift_stage).  Its
+// Shift stage of a tile shape's column DFT (Sub::shift_stage).  Its
 // twiddle omega_{2^LM}^(jj * k) depends on the task's offset jj inside its
 // block of 2^LM rows and on the register k; when every wave's tasks share one
 // jj the twiddles are shifts by compile-time amounts in each of 2^LT branches
@@ -107,9 +116,9 @@ struct Geom {
 // removes the conflicts cost more address arithmetic than the shifts saved.
 template <class G>
 struct ShiftTw {
-    static constexpr int I = Sub<G::LOGR>::shift_stage();
-    static constexpr int LM = I >= 0 ? Sub<G::LOGR>::log_m(I) : 0;
-    static constexpr int A = I >= 0 ? Sub<G::LOGR>::log(I) : 0;
+    static constexpr int I = G::S::shift_stage();
+    static constexpr int LM = I >= 0 ? G::S::log_m(I) : 0;
+    static constexpr int A = I >= 0 ? G::S::log(I) : 0;
     static constexpr int LT = LM - A;
     static constexpr int LOG_NBLK = G::LOGR - LM;
     static constexpr bool on = I >= 0 && G::strided && G::log_c >= 2 && G::log_c + LOG_NBLK >= 6;
@@ -207,8 +216,8 @@ GL_HD void shift_twiddles_by(u64 *v, int jj) {
 // shape with ShiftTw on, column, then block, then jj (whole waves share jj).
 template <class G, int I>
 GL_HD void task_coords(int L, int &c, int &jj, int &blk) {
-    constexpr int LM = Sub<G::LOGR>::log_m(I);
-    constexpr int LT = LM - Sub<G::LOGR>::log(I);
+    constexpr int LM = G::S::log_m(I);
+    constexpr int LT = LM - G::S::log(I);
     c = L & ((1 << G::log_c) - 1);
     const int rest = L >> G::log_c;
     if constexpr (ShiftTw<G>::on && I == ShiftTw<G>::I) {
@@ -228,11 +237,11 @@ GL_HD void task_coords(int L, int &c, int &jj, int &blk) {
 template <class G, int I, bool UPPER_ZERO = false>
 GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
     constexpr int LOGR = G::LOGR;
-    constexpr int A = Sub<LOGR>::log(I);
+    constexpr int A = G::S::log(I);
     constexpr int R = 1 << A;
-    constexpr int LM = Sub<LOGR>::log_m(I);
+    constexpr int LM = G::S::log_m(I);
     constexpr int LT = LM - A;
-    constexpr bool LAST = (I == Sub<LOGR>::count - 1);
+    constexpr bool LAST = (I == G::S::count - 1);
     static_assert(!UPPER_ZERO || (I == 0 && R >= 2), "upper-zero input: first sub-step only");
     int c, jj, blk;
     task_coords<G, I>(L, c, jj, blk);
@@ -263,11 +272,11 @@ GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
 template <class G, int I>
 GL_HD void inv_task(u64 *tile, const G &g, int L, const Tw &tw) {
     constexpr int LOGR = G::LOGR;
-    constexpr int A = Sub<LOGR>::log(I);
+    constexpr int A = G::S::log(I);
     constexpr int R = 1 << A;
-    constexpr int LM = Sub<LOGR>::log_m(I);
+    constexpr int LM = G::S::log_m(I);
     constexpr int LT = LM - A;
-    constexpr bool LAST = (I == Sub<LOGR>::count - 1);
+    constexpr bool LAST = (I == G::S::count - 1);
     int c, jj, blk;
     task_coords<G, I>(L, c, jj, blk);
     const int base = (blk << LM) + jj;
@@ -404,8 +413,8 @@ GL_HD void tile_load_step(u64 *tile, const G &g, const u64 *__restrict__ x, cons
 
 template <class X, class G, int I, bool UPPER_ZERO = false>
 GL_HD void tile_fwd(u64 *tile, const G &g, const Tw &tw) {
-    if constexpr (I < Sub<G::LOGR>::count) {
-        const int ntask = G::tile_n >> Sub<G::LOGR>::log(I);
+    if constexpr (I < G::S::count) {
+        const int ntask = G::tile_n >> G::S::log(I);
         GNT_FOR(X, L, ntask) fwd_task<G, I, UPPER_ZERO>(tile, g, L, tw);
         X::sync();
         tile_fwd<X, G, I + 1>(tile, g, tw);
@@ -434,7 +443,7 @@ GL_HD void tile_pack(u64 *tile, const G &g, const u64 *__restrict__ limbs, size_
 template <class X, class G, int I>
 GL_HD void tile_inv(u64 *tile, const G &g, const Tw &tw) {
     if constexpr (I >= 0) {
-        const int ntask = G::tile_n >> Sub<G::LOGR>::log(I);
+        const int ntask = G::tile_n >> G::S::log(I);
         GNT_FOR(X, L, ntask) inv_task<G, I>(tile, g, L, tw);
         X::sync();
         tile_inv<X, G, I - 1>(tile, g, tw);
@@ -541,7 +550,7 @@ GL_HD void body_inv(u64 *tile, const G &g, u64 *x, const Tw &tw, u64 *Ls, u64 *H
                     size_t n_limbs, int scale_e, const StoreTw &st) {
     if constexpr (G::strided && STEP) tile_load_step<X, G>(tile, g, x, tw, scale_e);
     else tile_load<X, G>(tile, g, x, nullptr, 0);
-    tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
+    tile_inv<X, G, G::S::count - 1>(tile, g, tw);
     if constexpr (SUMS) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
     else if constexpr (TSTORE) tile_store_tw<X, G>(tile, g, x, st.T, st.log_t);
     else tile_store<X, G>(tile, g, x);
@@ -561,7 +570,7 @@ GL_HD void body_fused(u64 *tile, const G &g, const u64 *xb, const u64 *limbs, si
     tile_load<X, G>(tile, g, xb, limbs, n_limbs_in);
     tile_fwd<X, G, 0>(tile, g, tw);
     tile_pointwise<X, G>(tile, g, a_other, log_n);
-    tile_inv<X, G, Sub<G::LOGR>::count - 1>(tile, g, tw);
+    tile_inv<X, G, G::S::count - 1>(tile, g, tw);
     if (Ls) tile_limb_sums<X, G>(tile, g, n_coeffs, n_limbs, Ls, Hs);
     else tile_store_any<X, G>(tile, g, xa, st);
 }
@@ -574,8 +583,8 @@ GL_HD void body_fused(u64 *tile, const G &g, const u64 *xb, const u64 *limbs, si
 // B's.  Plans of two or more passes (no pass twiddle on the last pass).
 template <class X, class G, int I>
 GL_HD void tile_fwd_pair(u64 *tile, const G &g, const Tw &tw) {
-    if constexpr (I < Sub<G::LOGR>::count) {
-        const int ntask = G::tile_n >> Sub<G::LOGR>::log(I);
+    if constexpr (I < G::S::count) {
+        const int ntask = G::tile_n >> G::S::log(I);
         GNT_FOR(X, L, 2 * ntask) {
             const bool b = L >= ntask;
             fwd_task<G, I>(tile + (b ? G::tile_n : 0), g, b ? L - ntask : L, tw);
@@ -607,7 +616,7 @@ GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_
         ta[t] = gl::mul(ta[t], tb[t]);
     }
     X::sync();
-    tile_inv<X, G, Sub<G::LOGR>::count - 1>(ta, g, tw);
+    tile_inv<X, G, G::S::count - 1>(ta, g, tw);
     tile_store_any<X, G>(ta, g, xa, st);
 }
 
@@ -616,7 +625,9 @@ GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_
 template <class FullG>
 struct HalfGeom {
     static constexpr bool ok = !FullG::strided && FullG::log_c >= 1 && FullG::LOGR + FullG::log_c == TILE_LOG;
-    using type = Geom<FullG::LOGR, ok ? FullG::log_c - 1 : FullG::log_c, false>;
+    // the fused multiply kernel holds few registers besides a task's values:
+    // a radix-16 last sub-step fits it without spilling
+    using type = Geom<FullG::LOGR, ok ? FullG::log_c - 1 : FullG::log_c, false, true>;
 };
 
 // Carry-lookahead pairs: bit 0 = generate, bit 1 = propagate.  `lo` covers
