@@ -9,6 +9,14 @@
 //                             field arithmetic is made of (SIMD cycles per
 //                             wave64 instruction; 8 independent chains per
 //                             lane, many waves per SIMD)
+//   gnt_calib copy4 <bytes>   the copy with four 16-byte loads in flight per
+//                             thread, plain and with non-temporal accesses
+//   gnt_calib persistent      a 2 GiB read-modify-write in 64 KiB tiles: one
+//                             workgroup per tile, against 512 persistent
+//                             workgroups that claim tiles from a counter and
+//                             signal each finished tile with a relaxed or a
+//                             device-scope release atomic (what dependencies
+//                             between tiles on different XCDs require)
 // Prints one JSON line per measurement.
 #include <hip/hip_runtime.h>
 
@@ -34,6 +42,58 @@ __global__ __launch_bounds__(256) void k_copy(const ulonglong2 *__restrict__ src
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     size_t stride = (size_t)gridDim.x * 256;
     for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// Four loads in flight per thread before the stores; NT: non-temporal.
+typedef unsigned long long v2u64 __attribute__((ext_vector_type(2)));
+template <bool NT>
+__global__ __launch_bounds__(256) void k_copy4(const v2u64 *__restrict__ src, v2u64 *__restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += 4 * stride) {
+        v2u64 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const size_t j = i + k * stride;
+            if (j < n) v[k] = NT ? __builtin_nontemporal_load(src + j) : src[j];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const size_t j = i + k * stride;
+            if (j < n) {
+                if (NT) __builtin_nontemporal_store(v[k], dst + j);
+                else dst[j] = v[k];
+            }
+        }
+    }
+}
+
+// 64 KiB tiles of x updated in place.  MODE 0: one workgroup per tile.
+// MODE 1/2: persistent workgroups claim tiles from *next until all are taken
+// (every workgroup leaves the loop once the counter passes the tile count)
+// and count each finished tile in *done, relaxed (1) or with a device-scope
+// release (2).
+constexpr int PT_ELEMS = 8192;
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_tiles(u64 *x, unsigned *done, unsigned long long *next,
+                                                unsigned long long tiles) {
+    __shared__ unsigned long long s_t;
+    for (unsigned long long t = blockIdx.x;;) {
+        if (MODE != 0) {
+            if (threadIdx.x == 0) s_t = atomicAdd(next, 1ull);
+            __syncthreads();
+            t = s_t;
+        }
+        if (t >= tiles) break;
+        u64 *p = x + t * PT_ELEMS;
+        for (int i = threadIdx.x; i < PT_ELEMS; i += 1024) p[i] = p[i] * 3 + t;
+        if (MODE == 0) break;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (MODE == 2) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            else __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();  // s_t is rewritten only after every thread has read it
+    }
 }
 
 constexpr int CHAINS = 8;
@@ -167,6 +227,76 @@ int main(int argc, char **argv) {
                ms, 2.0 * bytes / (ms * 1e-3) / 1e9);
         CK(hipFree(c.src));
         CK(hipFree(c.dst));
+    }
+    if (!strcmp(mode, "copy4") || !strcmp(mode, "all")) {
+        size_t bytes = argc > 2 && !strcmp(mode, "copy4") ? strtoull(argv[2], 0, 10) : ((size_t)1 << 30);
+        const size_t n = bytes / sizeof(v2u64);
+        v2u64 *src, *dst;
+        CK(hipMalloc(&src, bytes));
+        CK(hipMalloc(&dst, bytes));
+        CK(hipMemset(src, 1, bytes));
+        hipEvent_t e0, e1;
+        CK(hipEventCreate(&e0));
+        CK(hipEventCreate(&e1));
+        for (int nt = 0; nt < 2; nt++) {
+            for (int grid : {1024, 4096, 16384}) {
+                float best = 1e30f;
+                for (int r = 0; r < 10; r++) {
+                    CK(hipEventRecord(e0));
+                    if (nt) hipLaunchKernelGGL(k_copy4<true>, dim3(grid), dim3(256), 0, 0, src, dst, n);
+                    else hipLaunchKernelGGL(k_copy4<false>, dim3(grid), dim3(256), 0, 0, src, dst, n);
+                    CK(hipEventRecord(e1));
+                    CK(hipEventSynchronize(e1));
+                    float ms;
+                    CK(hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0 && ms < best) best = ms;
+                }
+                printf("{\"calib\": \"copy4\", \"nontemporal\": %d, \"grid\": %d, \"bytes_each_way\": %zu, "
+                       "\"best_ms\": %.4f, \"GBps_read_plus_write\": %.1f}\n",
+                       nt, grid, bytes, best, 2.0 * bytes / (best * 1e-3) / 1e9);
+            }
+        }
+        CK(hipFree(src));
+        CK(hipFree(dst));
+    }
+    if (!strcmp(mode, "persistent") || !strcmp(mode, "all")) {
+        const unsigned long long tiles = 32768;  // 2 GiB
+        u64 *x;
+        unsigned *done;
+        unsigned long long *next;
+        CK(hipMalloc(&x, tiles * PT_ELEMS * sizeof(u64)));
+        CK(hipMemset(x, 0, tiles * PT_ELEMS * sizeof(u64)));
+        CK(hipMalloc(&done, sizeof(unsigned)));
+        CK(hipMalloc(&next, sizeof(unsigned long long)));
+        hipEvent_t e0, e1;
+        CK(hipEventCreate(&e0));
+        CK(hipEventCreate(&e1));
+        const char *names[3] = {"one_workgroup_per_tile", "persistent_relaxed", "persistent_release"};
+        for (int mode_k = 0; mode_k < 3; mode_k++) {
+            float best = 1e30f;
+            for (int r = 0; r < 6; r++) {
+                CK(hipMemset(next, 0, sizeof(unsigned long long)));
+                CK(hipMemset(done, 0, sizeof(unsigned)));
+                CK(hipDeviceSynchronize());
+                CK(hipEventRecord(e0));
+                if (mode_k == 0) hipLaunchKernelGGL(k_tiles<0>, dim3((unsigned)tiles), dim3(1024), 0, 0, x, done, next, tiles);
+                else if (mode_k == 1) hipLaunchKernelGGL(k_tiles<1>, dim3(512), dim3(1024), 0, 0, x, done, next, tiles);
+                else hipLaunchKernelGGL(k_tiles<2>, dim3(512), dim3(1024), 0, 0, x, done, next, tiles);
+                CK(hipEventRecord(e1));
+                CK(hipEventSynchronize(e1));
+                float ms;
+                CK(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0 && ms < best) best = ms;
+            }
+            unsigned d = 0;
+            CK(hipMemcpy(&d, done, sizeof d, hipMemcpyDeviceToHost));
+            printf("{\"calib\": \"persistent\", \"mode\": \"%s\", \"tiles\": %llu, \"tile_bytes\": %d, "
+                   "\"best_ms\": %.4f, \"tiles_counted\": %u}\n",
+                   names[mode_k], tiles, PT_ELEMS * 8, best, d);
+        }
+        CK(hipFree(x));
+        CK(hipFree(done));
+        CK(hipFree(next));
     }
     if (!strcmp(mode, "valu") || !strcmp(mode, "all")) {
         ValuArg v;
