@@ -625,8 +625,8 @@ def serve_open(args):
 
 
 def _latency_by_size(lat_ms, sizes) -> dict:
-    """p50/p99 (ms, from scheduled arrival) per operand-size band 2^[8,12),
-    2^[12,16), 2^[16,20), 2^[20,24]."""
+    """p50/p99 (ms, from scheduled arrival) per band of the larger operand's
+    limbs: 2^[8,12), 2^[12,16), 2^[16,20), 2^[20,25)."""
     out = {}
     for lo, hi in ((8, 12), (12, 16), (16, 20), (20, 25)):
         sel = lat_ms[(sizes >= (1 << lo)) & (sizes < (1 << hi))]
@@ -713,7 +713,9 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
             t = srv.submit(None if a_buf is not None else a, b, a_buf=a_buf)
             n_in = n_a + n
         meta[t] = (n_in, check, r["keep"])
-        size_of[t] = n
+        # the band of a request is its larger operand: a reused device result
+        # can be far larger than the drawn size n
+        size_of[t] = n_a if r["sq"] else max(n_a, n)
         # time between the scheduled arrival and the submit call, so that a
         # submit delayed by an earlier one still counts from its arrival
         late[t] = max(0.0, t_call - (t0 + r["at"])) * 1e6
