@@ -138,7 +138,10 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
             std::memcpy(host.data() + tbytes + b->c_off[k] * sizeof(CItem), it.citems.data(),
                         it.citems.size() * sizeof(CItem));
     }
-    if (hipMemcpy(b->blob, host.data(), b->blob_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    // on the context stream, which orders the copy after earlier users of the
+    // pooled buffer; the host vector lives until the copy is done
+    if (hipMemcpyAsync(b->blob, host.data(), b->blob_bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
         release_batch(b);
         return set_error(GNT_E_DEVICE, "descriptor upload failed");
     }

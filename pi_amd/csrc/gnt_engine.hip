@@ -49,12 +49,13 @@ __device__ __forceinline__ T wave_uniform(const T &v) {
 
 template <class G, int KIND>
 __global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tpass(
-    const TItem *__restrict__ items, int n_items, uint64_t total, Tw tw) {
+    const TItem *__restrict__ items, int n_items, uint64_t total, uint64_t base, Tw tw) {
     // One tile per workgroup, as the single-product kernels run: a loop over
     // tiles inside the workgroup lets the compiler keep per-thread address
-    // arithmetic alive across iterations, which spills the tile bodies.
+    // arithmetic alive across iterations, which spills the tile bodies.  A
+    // launch covers tiles [base, base + gridDim.x) of the group.
     extern __shared__ uint64_t tile[];
-    const uint64_t t = blockIdx.x;
+    const uint64_t t = base + blockIdx.x;
     if (t >= total) return;
     const TItem it = wave_uniform(items[find_item(items, n_items, t)]);
     if constexpr (KIND == K_FUSED_AB) {
@@ -389,38 +390,80 @@ void finalize_iteration(Iteration &it) {
     }
 }
 
+hipError_t LaunchPacer::before() {
+    while ((int)inflight.size() >= depth) {
+        hipError_t e = hipEventSynchronize(inflight.front());
+        if (e != hipSuccess) return e;
+        spare.push_back(inflight.front());
+        inflight.erase(inflight.begin());
+    }
+    return hipSuccess;
+}
+
+hipError_t LaunchPacer::after() {
+    hipEvent_t e = nullptr;
+    if (!spare.empty()) {
+        e = spare.back();
+        spare.pop_back();
+    } else if (hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming)) {
+        return r;
+    }
+    hipError_t r = hipEventRecord(e, st);
+    if (r != hipSuccess) {
+        spare.push_back(e);
+        return r;
+    }
+    inflight.push_back(e);
+    return hipSuccess;
+}
+
+void LaunchPacer::release() {
+    for (hipEvent_t e : inflight) (void)hipEventDestroy(e);
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    inflight.clear();
+    spare.clear();
+}
+
 template <int KIND>
-static int launch_tgroup(const Group &g, const TItem *dev, const Tw &tw, hipStream_t st) {
+static int launch_tgroup(const Group &g, const TItem *dev, const Tw &tw, hipStream_t st, LaunchPacer *pace) {
     const size_t lds = ((size_t)1 << g.lds_log) * 8;
     if (g.tiles >= ((uint64_t)1 << 31)) return -1;
-    dim3 grid((unsigned)g.tiles);
     const TItem *items = dev + g.first;
     const int n = (int)g.count;
-    switch (g.shape) {
-#define TP_CASE(LR, LC, S)                                                                                  \
-    case shape_key(LR, LC, S):                                                                              \
-        hipLaunchKernelGGL((k_tpass<Geom<LR, LC, S>, KIND>), grid, dim3(TBLOCK), lds, st, items, n, g.tiles, \
-                           tw);                                                                             \
+    const uint64_t piece = pace ? std::max<uint64_t>(pace->max_tiles, 1) : g.tiles;
+    for (uint64_t base = 0; base < g.tiles; base += piece) {
+        if (pace && pace->before() != hipSuccess) return -2;
+        dim3 grid((unsigned)std::min<uint64_t>(piece, g.tiles - base));
+        switch (g.shape) {
+#define TP_CASE(LR, LC, S)                                                                                        \
+    case shape_key(LR, LC, S):                                                                                    \
+        hipLaunchKernelGGL((k_tpass<Geom<LR, LC, S>, KIND>), grid, dim3(TBLOCK), lds, st, items, n, g.tiles, base, \
+                           tw);                                                                                   \
         break;
-        GNT_FOR_EACH_SHAPE(TP_CASE)
+            GNT_FOR_EACH_SHAPE(TP_CASE)
 #undef TP_CASE
-        default: return -1;
+            default: return -1;
+        }
+        if (pace && pace->after() != hipSuccess) return -2;
     }
     return 0;
 }
 
-int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c, const Tw &tw, hipStream_t st) {
+int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c, const Tw &tw, hipStream_t st,
+                     LaunchPacer *pace) {
     for (const Group &g : it.groups) {
         int rc = 0;
-        if (g.kind == K_FWD) rc = launch_tgroup<K_FWD>(g, dev_t, tw, st);
-        else if (g.kind == K_INV) rc = launch_tgroup<K_INV>(g, dev_t, tw, st);
-        else if (g.kind == K_INV_SUMS) rc = launch_tgroup<K_INV_SUMS>(g, dev_t, tw, st);
-        else if (g.kind == K_INV_PLAIN) rc = launch_tgroup<K_INV_PLAIN>(g, dev_t, tw, st);
-        else if (g.kind == K_INV_TSTORE) rc = launch_tgroup<K_INV_TSTORE>(g, dev_t, tw, st);
-        else if (g.kind == K_INV_SUMS_PLAIN) rc = launch_tgroup<K_INV_SUMS_PLAIN>(g, dev_t, tw, st);
-        else if (g.kind == K_FUSED) rc = launch_tgroup<K_FUSED>(g, dev_t, tw, st);
-        else if (g.kind == K_FUSED_AB) rc = launch_tgroup<K_FUSED_AB>(g, dev_t, tw, st);
+        if (g.kind == K_FWD) rc = launch_tgroup<K_FWD>(g, dev_t, tw, st, pace);
+        else if (g.kind == K_INV) rc = launch_tgroup<K_INV>(g, dev_t, tw, st, pace);
+        else if (g.kind == K_INV_SUMS) rc = launch_tgroup<K_INV_SUMS>(g, dev_t, tw, st, pace);
+        else if (g.kind == K_INV_PLAIN) rc = launch_tgroup<K_INV_PLAIN>(g, dev_t, tw, st, pace);
+        else if (g.kind == K_INV_TSTORE) rc = launch_tgroup<K_INV_TSTORE>(g, dev_t, tw, st, pace);
+        else if (g.kind == K_INV_SUMS_PLAIN) rc = launch_tgroup<K_INV_SUMS_PLAIN>(g, dev_t, tw, st, pace);
+        else if (g.kind == K_FUSED) rc = launch_tgroup<K_FUSED>(g, dev_t, tw, st, pace);
+        else if (g.kind == K_FUSED_AB) rc = launch_tgroup<K_FUSED_AB>(g, dev_t, tw, st, pace);
         else {
+            // carry steps: short grid-stride launches, paced whole
+            if (pace && pace->before() != hipSuccess) return -2;
             const CItem *items = dev_c + g.first;
             const int n = (int)g.count;
             dim3 grid((unsigned)std::min<uint64_t>(g.tiles, 65536));
@@ -431,6 +474,7 @@ int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c
                 if (it.want_lens)
                     hipLaunchKernelGGL(k_lens, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, items, n);
             }
+            if (pace && pace->after() != hipSuccess) return -2;
         }
         if (rc) return rc;
     }

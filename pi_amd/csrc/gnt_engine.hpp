@@ -135,7 +135,26 @@ int next_stage_span(const Prod &p);
 bool waits_for_b(const Prod &p);
 // Sort the items into launch groups (kind, radix) and fill tile offsets.
 void finalize_iteration(Iteration &it);
-// Launch one iteration whose items were copied to dev_t / dev_c.
-int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c, const Tw &tw, hipStream_t st);
+// Bounds the work a stream has queued: tile launches are cut into pieces of
+// at most max_tiles workgroups, and before each launch the stream waits (on
+// the host) until fewer than depth earlier launches are unfinished.  Streams
+// may share a hardware queue, whose packets run in order, so a stream that
+// queues a whole large product makes another stream's small launches wait for
+// all of it; paced, they wait for at most depth pieces.
+struct LaunchPacer {
+    uint64_t max_tiles = 4096;
+    int depth = 2;
+    hipStream_t st = nullptr;
+    std::vector<hipEvent_t> inflight;  // oldest first
+    std::vector<hipEvent_t> spare;
+    hipError_t before();  // wait until fewer than depth launches are unfinished
+    hipError_t after();   // mark the launch just issued
+    void release();       // destroy the events (the stream drained)
+};
+
+// Launch one iteration whose items were copied to dev_t / dev_c, paced by
+// `pace` when given (one launch per group otherwise).
+int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c, const Tw &tw, hipStream_t st,
+                     LaunchPacer *pace = nullptr);
 
 }  // namespace gnt

@@ -3,7 +3,8 @@
 // Two lanes, each a scheduler thread with its own stream: products up to a
 // transform length go to the small lane, larger ones to the large lane, so
 // a small request never waits behind a large product's stages while both
-// streams share the device.  Each loop iteration of a lane (1) admits its
+// streams share the device (the large lane keeps little queued on its stream,
+// since the runtime may put both streams on one in-order hardware queue).  Each loop iteration of a lane (1) admits its
 // waiting requests in arrival order while the bytes in flight (both lanes)
 // stay under the HBM cap, (2) advances every admitted product by one stage
 // through the stage engine (one launch per kernel shape for the whole
@@ -17,7 +18,6 @@
 // compute: gnt_submit returns at once, and the uploader moves one slice at a
 // time of the request with the fewest bytes left, so a small request never
 // waits behind the whole upload of a large one.
-#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -119,6 +119,16 @@ struct Lane {
     hipEvent_t ctx_ev = nullptr; // orders admissions after the context stream's work
     size_t max_inflight = 0;     // products advanced at once (0: any number)
     size_t pending_admit = 0;    // admitted in this pass, not yet in `active`
+    // The large-request lane cuts its tile launches into pieces of 4096 tiles
+    // (0.2-0.5 ms at 2^27 elements) and keeps at most two unfinished: the
+    // runtime may give both lanes' streams one hardware queue, whose packets
+    // run in order, and a small request's launches would then wait behind
+    // every launch of the large products queued before them.  Measured at 400
+    // requests/s: p99 of requests below 2^16 limbs 4.7-5.4 ms unpaced,
+    // 1.1-1.4 ms paced (0.7-1.0 ms with pieces of 2048 tiles, which cost 9%
+    // of the throughput at saturation against 4% for 4096).
+    bool paced = false;
+    LaunchPacer pace;
 
     int init(Server *srv, Ctx *ctx);
     void destroy();
@@ -259,29 +269,11 @@ void Server::upload_loop() {
     }
 }
 
-// A stream on a hardware queue of its own.  Streams from hipStreamCreate
-// share the runtime's few hardware queues (four by default), and a kernel
-// queued behind another stream's launches in a shared queue waits for all of
-// them: with a context's two streams already made, a one-workgroup kernel on
-// the next stream waited 3.9 ms behind a 4.4 ms grid on the one after, against
-// 0.05-0.1 ms on a queue of its own.  A stream created with a CU mask always
-// gets its own queue; this mask names every CU.  (Such a stream also orders
-// with the null stream, which the library does not use.)
-static hipError_t own_queue_stream(hipStream_t *st, int device) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) {
-        std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xFFFFFFFFu);
-        if (cus % 32) mask.back() = (1u << (cus % 32)) - 1;
-        if (hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data()) == hipSuccess) return hipSuccess;
-        (void)hipGetLastError();
-    }
-    return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
-}
-
 int Lane::init(Server *srv, Ctx *ctx) {
     s = srv;
     c = ctx;
-    if (own_queue_stream(&stream, c->device) != hipSuccess) return GNT_E_DEVICE;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return GNT_E_DEVICE;
+    pace.st = stream;
     if (hipEventCreateWithFlags(&ctx_ev, hipEventDisableTiming) != hipSuccess) return GNT_E_DEVICE;
     if (hipHostMalloc((void **)&lens, NLEN * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         lens = nullptr;
@@ -292,6 +284,7 @@ int Lane::init(Server *srv, Ctx *ctx) {
 
 void Lane::destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
+    pace.release();
     for (auto &slot : slots) {
         if (slot.ev) (void)hipEventDestroy(slot.ev);
         if (slot.host) (void)hipHostFree(slot.host);
@@ -419,8 +412,8 @@ void Lane::loop() {
     for (;;) {
         // pace with the device: at most two iterations queued, so requests
         // that arrive meanwhile join the next iteration
-        Slot &pace = slots[slot_next];
-        if (pace.used) (void)hipEventSynchronize(pace.ev);
+        Slot &oldest = slots[slot_next];
+        if (oldest.used) (void)hipEventSynchronize(oldest.ev);
         std::vector<Req *> admitted;
         {
             std::unique_lock<std::mutex> g(s->mu);
@@ -456,8 +449,10 @@ void Lane::loop() {
                 else active.push_back(q);
             }
             pending_admit = 0;  // now counted in `active`
-            if (!active.empty()) run_iteration();
         }
+        // the iteration touches only this lane's stream, scratch and slots, so
+        // it runs without the context lock (a paced lane waits in it)
+        if (!active.empty()) run_iteration();
         retire();
     }
 }
@@ -493,7 +488,8 @@ void Lane::run_iteration() {
     std::memcpy(sl.host, it.titems.data(), it.titems.size() * sizeof(TItem));
     std::memcpy((unsigned char *)sl.host + tb, it.citems.data(), it.citems.size() * sizeof(CItem));
     (void)hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, stream);
-    launch_iteration(it, (const TItem *)sl.dev, (const CItem *)((unsigned char *)sl.dev + tb), c->tw(), stream);
+    launch_iteration(it, (const TItem *)sl.dev, (const CItem *)((unsigned char *)sl.dev + tb), c->tw(), stream,
+                     paced ? &pace : nullptr);
     (void)hipEventRecord(sl.ev, stream);
     if (!finishing.empty()) {
         // one event marks every product that finishes in this iteration
@@ -577,6 +573,7 @@ int gnt_serve_start(gnt_ctx *ctx) {
     if (c->server) return GNT_OK;
     gnt::Server *s = new gnt::Server(c, c->serve_split_log_n);
     s->lanes[1].max_inflight = c->serve_large_inflight;
+    s->lanes[1].paced = true;
     for (gnt::Lane &l : s->lanes)
         if (l.init(s, c) != GNT_OK) {
             for (gnt::Lane &m : s->lanes) m.destroy();
