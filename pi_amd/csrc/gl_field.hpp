@@ -53,6 +53,21 @@ GL_HD u64 sub_borrow(u64 a, u64 b, u32 *borrow) {
     return ((u64)hi << 32) | lo;
 }
 
+// The device form of the reduction takes the borrow of lo - hh from
+// v_subbrev_co_u32 and the carry of t + hl * EPS from v_mad_u64_u32's
+// carry-out, as lane masks.  The compiler expresses neither: it recomputes a
+// borrow or a carry with a separate 64-bit compare, and forms hl * EPS and the
+// 64-bit add in two instructions (a general multiply goes from 21 to 18
+// instructions).  Each asm block keeps its masks to itself, declares SCC
+// clobbered where it writes it, and puts every VALU write of a mask two wait
+// states (s_nop 1) ahead of the VALU instruction that reads it.  The host form
+// below the #else is the reference the CPU tests check; the device and host
+// forms are compared on edge values by gnt_calib's field check, and whole
+// products bit for bit by the GPU tests.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GL_DEVICE_ASM 1
+#endif
+
 // (a - b) mod p for a, b in [0, p).  On borrow add p, i.e. subtract EPS
 // (cannot underflow again: a - b + 2^64 >= 2^64 - p + 1 > EPS).
 GL_HD u64 sub(u64 a, u64 b) {
@@ -63,9 +78,55 @@ GL_HD u64 sub(u64 a, u64 b) {
 
 GL_HD u64 neg(u64 a) { return sub(0, a); }
 
+#ifdef GL_DEVICE_ASM
+// (t + hl * EPS) mod p for any t < 2^64, hl < 2^32: s = t + hl * EPS with its
+// carry c (< 2^65 - 2^33); when c, s + EPS < p; otherwise s >= p wraps to
+// s - p by the same + EPS.
+__device__ __forceinline__ u64 add_hl_eps(u64 t, u32 hl) {
+    // s = t + hl * EPS (carry c); g = c or s >= p; s += g * EPS mod 2^64.
+    // The scalar OR of the two masks writes SCC, which the compiler may hold
+    // live across the block (its own selects in single-lane regions), so SCC
+    // is declared clobbered.  The first instruction reads every vector input,
+    // so nothing is early clobbered but the masks.
+    u64 s, c, g, junk;
+    u32 f;
+    asm("v_mad_u64_u32 %0, %1, %5, -1, %6\n\t"
+        "v_cmp_lt_u64_e64 %2, %7, %0\n\t"
+        "s_or_b64 %2, %2, %1\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %4, 0, 1, %2\n\t"
+        "v_mad_u64_u32 %0, %3, %4, -1, %0"
+        : "=v"(s), "=&s"(c), "=&s"(g), "=&s"(junk), "=v"(f)
+        : "v"(hl), "v"(t), "s"(P - 1)
+        : "scc");
+    return s;
+}
+
+// lo - hh as a value in [0, 2^64) congruent mod p: on borrow subtract EPS
+// (lo - hh + 2^64 - EPS = lo - hh + p, in [p - hh, p)).
+__device__ __forceinline__ u64 sub_hh(u64 lo, u32 hh) {
+    u32 r0, r1, e;
+    u64 m, m2;
+    asm("v_sub_co_u32_e64 %0, %3, %5, %7\n\t"
+        "s_nop 1\n\t"
+        "v_subbrev_co_u32_e64 %1, %3, 0, %6, %3\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %2, 0, -1, %3\n\t"
+        "v_sub_co_u32_e64 %0, %4, %0, %2\n\t"
+        "s_nop 1\n\t"
+        "v_subbrev_co_u32_e64 %1, %4, 0, %1, %4"
+        : "=&v"(r0), "=v"(r1), "=v"(e), "=&s"(m), "=&s"(m2)
+        : "v"((u32)lo), "v"((u32)(lo >> 32)), "v"(hh));
+    return ((u64)r1 << 32) | r0;
+}
+#endif
+
 // Reduce hi*2^64 + lo (any 128-bit value) to [0, p):
 //   x = lo + (hh*2^32 + hl)*2^64 = lo - hh + hl*(2^32 - 1)  (mod p)
 GL_HD u64 reduce128(u64 lo, u64 hi) {
+#ifdef GL_DEVICE_ASM
+    return add_hl_eps(sub_hh(lo, (u32)(hi >> 32)), (u32)hi);
+#else
     u64 hh = hi >> 32;
     u64 hl = hi & 0xFFFFFFFFull;
     u32 bw, bw2;
@@ -76,6 +137,7 @@ GL_HD u64 reduce128(u64 lo, u64 hi) {
     bool c = __builtin_add_overflow(t0, t1, &s);    // wrapped: true value s + 2^64 = s + EPS (mod p)
     u32 m = (c | (s >= P)) ? (u32)EPS : 0u;
     return s + (u64)m;                          // lands in [0, p) in both cases
+#endif
 }
 
 // Full 64x64 -> 128 product from four 32x32 -> 64 pieces.
@@ -96,6 +158,10 @@ GL_HD u64 mul(u64 a, u64 b) {
 
 // x * 2^f mod p for 0 < f < 64: the shifted value is hi*2^64 + lo, hi < 2^f.
 GL_HD u64 shl_reduce(u64 x, int f) {
+#ifdef GL_DEVICE_ASM
+    // f <= 32: the high word is below 2^32, so nothing is subtracted
+    if (f <= 32) return add_hl_eps(x << f, (u32)(x >> (64 - f)));
+#endif
     return reduce128(x << f, x >> (64 - f));
 }
 
@@ -159,23 +225,40 @@ GL_HD u32 bitrev(u32 x, int bits) {
 // Register-resident small DFTs.  R in {2, 4, 8, 16, 32, 64}; omega_R = 2^(192/R).
 // ---------------------------------------------------------------------------
 
+// The butterflies are unrolled by template recursion (butterfly K of the
+// level of blocks of M: block b = (K / h) * M, offset j = K % h, h = M / 2), so
+// every register index and twiddle exponent is a compile-time constant
+// whatever the unroller makes of the inline-assembly reduction.
+
+// One level of the forward DIF: blocks of M, butterflies K .. R/2 - 1.
+template <int R, int M, int K = 0>
+GL_HD void dif_level(u64 *v) {
+    if constexpr (K < R / 2) {
+        constexpr int h = M / 2, b = (K / h) * M, j = K % h;
+        const u64 u = v[b + j], w = v[b + j + h];
+        v[b + j] = add(u, w);
+        v[b + j + h] = mul_pow2(sub(u, w), (192 / M) * j);
+        dif_level<R, M, K + 1>(v);
+    }
+}
+
 // Forward DIF: natural-order input, output register q holds frequency
 // bitrev_R(q).  Sum_i v[i] * omega_R^(i*k).  M0 < R starts at the level of
 // blocks of M0 (the levels above done by the caller).
 template <int R, int M0 = R>
 GL_HD void dft_dif(u64 *v) {
-#pragma unroll
-    for (int m = M0; m >= 2; m >>= 1) {
-        const int h = m >> 1;
-#pragma unroll
-        for (int b = 0; b < R; b += m) {
-#pragma unroll
-            for (int j = 0; j < h; j++) {
-                u64 u = v[b + j], w = v[b + j + h];
-                v[b + j] = add(u, w);
-                v[b + j + h] = mul_pow2(sub(u, w), (192 / m) * j);
-            }
-        }
+    if constexpr (M0 >= 2) {
+        dif_level<R, M0>(v);
+        dft_dif<R, M0 / 2>(v);
+    }
+}
+
+// v[j + R/2] = v[j] * omega_R^j for j >= J.
+template <int R, int J = 0>
+GL_HD void upper_from_lower(u64 *v) {
+    if constexpr (J < R / 2) {
+        v[J + R / 2] = mul_pow2(v[J], (192 / R) * J);
+        upper_from_lower<R, J + 1>(v);
     }
 }
 
@@ -183,37 +266,39 @@ GL_HD void dft_dif(u64 *v) {
 // butterflies reduce to v[j + R/2] = v[j] * omega_R^j (v[R/2 ..] not read).
 template <int R>
 GL_HD void dft_dif_upper_zero(u64 *v) {
-#pragma unroll
-    for (int j = 0; j < R / 2; j++) v[j + R / 2] = mul_pow2(v[j], (192 / R) * j);
+    upper_from_lower<R>(v);
     dft_dif<R, R / 2>(v);
+}
+
+// One level of the inverse DIT: blocks of M, butterflies K .. R/2 - 1.
+template <int R, int M, int K = 0>
+GL_HD void dit_level(u64 *v) {
+    if constexpr (K < R / 2) {
+        constexpr int h = M / 2, b = (K / h) * M, j = K % h;
+        constexpr int e = (192 - (192 / M) * j) % 192;
+        const u64 u = v[b + j];
+        // 2^e = -2^(e-96) for e >= 96: multiply by 2^(e-96) and swap the
+        // butterfly's add and subtract instead of negating
+        if constexpr (e >= 96) {
+            const u64 w = mul_pow2(v[b + j + h], e - 96);
+            v[b + j + h] = add(u, w);  // the add first, as below: fewer live registers
+            v[b + j] = sub(u, w);
+        } else {
+            const u64 w = mul_pow2(v[b + j + h], e);
+            v[b + j] = add(u, w);
+            v[b + j + h] = sub(u, w);
+        }
+        dit_level<R, M, K + 1>(v);
+    }
 }
 
 // Inverse DIT (unscaled): register q holds frequency bitrev_R(q); output is
 // natural order, v[i] = Sum_k V[k] * omega_R^(-i*k)  (R times the inverse DFT).
-template <int R>
+template <int R, int M = 2>
 GL_HD void dft_dit_inv(u64 *v) {
-#pragma unroll
-    for (int m = 2; m <= R; m <<= 1) {
-        const int h = m >> 1;
-#pragma unroll
-        for (int b = 0; b < R; b += m) {
-#pragma unroll
-            for (int j = 0; j < h; j++) {
-                const u64 u = v[b + j];
-                const int e = (192 - (192 / m) * j) % 192;
-                // 2^e = -2^(e-96) for e >= 96: multiply by 2^(e-96) and swap
-                // the butterfly's add and subtract instead of negating
-                if (e >= 96) {
-                    const u64 w = mul_pow2(v[b + j + h], e - 96);
-                    v[b + j] = sub(u, w);
-                    v[b + j + h] = add(u, w);
-                } else {
-                    const u64 w = mul_pow2(v[b + j + h], e);
-                    v[b + j] = add(u, w);
-                    v[b + j + h] = sub(u, w);
-                }
-            }
-        }
+    if constexpr (M <= R) {
+        dit_level<R, M>(v);
+        dft_dit_inv<R, M * 2>(v);
     }
 }
 

@@ -17,12 +17,19 @@
 //                             signal each finished tile with a relaxed or a
 //                             device-scope release atomic (what dependencies
 //                             between tiles on different XCDs require)
+//   gnt_calib field           field arithmetic as compiled for the device
+//                             (add, sub, mul, the 128-bit reduction, every
+//                             power-of-two multiply, radix-8/16 register DFTs)
+//                             against the same functions compiled for the
+//                             host, on edge values and random ones; exits 1
+//                             on any difference
 // Prints one JSON line per measurement.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "gl_field.hpp"
 
@@ -191,6 +198,59 @@ static float time_ms(void (*launch)(void *), void *arg, int reps) {
     return ms / reps;
 }
 
+// ---- field check: device forms of the arithmetic against the host forms ----
+
+__host__ __device__ inline u64 canon(u64 x) { return x >= gl::P ? x - gl::P : x; }
+
+constexpr int FIELD_OUTS = 6;
+
+// Per pair (a, b): add and sub of the canonical values, mul and the 128-bit
+// reduction of the raw words, x * 2^e with a run-time e, and the shift
+// reduction with a run-time shift.
+__host__ __device__ inline void field_ops(u64 a, u64 b, u64 *o) {
+    const u64 ca = canon(a), cb = canon(b);
+    o[0] = gl::add(ca, cb);
+    o[1] = gl::sub(ca, cb);
+    o[2] = gl::mul(a, b);
+    o[3] = gl::reduce128(a, b);
+    o[4] = gl::mul_pow2(ca, (int)(b % 192));
+    o[5] = gl::shl_reduce(a, 1 + (int)(b % 63));
+}
+
+// x * 2^e for every e < 192 with e a compile-time constant (the forms the
+// register DFTs use), and a radix-16 forward DFT then inverse DFT of 16 values.
+constexpr int POW2_OUTS = 192;
+__host__ __device__ inline void pow2_ops(u64 x, u64 *o) {
+#pragma unroll
+    for (int e = 0; e < 192; e++) o[e] = gl::mul_pow2(canon(x), e);
+}
+__host__ __device__ inline void dft_ops(const u64 *x, u64 *o) {
+    u64 v[16];
+    for (int i = 0; i < 16; i++) v[i] = canon(x[i]);
+    gl::dft_dif<16>(v);
+    for (int i = 0; i < 16; i++) o[i] = v[i];
+    gl::dft_dit_inv<16>(v);
+    for (int i = 0; i < 16; i++) o[16 + i] = v[i];
+    u64 w[8];
+    for (int i = 0; i < 8; i++) w[i] = canon(x[i]);
+    gl::dft_dif<8>(w);
+    gl::dft_dit_inv<8>(w);
+    for (int i = 0; i < 8; i++) o[32 + i] = w[i];
+}
+
+__global__ __launch_bounds__(256) void k_field(const u64 *a, const u64 *b, size_t n, u64 *out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) field_ops(a[i], b[i], out + i * FIELD_OUTS);
+}
+__global__ __launch_bounds__(256) void k_pow2(const u64 *x, size_t n, u64 *out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) pow2_ops(x[i], out + i * POW2_OUTS);
+}
+__global__ __launch_bounds__(256) void k_dft(const u64 *x, size_t n, u64 *out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dft_ops(x + i * 16, out + i * 40);
+}
+
 struct CopyArg { ulonglong2 *src, *dst; size_t n; };
 static void launch_copy(void *p) {
     CopyArg *c = (CopyArg *)p;
@@ -215,6 +275,94 @@ static void launch_isa(void *p) {
 
 int main(int argc, char **argv) {
     const char *mode = argc > 1 ? argv[1] : "all";
+    if (!strcmp(mode, "field")) {
+        // edge values: around 0, 2^32 - 1 = EPS, 2^32, 2^63, p and 2^64, words
+        // whose halves are 0 or all ones, and values whose high word equals or
+        // exceeds the low word (the borrow cases of the reduction)
+        std::vector<u64> edge = {0, 1, 2, 3, 0x7FFFFFFFull, 0x80000000ull, gl::EPS - 1, gl::EPS, gl::EPS + 1,
+                                 gl::EPS + 2, 1ull << 33, (1ull << 33) - 1, 0x1FFFFFFFFull, 0x100000001ull,
+                                 (1ull << 63) - 1, 1ull << 63, (1ull << 63) + 1, 0x80000000FFFFFFFFull,
+                                 gl::P - 1, gl::P - 2, gl::P - 3, gl::P - gl::EPS, gl::P - gl::EPS - 1,
+                                 gl::P - (1ull << 32), gl::P - (1ull << 33), 0xFFFFFFFE00000000ull,
+                                 0xFFFFFFFEFFFFFFFFull, 0xFFFFFFFE00000001ull, gl::P, gl::P + 1, gl::P + 2,
+                                 ~0ull, ~0ull - 1, ~0ull - gl::EPS, 0x0000000100000000ull * 5 + 4,
+                                 0x0000000100000000ull * 5 + 5, 0x0000000100000000ull * 5 + 6,
+                                 0xFFFFFFFF00000000ull + 0xFFFFFFFEull, 0x12345678ull << 32};
+        std::vector<u64> A, B;
+        for (u64 x : edge)
+            for (u64 y : edge) { A.push_back(x); B.push_back(y); }
+        u64 st = 0x9E3779B97F4A7C15ull;
+        auto rnd = [&st]() {
+            st += 0x9E3779B97F4A7C15ull;
+            u64 z = st;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        // random words, and random words with forced halves (high word = low
+        // word, all-ones or zero halves)
+        for (int k = 0; k < (1 << 20); k++) {
+            u64 x = rnd(), y = rnd();
+            const int f = k & 7;
+            if (f == 1) x = y >> 32;               // lo equal to hh
+            if (f == 2) x = (y >> 32) - 1;         // lo just below hh
+            if (f == 3) x |= 0xFFFFFFFF00000000ull;
+            if (f == 4) y &= 0xFFFFFFFF00000000ull;  // hl = 0
+            if (f == 5) y |= 0xFFFFFFFFull;          // hl = 2^32 - 1
+            A.push_back(x);
+            B.push_back(y);
+        }
+        const size_t n = A.size();
+        u64 *dA, *dB, *dO;
+        CK(hipMalloc(&dA, n * 8));
+        CK(hipMalloc(&dB, n * 8));
+        CK(hipMalloc(&dO, n * 8 * POW2_OUTS));
+        CK(hipMemcpy(dA, A.data(), n * 8, hipMemcpyHostToDevice));
+        CK(hipMemcpy(dB, B.data(), n * 8, hipMemcpyHostToDevice));
+        std::vector<u64> got(n * POW2_OUTS), want(POW2_OUTS);
+        size_t bad[3] = {0, 0, 0}, checked[3] = {0, 0, 0};
+        hipLaunchKernelGGL(k_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dA, dB, n, dO);
+        CK(hipDeviceSynchronize());
+        CK(hipMemcpy(got.data(), dO, n * 8 * FIELD_OUTS, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            field_ops(A[i], B[i], want.data());
+            for (int k = 0; k < FIELD_OUTS; k++, checked[0]++)
+                if (got[i * FIELD_OUTS + k] != want[k] && bad[0]++ < 8)
+                    fprintf(stderr, "field op %d a=%016llx b=%016llx device %016llx host %016llx\n", k,
+                            (unsigned long long)A[i], (unsigned long long)B[i],
+                            (unsigned long long)got[i * FIELD_OUTS + k], (unsigned long long)want[k]);
+        }
+        const size_t np = edge.size() + 65536;
+        hipLaunchKernelGGL(k_pow2, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, 0, dA + (n - np), np, dO);
+        CK(hipDeviceSynchronize());
+        CK(hipMemcpy(got.data(), dO, np * 8 * POW2_OUTS, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < np; i++) {
+            pow2_ops(A[n - np + i], want.data());
+            for (int e = 0; e < POW2_OUTS; e++, checked[1]++)
+                if (got[i * POW2_OUTS + e] != want[e] && bad[1]++ < 8)
+                    fprintf(stderr, "pow2 e=%d x=%016llx device %016llx host %016llx\n", e,
+                            (unsigned long long)A[n - np + i], (unsigned long long)got[i * POW2_OUTS + e],
+                            (unsigned long long)want[e]);
+        }
+        const size_t nd = n / 16;
+        hipLaunchKernelGGL(k_dft, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, 0, dA, nd, dO);
+        CK(hipDeviceSynchronize());
+        CK(hipMemcpy(got.data(), dO, nd * 8 * 40, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nd; i++) {
+            dft_ops(A.data() + i * 16, want.data());
+            for (int k = 0; k < 40; k++, checked[2]++)
+                if (got[i * 40 + k] != want[k] && bad[2]++ < 8)
+                    fprintf(stderr, "dft vector %zu output %d device %016llx host %016llx\n", i, k,
+                            (unsigned long long)got[i * 40 + k], (unsigned long long)want[k]);
+        }
+        printf("{\"calib\": \"field\", \"pairs\": %zu, \"ops_checked\": %zu, \"ops_differing\": %zu, "
+               "\"pow2_checked\": %zu, \"pow2_differing\": %zu, \"dft_checked\": %zu, \"dft_differing\": %zu}\n",
+               n, checked[0], bad[0], checked[1], bad[1], checked[2], bad[2]);
+        CK(hipFree(dA));
+        CK(hipFree(dB));
+        CK(hipFree(dO));
+        return (bad[0] | bad[1] | bad[2]) ? 1 : 0;
+    }
     if (!strcmp(mode, "copy") || !strcmp(mode, "all")) {
         size_t bytes = argc > 2 ? strtoull(argv[2], 0, 10) : ((size_t)1 << 30);
         CopyArg c;

@@ -61,6 +61,27 @@ def test_field_ops_exact(emu):
         assert emu.emu_reduce128(x & (2**64 - 1), x >> 64) == x % P
 
 
+def test_reduce128_borrow_and_carry_edges(emu):
+    """The reduction's corner cases: lo equal to or just below the top word hh
+    (the borrow of lo - hh), hl = 0 and hl = 2^32 - 1 (the carry of
+    t + hl * EPS), sums landing on p and just above it."""
+    rnd = random.Random(128)
+    words = [0, 1, 2, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFE, 0xFFFFFFFF]
+    his = [(h << 32) | l for h in words for l in words]
+    cases = []
+    for hi in his:
+        hh = hi >> 32
+        for lo in {0, 1, hh, (hh - 1) % 2**64, hh + 1, P - 1, P, P + 1, 2**64 - 1, 2**64 - 2**32, 2**32 - 1}:
+            cases.append((lo % 2**64, hi))
+    for _ in range(20000):
+        hi = rnd.randrange(2**64)
+        k = rnd.randrange(4)
+        lo = [hi >> 32, (hi >> 32) - 1, rnd.randrange(2**64), rnd.randrange(2**32)][k] % 2**64
+        cases.append((lo, hi))
+    for lo, hi in cases:
+        assert emu.emu_reduce128(lo, hi) == ((hi << 64) | lo) % P, (hex(lo), hex(hi))
+
+
 def test_shift_multiplies(emu):
     """x * 2^e for every exponent used by the register DFTs (2^192 = 1, 2^96 = -1)."""
     assert pow(2, 96, P) == P - 1 and pow(2, 192, P) == 1
