@@ -41,8 +41,13 @@ typedef struct gnt_config {
                                    small-request lane, larger ones on the large lane; 0 = 22 */
     size_t serve_large_inflight;/* serving: products the large lane advances at once (admitted
                                    in arrival order); 0 = 1, SIZE_MAX = no limit */
-    size_t reserved[2];
+    size_t serve_device_limbs;  /* serving: requests whose result has at most this many limbs
+                                   (products na+nb <= 2048, sums and differences of operands up to
+                                   2047 limbs) run in the device-resident loop; 0 = 2048,
+                                   GNT_SERVE_DEVICE_OFF = none */
+    size_t reserved[1];
 } gnt_config;
+#define GNT_SERVE_DEVICE_OFF ((size_t)-1)
 
 /* ---- lifetime ---- */
 const char *gnt_version(void);
@@ -153,7 +158,11 @@ int gnt_batch_info(gnt_batch *b, size_t *launches_per_run, size_t *iterations, s
 int gnt_batch_destroy(gnt_batch *b);
 
 /* ---- serving loop (continuous batching) ----
- * Two lanes (transforms up to 2^gnt_config.serve_split_log_n, and larger),
+ * Requests with at most gnt_config.serve_device_limbs result limbs run in a
+ * device-resident loop: a persistent kernel whose workgroups claim them from
+ * a ring in pinned host memory and compute each whole in LDS (gnt_submit
+ * copies host operands into the ring; no launch per request).
+ * Other requests go to two lanes (transforms up to 2^gnt_config.serve_split_log_n, and larger),
  * each a scheduler thread on its own stream, advance all their admitted
  * requests one stage per iteration (one launch per kernel shape per
  * iteration) and admit waiting requests, in the order their operands are on
@@ -189,7 +198,8 @@ int gnt_poll(gnt_ctx *ctx, uint64_t ticket, int *done);
  * device round trip in gnt_buf_len); *latency_us = submit-to-completion. */
 int gnt_wait(gnt_ctx *ctx, uint64_t ticket, int64_t timeout_us, gnt_buf **result, double *latency_us);
 /* [0] iterations [1] launches [2] completed [3] peak admitted [4] peak bytes in flight [5] bytes in flight
- * [6] iterations of the small-request lane [7] of the large-request lane (n <= 8 values) */
+ * [6] iterations of the small-request lane [7] of the large-request lane [8] requests run by the
+ * device-resident loop [9] launches of its persistent kernel (n <= 10 values) */
 int gnt_serve_stats(gnt_ctx *ctx, uint64_t *stats, int n);
 
 /* ---- transform plans and the autotuner (no reference counterpart: the

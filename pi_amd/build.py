@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libgnt.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["gnt_kernels.hip", "gnt_engine.hip", "gnt_batch.hip", "gnt_api.hip", "gnt_dispatch.hip", "gnt_serve.hip",
-           "gnt_tune.hip", "gnt_stage.hip", "gnt_noalloc.hip"]
+           "gnt_tune.hip", "gnt_stage.hip", "gnt_noalloc.hip", "gnt_devloop.hip"]
 
 # Packed-FP32 forms are switched off for all device code; the library has no
 # floating point, the flag keeps the compiler from introducing any.

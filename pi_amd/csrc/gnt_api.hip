@@ -388,6 +388,9 @@ int gnt_init(gnt_ctx **out, const gnt_config *cfg) {
     // one large product at a time: overlapping large products share the
     // device and all finish late (measured: p99 at 400 req/s 18 ms against 23)
     c->serve_large_inflight = (cfg && cfg->serve_large_inflight) ? cfg->serve_large_inflight : 1;
+    c->serve_device_limbs = !cfg || !cfg->serve_device_limbs            ? 2048
+                            : cfg->serve_device_limbs == GNT_SERVE_DEVICE_OFF ? 0
+                                                                            : cfg->serve_device_limbs;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;

@@ -53,17 +53,38 @@ struct SumsSrc {
     }
 };
 
+// Limb values of a sum or difference: a + b, or a + ~b + 1 over n_limbs = na
+// limbs (b zero-extended before the complement).
+struct LinearSrc {
+    const uint64_t *a, *b;
+    uint64_t na, nb;
+    bool sub;
+    __device__ __forceinline__ void operator()(uint64_t j, uint64_t &y, uint32_t &g) const {
+        const uint64_t x = j < na ? a[j] : 0;
+        uint64_t v = j < nb ? b[j] : 0;
+        if (sub) v = ~v;
+        y = x + v;
+        g = (uint32_t)(y < x);
+        if (sub && j == 0) {  // the +1 of the two's complement
+            y += 1;
+            g |= (uint32_t)(y == 0);
+        }
+    }
+};
+
 // Phase 1 of carry tile `tile`: limb values from src(j, y, g) for j < n_limbs,
 // carries resolved inside the tile as if none entered it, limbs written to
 // out.  Returns, identical in every thread, the tile's (generate, propagate)
 // pair (gp_combine form) and 1 + the index of its highest nonzero limb (0 if
-// none).  Called by all CARRY_BLOCK threads; `sh` is 2*CARRY_WAVES words of LDS.
-template <class Src>
+// none).  Called by all 64 * WAVES threads of the workgroup; a tile is
+// 64 * WAVES * CARRY_ROWS limbs (CARRY_TILE for the carry kernels' WAVES =
+// CARRY_WAVES); `sh` is 2*WAVES words of LDS.
+template <class Src, int WAVES = CARRY_WAVES>
 __device__ __forceinline__ void carry_tile(const Src &src, uint64_t n_limbs, uint64_t tile,
                                            uint64_t *__restrict__ out, uint32_t *sh, uint32_t &gp_tile,
                                            uint32_t &hi_tile) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t jw = tile * CARRY_TILE + (uint64_t)w * (CARRY_ROWS * 64);
+    const uint64_t jw = tile * (uint64_t)(64 * WAVES * CARRY_ROWS) + (uint64_t)w * (CARRY_ROWS * 64);
     uint64_t y[CARRY_ROWS];
     uint32_t gb = 0, pb = 0;
 #pragma unroll
@@ -102,7 +123,7 @@ __device__ __forceinline__ void carry_tile(const Src &src, uint64_t n_limbs, uin
     __syncthreads();
     uint32_t cw = 0, agg = 2;
 #pragma unroll
-    for (int v = 0; v < CARRY_WAVES; v++) {
+    for (int v = 0; v < WAVES; v++) {
         const uint32_t gp = sh[v];
         if (v < w) cw = (gp & 1) | ((gp >> 1) & cw);
         agg = gp_combine(agg, gp);
@@ -121,11 +142,11 @@ __device__ __forceinline__ void carry_tile(const Src &src, uint64_t n_limbs, uin
         const uint64_t nz = __ballot(in && z != 0);
         if (nz) hi = jw + r * 64 + (63 - __clzll((long long)nz)) + 1;
     }
-    if (lane == 0) sh[CARRY_WAVES + w] = (uint32_t)hi;
+    if (lane == 0) sh[WAVES + w] = (uint32_t)hi;
     __syncthreads();
     uint32_t h = 0;
 #pragma unroll
-    for (int v = 0; v < CARRY_WAVES; v++) h = max(h, sh[CARRY_WAVES + v]);
+    for (int v = 0; v < WAVES; v++) h = max(h, sh[WAVES + v]);
     gp_tile = agg;
     hi_tile = h;
 }

@@ -82,24 +82,6 @@ __global__ __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                           it.n_coeffs, it.n_limbs, st);
 }
 
-// Limb values of a sum or difference: a + b, or a + ~b + 1 over n_limbs = na
-// limbs (b zero-extended before the complement).
-struct LinearSrc {
-    const uint64_t *a, *b;
-    uint64_t na, nb;
-    bool sub;
-    __device__ __forceinline__ void operator()(uint64_t j, uint64_t &y, uint32_t &g) const {
-        const uint64_t x = j < na ? a[j] : 0;
-        uint64_t v = j < nb ? b[j] : 0;
-        if (sub) v = ~v;
-        y = x + v;
-        g = (uint32_t)(y < x);
-        if (sub && j == 0) {  // the +1 of the two's complement
-            y += 1;
-            g |= (uint32_t)(y == 0);
-        }
-    }
-};
 
 __global__ __launch_bounds__(CARRY_BLOCK) void k_ctiles(const CItem *__restrict__ items, int n_items, uint64_t total) {
     __shared__ uint32_t sh[2 * CARRY_WAVES];

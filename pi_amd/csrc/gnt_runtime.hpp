@@ -57,6 +57,7 @@ struct Ctx {
     size_t hbm_cap = 0;
     int serve_split_log_n = 22;  // serving: largest transform (log2) on the small-request lane
     size_t serve_large_inflight = 1;  // serving: products the large lane advances at once
+    size_t serve_device_limbs = 2048; // serving: largest result (limbs) of the device-resident loop; 0 = off
     bool timing = false;
     double phase_ms[5] = {0, 0, 0, 0, 0};
     uint64_t products = 0;

@@ -18,6 +18,11 @@
 // compute: gnt_submit returns at once, and the uploader moves one slice at a
 // time of the request with the fewest bytes left, so a small request never
 // waits behind the whole upload of a large one.
+// Requests small enough to compute in one workgroup's LDS (products of at most
+// 2048 result limbs, sums and differences of up to 2047-limb operands) bypass
+// the lanes: they go to the device-resident loop (gnt_devloop.hpp), a
+// persistent kernel that claims them from a ring in pinned host memory, with
+// host operands copied into the ring's pinned slots by gnt_submit itself.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +35,7 @@
 #include <vector>
 
 #include "../../include/gnt.h"
+#include "gnt_devloop.hpp"
 #include "gnt_engine.hpp"
 #include "gnt_runtime.hpp"
 
@@ -66,6 +72,7 @@ struct Req {
     bool b_arrived = false;                  // B's last slice issued (Server::mu)
     hipEvent_t uploaded_b = nullptr;         // recorded after B's last slice
     void *lane = nullptr;                    // the lane that runs it
+    bool dev = false;                        // runs in the device-resident loop
 };
 
 struct Slot {  // descriptor staging for one iteration in flight
@@ -166,6 +173,10 @@ struct Server {
     // stages only, while the two streams share the device.
     int small_max_log_n;
     Lane lanes[2];
+    // The device-resident loop (null when off) and the largest result (limbs)
+    // it takes.
+    DevLane *dl = nullptr;
+    size_t dl_max_limbs = 0;
 
     explicit Server(Ctx *ctx, int split) : c(ctx), small_max_log_n(split) {}
 
@@ -563,6 +574,89 @@ void Lane::retire() {
 
 Server *server_of(Ctx *c) { return reinterpret_cast<Server *>(c->server); }
 
+// A device-loop request has finished (its result limbs are in device memory).
+static void dl_complete(void *srv, void *owner, uint32_t len) {
+    Server *s = static_cast<Server *>(srv);
+    Req *q = static_cast<Req *>(owner);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        q->len = len;
+        q->t_done = Clock::now();
+        q->done = true;
+        s->completed++;
+    }
+    s->cv_done.notify_all();
+}
+
+// Route q to the device-resident loop when it fits and the ring has a free
+// slot: result buffers from the context's pool (after any context-stream work
+// queued before, through an event the loop waits on), host operands copied
+// into the slot's pinned limbs, device operands by address.  Returns false to
+// leave q to the lanes (nothing allocated then).
+static bool dl_submit(Ctx *c, Server *s, Req *q) {
+    gnt::Prod &p = q->p;
+    const gnt_req &r = q->r;
+    if (!s->dl || p.nl > s->dl_max_limbs || !dl_fits(r.op, p.na, p.nb)) return false;
+    hipEvent_t ready = nullptr;
+    {
+        std::lock_guard<std::mutex> cg(c->mu);
+        p.dout = (u64 *)c->pool.acquire(p.nl * 8);
+        p.dlen = (uint32_t *)c->pool.acquire(4);
+        if (p.dout && p.dlen && hipStreamQuery(c->stream) != hipSuccess &&
+            (hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess ||
+             hipEventRecord(ready, c->stream) != hipSuccess)) {
+            if (ready) (void)hipEventDestroy(ready);
+            ready = nullptr;
+            (void)hipStreamSynchronize(c->stream);  // no event: wait here instead
+        }
+    }
+    uint64_t seq = 0;
+    if (!p.dout || !p.dlen || !s->dl->reserve(q, seq)) {
+        std::lock_guard<std::mutex> cg(c->mu);
+        c->pool.release(p.dout, p.nl * 8);
+        c->pool.release(p.dlen, 4);
+        p.dout = nullptr;
+        p.dlen = nullptr;
+        if (ready) (void)hipEventDestroy(ready);
+        return false;
+    }
+    const bool sq = r.op == GNT_OP_SQR;
+    u64 *ops = s->dl->operands(seq);
+    DevReq d{};
+    u64 *dp;
+    size_t cap;
+    if (r.a_buf) {
+        buf_info(r.a_buf, &dp, &cap);
+        d.a = dp;
+    } else {
+        std::memcpy(ops, r.a, p.na * 8);
+        d.a = ops;
+    }
+    if (sq) {
+        d.b = nullptr;
+    } else if (r.b_buf) {
+        buf_info(r.b_buf, &dp, &cap);
+        d.b = dp;
+    } else {
+        std::memcpy(ops + p.na, r.b, p.nb * 8);
+        d.b = ops + p.na;
+    }
+    d.out = p.dout;
+    d.na = (uint32_t)p.na;
+    d.nb = (uint32_t)p.nb;
+    d.op = (uint32_t)r.op;
+    d.log_n = p.linear() ? 0 : (uint32_t)p.pl.log_n;
+    d.nl = (uint32_t)p.nl;
+    q->dev = true;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        q->ticket = s->next_ticket++;
+        s->tickets[q->ticket] = q;
+    }
+    s->dl->stage(seq, d, ready);
+    return true;
+}
+
 }  // namespace gnt
 
 extern "C" {
@@ -580,6 +674,17 @@ int gnt_serve_start(gnt_ctx *ctx) {
             delete s;
             return gnt::set_error(GNT_E_DEVICE, "serving lane setup failed");
         }
+    if (c->serve_device_limbs) {
+        s->dl = new gnt::DevLane();
+        s->dl_max_limbs = std::min(c->serve_device_limbs, gnt::DL_MAX_LIMBS);
+        if (s->dl->init(c->device, c->tw(), gnt::dl_complete, s) != GNT_OK) {
+            s->dl->destroy();
+            delete s->dl;
+            for (gnt::Lane &m : s->lanes) m.destroy();
+            delete s;
+            return gnt::set_error(GNT_E_DEVICE, "device-resident loop setup failed");
+        }
+    }
     c->server = s;
     for (gnt::Lane &l : s->lanes) l.th = std::thread([&l] { l.loop(); });
     s->up_th = std::thread([s] { s->upload_loop(); });
@@ -590,6 +695,12 @@ int gnt_serve_stop(gnt_ctx *ctx) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c || !c->server) return GNT_OK;
     gnt::Server *s = gnt::server_of(c);
+    // the device-resident loop finishes its requests and its last launch ends
+    if (s->dl) {
+        s->dl->destroy();
+        delete s->dl;
+        s->dl = nullptr;
+    }
     // pending uploads finish and reach their lanes before the lanes drain
     {
         std::lock_guard<std::mutex> g(s->mu);
@@ -658,6 +769,10 @@ int gnt_submit(gnt_ctx *ctx, const gnt_req *req, uint64_t *ticket) {
         return gnt::set_error(GNT_E_NOMEM, "request exceeds the HBM admission cap");
     }
     q->t_submit = Clock::now();
+    if (gnt::dl_submit(c, s, q)) {  // small enough for the device-resident loop
+        *ticket = q->ticket;
+        return GNT_OK;
+    }
     // Device buffers for host operands now (an allocation failure is this
     // call's error); the uploader thread fills them.  They count against the
     // cap once admitted.
@@ -735,9 +850,16 @@ int gnt_serve_stats(gnt_ctx *ctx, uint64_t *stats, int n) {
     if (!c->server) return gnt::set_error(GNT_E_STATE, "serving loop not running");
     gnt::Server *s = gnt::server_of(c);
     std::lock_guard<std::mutex> g(s->mu);
-    uint64_t v[8] = {s->iterations, s->launches,       s->completed,         s->peak_active,
-                     s->peak_bytes, s->bytes_in_flight, s->lane_iterations[0], s->lane_iterations[1]};
-    for (int i = 0; i < n && i < 8; i++) stats[i] = v[i];
+    uint64_t dl_req = 0, dl_launch = 0;
+    if (s->dl) {
+        std::lock_guard<std::mutex> dg(s->dl->mu);
+        dl_req = s->dl->requests;
+        dl_launch = s->dl->launches;
+    }
+    uint64_t v[10] = {s->iterations, s->launches,       s->completed,         s->peak_active,
+                      s->peak_bytes, s->bytes_in_flight, s->lane_iterations[0], s->lane_iterations[1],
+                      dl_req,        dl_launch};
+    for (int i = 0; i < n && i < 10; i++) stats[i] = v[i];
     return GNT_OK;
 }
 

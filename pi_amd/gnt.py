@@ -50,7 +50,10 @@ class GntError(RuntimeError):
 
 class GntConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_cap_bytes", SZ), ("plan_cache_path", ctypes.c_char_p),
-                ("serve_split_log_n", SZ), ("serve_large_inflight", SZ), ("reserved", SZ * 2)]
+                ("serve_split_log_n", SZ), ("serve_large_inflight", SZ), ("serve_device_limbs", SZ),
+                ("reserved", SZ * 1)]
+
+SERVE_DEVICE_OFF = 2**64 - 1  # gnt_config.serve_device_limbs: no device-resident loop
 
 
 class GntReq(ctypes.Structure):
@@ -171,11 +174,12 @@ class Context:
     """One device context (gnt_ctx)."""
 
     def __init__(self, device: int = 0, hbm_cap_bytes: int = 0, plan_cache: Optional[str] = None,
-                 serve_split_log_n: int = 0, serve_large_inflight: int = 0):
+                 serve_split_log_n: int = 0, serve_large_inflight: int = 0, serve_device_limbs: int = 0):
         L = load()
         cfg = GntConfig(device=device, hbm_cap_bytes=hbm_cap_bytes,
                         plan_cache_path=plan_cache.encode() if plan_cache else None,
-                        serve_split_log_n=serve_split_log_n, serve_large_inflight=serve_large_inflight)
+                        serve_split_log_n=serve_split_log_n, serve_large_inflight=serve_large_inflight,
+                        serve_device_limbs=serve_device_limbs)
         h = ctypes.c_void_p()
         _check(L.gnt_init(ctypes.byref(h), ctypes.byref(cfg)))
         self.h = h
@@ -361,10 +365,11 @@ class Server:
         return DeviceBuf(self.ctx, h), lat.value
 
     def stats(self):
-        v = (ctypes.c_uint64 * 8)()
-        _check(load().gnt_serve_stats(self.ctx.h, v, 8))
+        v = (ctypes.c_uint64 * 10)()
+        _check(load().gnt_serve_stats(self.ctx.h, v, 10))
         return dict(zip(["iterations", "launches", "completed", "peak_admitted", "peak_bytes", "bytes_in_flight",
-                         "small_lane_iterations", "large_lane_iterations"], list(v)))
+                         "small_lane_iterations", "large_lane_iterations", "device_loop_requests",
+                         "device_loop_launches"], list(v)))
 
     def stop(self):
         _check(load().gnt_serve_stop(self.ctx.h))

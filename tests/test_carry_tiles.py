@@ -50,11 +50,13 @@ def _bits(flags):
     return sum(1 << i for i, f in enumerate(flags) if f)
 
 
-def carry_tile(mode, n_limbs, A, B, tile, out):
-    """One tile as the kernel works it: returns ((generate, propagate), hi)."""
+def carry_tile(mode, n_limbs, A, B, tile, out, nwaves=WAVES):
+    """One tile as the kernel works it (nwaves waves of 64 lanes): returns
+    ((generate, propagate), hi)."""
+    tile_limbs = nwaves * ROWS * 64
     waves = []
-    for w in range(WAVES):
-        jw = tile * TILE + w * ROWS * 64
+    for w in range(nwaves):
+        jw = tile * tile_limbs + w * ROWS * 64
         rows = []
         for r in range(ROWS):
             vals = [_limb_values(mode, n_limbs, A, B, jw + r * 64 + lane) for lane in range(64)]
@@ -71,7 +73,7 @@ def carry_tile(mode, n_limbs, A, B, tile, out):
     # second pass: every wave with its carry-in
     cw, agg_g, agg_p, hi = 0, 0, 1, 0
     for w, (rows, g, p) in enumerate(waves):
-        jw = tile * TILE + w * ROWS * 64
+        jw = tile * tile_limbs + w * ROWS * 64
         c = cw
         for r, (y, m0, pre, co, rp) in enumerate(rows):
             for lane in range(64):
@@ -192,6 +194,26 @@ def _cases():
         ("sub", MASK | (0xDEADBEEF << 64), 0x1234 << 64),
         ("sub", MASK | (X << 64), (X - 1) << 64),
     ]
+
+
+@pytest.mark.parametrize("i", range(12))
+def test_one_tile_of_sixteen_waves(i):
+    """The device-resident loop's workgroup (16 waves) takes a whole request in
+    one carry tile of 8192 limbs (no tile scan, no ripple): the same cases,
+    cut to that size."""
+    op, a, b = _cases()[i]
+    a &= (1 << (64 * 8191)) - 1
+    b &= (1 << (64 * 8191)) - 1
+    if op == "sub" and b > a:
+        a, b = b, a
+    A, B = to_limbs(a), to_limbs(b)
+    n = max(len(A), 1) if op == "sub" else max(len(A), len(B)) + 1
+    out = [0] * n
+    (g, p), hi = carry_tile(SUB if op == "sub" else ADD, n, A, B, 0, out, nwaves=16)
+    want = (a - b) if op == "sub" else (a + b)
+    assert value(out) == want
+    assert hi == len(to_limbs(want))
+
 
 
 @pytest.mark.parametrize("i", range(12))

@@ -61,7 +61,7 @@ def test_scc_checker_flags_an_undeclared_write():
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-@pytest.mark.parametrize("src", ["gnt_kernels.hip", "gnt_engine.hip"])
+@pytest.mark.parametrize("src", ["gnt_kernels.hip", "gnt_engine.hip", "gnt_devloop.hip"])
 def test_no_scc_read_after_an_asm_block_writes_it(src, tmp_path):
     out = tmp_path / "k.s"
     cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
