@@ -660,7 +660,7 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
     ready = []                      # finished results kept on the device for reuse
     consumed, sampled = [], []
     lat, lat_open, limbs_done = [], [], [0]
-    tickets, meta, late, size_of, sizes = [], {}, {}, {}, []
+    tickets, meta, late, size_of, sizes, small = [], {}, {}, {}, [], []
     done_evt = threading.Event()
 
     def waiter():
@@ -673,6 +673,7 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
             lat_open.append(us + late[t])
             sizes.append(size_of[t])
             n_in, check, keep = meta[t]
+            small.append(n_in <= DEVICE_LOOP_LIMBS)
             limbs_done[0] += n_in
             if check is not None:
                 sampled.append((buf, check))
@@ -741,16 +742,27 @@ def serve_rate(srv, src, rng, rate: float, requests: int) -> dict:
         buf.release()
     lat_ms = np.array(lat_open) / 1e3
     sub_ms = np.array(lat) / 1e3
+    sm = np.array(small, dtype=bool)
+    loop = {"requests": int(sm.sum())}
+    if sm.any():
+        loop.update(p50_ms=round(float(np.percentile(lat_ms[sm], 50)), 3),
+                    p99_ms=round(float(np.percentile(lat_ms[sm], 99)), 3))
     return {"rate_req_per_s": rate, "requests": requests,
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
             "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
             "submit_to_done_p50_ms": round(float(np.percentile(sub_ms, 50)), 3),
             "submit_to_done_p99_ms": round(float(np.percentile(sub_ms, 99)), 3),
             "by_size": _latency_by_size(lat_ms, np.array(sizes)),
+            # requests whose result has at most 2048 limbs (the device-resident loop's)
+            "result_le_2048_limbs": loop,
             "throughput_Mlimbs_per_s": round(limbs_done[0] / wall / 1e6, 1),
             "achieved_req_per_s": round(requests / wall, 1),
             "exact_checks_passed": f"{ok}/{len(sampled)}"}
 
+
+# Largest result (limbs) of the serving loop's device-resident path (the
+# library default of gnt_config.serve_device_limbs).
+DEVICE_LOOP_LIMBS = 2048
 
 SERVE_WORKLOAD = ("open-loop Poisson; n = round(2^U[8,24]); 70/30 mul/sqr; 20% reuse of device results; "
                   "HBM cap 200 GB")

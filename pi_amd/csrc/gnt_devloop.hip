@@ -199,6 +199,7 @@ void DevLane::stage(uint64_t seq, const DevReq &d, hipEvent_t ready) {
         it.ready = ready;
         it.staged = true;
         requests++;
+        publish_locked();  // from the submitting thread: no hand-off before the device sees it
     }
     cv.notify_one();
 }
@@ -226,39 +227,45 @@ int DevLane::launch() {
 }
 
 // Publish staged requests in sequence order (each after its context-stream
-// event), keep a launch alive while published requests are unfinished, and
-// complete requests whose completion word has appeared.  While requests are
-// unfinished the thread polls; otherwise it sleeps on the condition variable.
+// event), and launch the kernel when published requests are unfinished and
+// the previous launch has ended (requests published as its workgroups left
+// would otherwise wait for the next one).  mu held.
+void DevLane::publish_locked() {
+    uint64_t p = published;
+    while (p < next) {
+        Item &it = items[p % DL_SLOTS];
+        if (!it.staged) break;
+        if (it.ready) {
+            if (hipEventQuery(it.ready) != hipSuccess) break;
+            (void)hipEventDestroy(it.ready);
+            it.ready = nullptr;
+        }
+        p++;
+    }
+    if (p != published) {
+        published = p;
+        __atomic_store_n(tail, p, __ATOMIC_RELEASE);
+    }
+    if (published > finished && (!launched || hipEventQuery(ended) == hipSuccess)) (void)launch();
+}
+
+// Publishes what the submitting threads could not (requests behind a
+// context-stream event), keeps a launch alive, and completes requests whose
+// completion word has appeared, oldest first.  While requests are unfinished
+// the thread polls, yielding between rounds; otherwise it sleeps on the
+// condition variable.
 void DevLane::loop() {
     (void)hipSetDevice(device);
+    std::vector<std::pair<void *, uint32_t>> fin;
     for (;;) {
-        std::vector<std::pair<void *, uint32_t>> fin;
-        bool progress = false;
+        fin.clear();
         {
             std::unique_lock<std::mutex> g(mu);
-            // something to complete, or the next request to publish is staged
             cv.wait(g, [&] {
                 return quit || published > finished || (published < next && items[published % DL_SLOTS].staged);
             });
             if (quit && next == finished) return;
-            // publish in order
-            uint64_t p = published;
-            while (p < next) {
-                Item &it = items[p % DL_SLOTS];
-                if (!it.staged) break;
-                if (it.ready) {
-                    if (hipEventQuery(it.ready) != hipSuccess) break;
-                    (void)hipEventDestroy(it.ready);
-                    it.ready = nullptr;
-                }
-                p++;
-            }
-            if (p != published) {
-                published = p;
-                __atomic_store_n(tail, p, __ATOMIC_RELEASE);
-                progress = true;
-            }
-            // finished requests, oldest first
+            publish_locked();
             while (finished < published) {
                 const uint64_t w = __atomic_load_n(&done[finished % DL_SLOTS], __ATOMIC_ACQUIRE);
                 if ((w & 0xFFFFFFFFull) != ((finished + 1) & 0xFFFFFFFFull)) break;
@@ -266,18 +273,9 @@ void DevLane::loop() {
                 items[finished % DL_SLOTS].owner = nullptr;
                 finished++;
             }
-            if (!fin.empty()) progress = true;
-        }
-        // a launch that has ended while published requests are unclaimed or
-        // unfinished: launch again (requests published as its workgroups left)
-        if (published > finished && (!launched || hipEventQuery(ended) == hipSuccess)) {
-            if (launch() != GNT_OK) {
-                // nothing else can run them: leave them unfinished and retry
-                std::this_thread::sleep_for(std::chrono::microseconds(100));
-            }
         }
         for (auto &f : fin) complete(server, f.first, f.second);
-        if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(2));
+        if (fin.empty()) std::this_thread::yield();
     }
 }
 

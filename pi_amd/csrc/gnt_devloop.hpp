@@ -82,9 +82,11 @@ inline bool dl_fits(int op, size_t na, size_t nb) {
     return na <= DL_MAX_LIMBS;                                          // GNT_OP_SUB
 }
 
-// Host side: the ring in pinned memory, the kernel's stream, and a thread
-// that publishes staged requests in sequence order, keeps a launch alive
-// while any is unfinished, and completes requests as their words appear.
+// Host side: the ring in pinned memory and the kernel's stream.  Submitting
+// threads publish their own requests (in sequence order) and launch the
+// kernel when no launch is running; a lane thread publishes requests that had
+// to wait for context-stream work, relaunches while requests are unfinished,
+// and completes requests as their words appear.
 struct DevLane {
     struct Item {
         void *owner = nullptr;       // the serving request
@@ -123,7 +125,8 @@ struct DevLane {
     void stage(uint64_t seq, const DevReq &d, hipEvent_t ready);
     size_t unfinished();
     void loop();
-    int launch();
+    void publish_locked();
+    int launch();  // mu held
 };
 
 }  // namespace gnt
