@@ -140,10 +140,12 @@ int gnt_mul_batch(gnt_ctx *ctx, gnt_job *jobs, size_t n);
 int gnt_mul_batch_serial(gnt_ctx *ctx, gnt_job *jobs, size_t n);
 
 /* Batch plans: a fixed set of product shapes with device-resident operand,
- * scratch and result buffers, scheduled once (every level of every product
- * grouped into one launch per kernel shape) and optionally captured as a
- * hipGraph; replay with new operands of the same shapes.  No reference
- * counterpart: they serve the per-level products of binary splitting. */
+ * scratch and result buffers, scheduled once (products of at most 2048 result
+ * limbs in one persistent launch whose workgroups claim them and compute each
+ * whole in LDS; every level of the other products grouped into one launch per
+ * kernel shape) and optionally captured as a hipGraph; replay with new
+ * operands of the same shapes.  No reference counterpart: they serve the
+ * per-level products of binary splitting. */
 typedef struct gnt_batch gnt_batch;
 int gnt_batch_create(gnt_ctx *ctx, const size_t *na, const size_t *nb, const int *is_sqr, size_t n, int use_graph,
                      gnt_batch **out);

@@ -1,6 +1,9 @@
-// Batched products: a plan for a fixed set of product shapes, run through the
-// stage engine (gnt_engine.hpp) as level-grouped launches over device
-// descriptor tables, optionally captured once as a hipGraph and replayed.
+// Batched products: a plan for a fixed set of product shapes.  Products small
+// enough for one workgroup's LDS (at most 2048 result limbs) run in one
+// persistent launch whose workgroups claim them one after another
+// (gnt_devloop.hpp); the others run through the stage engine (gnt_engine.hpp)
+// as level-grouped launches over device descriptor tables.  The whole run is
+// optionally captured once as a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -8,6 +11,7 @@
 #include <vector>
 
 #include "../../include/gnt.h"
+#include "gnt_devloop.hpp"
 #include "gnt_engine.hpp"
 #include "gnt_kernels.hpp"
 #include "gnt_runtime.hpp"
@@ -32,6 +36,9 @@ struct gnt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     size_t scratch_bytes = 0;
+    // products of the persistent launch: descriptors and claim counter in the blob
+    uint32_t n_small = 0;
+    size_t small_off = 0, next_off = 0;  // byte offsets in blob
 };
 
 namespace gnt {
@@ -52,6 +59,13 @@ static void release_batch(gnt_batch *b) {
     }
     c->pool.release(b->blob, b->blob_bytes);
     delete b;
+}
+
+// The persistent launch over the plan's small products (none: nothing).
+static int launch_small(gnt_batch *b) {
+    unsigned char *blob = (unsigned char *)b->blob;
+    return launch_devbatch((const DevReq *)(blob + b->small_off), b->n_small, (uint32_t *)(blob + b->next_off),
+                           b->c->tw(), b->c->stream);
 }
 
 // Allocate buffers, build the schedule and upload the descriptor tables.
@@ -91,6 +105,7 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
             return set_error(GNT_E_NOMEM, "device memory exhausted");
         }
     }
+    std::vector<DevReq> small;
     for (size_t i = 0; i < n; i++) {
         Prod &p = b->prods[i];
         const bool sq = p.sqr();
@@ -98,9 +113,26 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
         if (!sq) b->in_b[i] = (u64 *)((char *)b->in_blob + b_off[i]);
         p.da = b->in_a[i];
         p.db = sq ? nullptr : b->in_b[i];
-        c->pass_tables(p.pl, p.tabs);
         p.dout = (u64 *)((char *)b->out_blob + b->out_off[i]);
         p.dlen = b->len_blob + i;
+        if (dl_fits(sq ? GNT_OP_SQR : GNT_OP_MUL, p.na, p.nb)) {
+            // whole in one workgroup of the persistent launch: no scratch, no stages
+            DevReq d{};
+            d.a = p.da;
+            d.b = p.db;
+            d.out = p.dout;
+            d.len = p.dlen;
+            d.na = (uint32_t)p.na;
+            d.nb = (uint32_t)p.nb;
+            d.op = sq ? GNT_OP_SQR : GNT_OP_MUL;
+            d.log_n = (uint32_t)p.pl.log_n;
+            d.nl = (uint32_t)p.nl;
+            small.push_back(d);
+            p.stage = p.nstages;
+            b->scratch_bytes += p.na * 8 + p.nb * 8 + p.nl * 8;
+            continue;
+        }
+        c->pass_tables(p.pl, p.tabs);
         p.xa = (u64 *)c->pool.acquire(p.N * 8);
         if (!sq) p.xb = (u64 *)c->pool.acquire(p.N * 8);
         p.lh = (u64 *)c->pool.acquire(p.nl * 16);
@@ -126,7 +158,10 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
         b->iters.push_back(std::move(it));
     }
     const size_t tbytes = (nt * sizeof(TItem) + 255) & ~(size_t)255;
-    b->blob_bytes = tbytes + nc * sizeof(CItem) + 256;
+    b->n_small = (uint32_t)small.size();
+    b->small_off = (tbytes + nc * sizeof(CItem) + 255) & ~(size_t)255;
+    b->next_off = b->small_off + small.size() * sizeof(DevReq);
+    b->blob_bytes = b->next_off + 256;
     b->blob = c->pool.acquire(b->blob_bytes);
     if (!b->blob) { release_batch(b); return set_error(GNT_E_NOMEM, "device memory exhausted"); }
     std::vector<unsigned char> host(b->blob_bytes, 0);
@@ -138,6 +173,7 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
             std::memcpy(host.data() + tbytes + b->c_off[k] * sizeof(CItem), it.citems.data(),
                         it.citems.size() * sizeof(CItem));
     }
+    if (!small.empty()) std::memcpy(host.data() + b->small_off, small.data(), small.size() * sizeof(DevReq));
     // on the context stream, which orders the copy after earlier users of the
     // pooled buffer; the host vector lives until the copy is done
     if (hipMemcpyAsync(b->blob, host.data(), b->blob_bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
@@ -150,6 +186,7 @@ static int create_batch(Ctx *c, const size_t *na, const size_t *nb, const int *i
         const CItem *dc = (const CItem *)((unsigned char *)b->blob + tbytes);
         int rc = 0;
         if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = -1;
+        if (!rc) rc = launch_small(b);
         for (size_t k = 0; !rc && k < b->iters.size(); k++)
             rc = launch_iteration(b->iters[k], dt + b->t_off[k], dc + b->c_off[k], c->tw(), c->stream);
         hipGraph_t g = nullptr;
@@ -170,6 +207,7 @@ static int run_batch_plan(gnt_batch *b) {
     Ctx *c = b->c;
     if (b->exec) return hipGraphLaunch(b->exec, c->stream) == hipSuccess ? GNT_OK
                                                                          : set_error(GNT_E_DEVICE, "graph launch failed");
+    if (launch_small(b)) return set_error(GNT_E_DEVICE, "batch launch failed");
     size_t nt = b->t_off.empty() ? 0 : b->t_off.back() + b->iters.back().titems.size();
     const size_t tbytes = (nt * sizeof(TItem) + 255) & ~(size_t)255;
     const TItem *dt = (const TItem *)b->blob;
@@ -327,7 +365,7 @@ int gnt_batch_download(gnt_batch *b, gnt_job *jobs) {
 
 int gnt_batch_info(gnt_batch *b, size_t *launches_per_run, size_t *iterations, size_t *device_bytes) {
     if (!b) return gnt::set_error(GNT_E_ARG, "null batch");
-    size_t L = 0;
+    size_t L = b->n_small ? 2 : 0;  // the counter reset and the persistent launch
     for (const auto &it : b->iters) L += it.groups.size();
     if (launches_per_run) *launches_per_run = L;
     if (iterations) *iterations = b->iters.size();

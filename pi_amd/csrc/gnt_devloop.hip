@@ -126,6 +126,26 @@ __global__ __launch_bounds__(DL_BLOCK) void k_devloop(DevRing ring, Tw tw) {
     }
 }
 
+// Batched: claim requests through *next until none is left.
+__global__ __launch_bounds__(DL_BLOCK) void k_devbatch(const DevReq *__restrict__ reqs, uint32_t n, uint32_t *next,
+                                                       Tw tw) {
+    __shared__ uint64_t lds[2 << DL_MAX_LOG_N];
+    __shared__ uint32_t sh[2 * DL_WAVES];
+    __shared__ uint32_t cur;
+    for (;;) {
+        if (threadIdx.x == 0) cur = atomicAdd(next, 1u);
+        __syncthreads();
+        const uint32_t i = cur;
+        if (i >= n) return;  // uniform
+        const DevReq r = reqs[i];
+        const uint32_t len = dl_run(lds, lds + (1 << DL_MAX_LOG_N), r, tw, sh);
+        if (threadIdx.x == 0) *r.len = len;
+        __syncthreads();  // cur and the tiles are rewritten by the next claim
+    }
+}
+
+__global__ void k_devbatch_reset(uint32_t *next) { *next = 0; }
+
 template <class T>
 int pinned(T **p, size_t n) {
     void *h = nullptr;
@@ -136,6 +156,13 @@ int pinned(T **p, size_t n) {
 }
 
 }  // namespace
+
+int launch_devbatch(const DevReq *reqs, uint32_t n, uint32_t *next, const Tw &tw, hipStream_t st) {
+    if (n == 0) return GNT_OK;
+    hipLaunchKernelGGL(k_devbatch_reset, dim3(1), dim3(1), 0, st, next);
+    hipLaunchKernelGGL(k_devbatch, dim3(devbatch_groups(n)), dim3(DL_BLOCK), 0, st, reqs, n, next, tw);
+    return hipGetLastError() == hipSuccess ? GNT_OK : GNT_E_DEVICE;
+}
 
 int DevLane::init(int dev, const Tw &t, void (*cb)(void *, void *, uint32_t), void *srv) {
     device = dev;

@@ -52,15 +52,18 @@ static_assert(DL_MAX_LIMBS <= (size_t)64 * DL_WAVES * CARRY_ROWS, "one carry til
 
 // One request (written by the host before publication).  a, b: operand limbs
 // in device memory or pinned host memory (b null squares); out: result limbs
-// in device memory, nl of them; log_n: the product's transform length.
+// in device memory, nl of them; len: where the trimmed length goes (device
+// memory; null in the serving ring, whose completion word carries it); log_n:
+// the product's transform length.
 struct DevReq {
     const uint64_t *a, *b;
     uint64_t *out;
+    uint32_t *len;
     uint32_t na, nb;
     uint32_t op;  // GNT_OP_*
     uint32_t log_n;
     uint32_t nl;
-    uint32_t pad[5];
+    uint32_t pad[3];
 };
 static_assert(sizeof(DevReq) == 64, "descriptor of one cache line");
 
@@ -81,6 +84,16 @@ inline bool dl_fits(int op, size_t na, size_t nb) {
     if (op == 2) return (na > nb ? na : nb) + 1 <= DL_MAX_LIMBS;        // GNT_OP_ADD
     return na <= DL_MAX_LIMBS;                                          // GNT_OP_SUB
 }
+
+// The batched form of the same routine: one persistent launch whose
+// workgroups claim the n requests of reqs (device memory) through the counter
+// *next, each computed whole in LDS with its trimmed length written to
+// req.len; workgroups leave when no request is left.  *next is zeroed by a
+// kernel launched first on the same stream (no memset node: the pair may be
+// captured in a graph and replayed).
+int launch_devbatch(const DevReq *reqs, uint32_t n, uint32_t *next, const Tw &tw, hipStream_t st);
+// Workgroups launch_devbatch uses for n requests.
+inline uint32_t devbatch_groups(uint32_t n) { return n < 256 ? n : 256; }
 
 // Host side: the ring in pinned memory and the kernel's stream.  Submitting
 // threads publish their own requests (in sequence order) and launch the

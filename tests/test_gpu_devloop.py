@@ -177,3 +177,49 @@ def test_device_loop_many_submitters():
         c.close()
     for (a, b), v in zip(jobs, out):
         assert val(v) == a * b
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_of_small_products_is_one_persistent_launch(graph):
+    """A batch plan whose products all fit one workgroup (at most 2048 result
+    limbs) runs as one persistent launch (and the kernel that zeroes its claim
+    counter); replayed three times with new operands and eager work between
+    the replays, every replay exact.  A plan mixing small and large products
+    takes the persistent launch beside the level-grouped ones."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(12)
+    shapes = [(1, 1), (1, 2047), (1024, 1024), (700, None), (1024, None), (3, 5)]
+    shapes += [(int(2 ** rng.uniform(0, 10)), None if i % 3 == 0 else int(2 ** rng.uniform(0, 10)))
+               for i in range(90)]
+
+    def draw(seed):
+        return [(orc.splitmix64(seed + 2 * i, na), None if nb is None else orc.splitmix64(seed + 2 * i + 1, nb))
+                for i, (na, nb) in enumerate(shapes)]
+
+    c = gnt.Context(0)
+    try:
+        pairs = draw(1000)
+        plan = c.batch_plan(pairs, graph=graph)
+        assert plan.info()["launches_per_run"] == 2, plan.info()
+        for rep in range(3):
+            print(f"replay {rep}", flush=True)
+            if rep:
+                pairs = draw(1000 + 500 * rep)
+                jobs, keep, outs = c._jobs(pairs)
+                gnt._check(gnt.load().gnt_batch_upload(plan.h, jobs))
+            plan.run()
+            got = plan.results()
+            x, y = orc.splitmix64(rep, 3000), orc.splitmix64(rep + 9, 2000)
+            assert np.array_equal(c.mul_mag_ntt(x, y), orc.gmp_mul(x, y))   # eager work between replays
+            for (a, b), g in zip(pairs, got):
+                assert np.array_equal(g, orc.gmp_sqr(a) if b is None else orc.gmp_mul(a, b))
+        plan.close()
+        mixed = draw(7) + [(orc.splitmix64(5, 3000), orc.splitmix64(6, 5000)), (orc.splitmix64(8, 40000), None)]
+        plan = c.batch_plan(mixed, graph=graph)
+        plan.run()
+        for (a, b), g in zip(mixed, plan.results()):
+            assert np.array_equal(g, orc.gmp_sqr(a) if b is None else orc.gmp_mul(a, b))
+        assert plan.info()["launches_per_run"] > 2
+        plan.close()
+    finally:
+        c.close()
