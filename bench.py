@@ -314,12 +314,14 @@ def main():
     ap.add_argument("--cpu-timed", type=int, default=3, help="timed CPU-baseline products after the warm-up")
     ap.add_argument("--no-batched", action="store_true", help="single: skip the 1024-pair batched run")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky", "c2", "copies"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "serve", "chudnovsky", "c2", "copies", "small"],
+                    default="single",
                     help="single: the metric (one 2^24 product per step); batched: 1024 uneven pairs per step; "
                          "serve: open-loop Poisson request stream through the serving loop; "
                          "chudnovsky: pi to --digits digits end to end; "
                          "c2: mul and sqr at 2^20 and 2^24 limbs over three operand distributions; "
-                         "copies: host <-> device copy rates and the PCIe-inclusive host-buffer multiply")
+                         "copies: host <-> device copy rates and the PCIe-inclusive host-buffer multiply; "
+                         "small: closed-loop latency of small served products, device-resident loop against the lanes")
     ap.add_argument("--rates", default="100,200,400,800,1600", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--pairs", type=int, default=1024)
@@ -359,6 +361,8 @@ def main():
         return bench_c2(args)
     if args.workload == "copies":
         return bench_copies(args)
+    if args.workload == "small":
+        return bench_small(args)
 
     n = 1 << args.log_size
     a, b = operand(3, n), operand(4, n)
@@ -791,6 +795,60 @@ def bench_serve(args):
                    "workload": SERVE_WORKLOAD, "sweep": points, "server": stats},
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_small(args):
+    """Closed-loop latency of small served products, one request at a time:
+    n x n-limb multiplies (n = 64, 512, 1024; results of at most 2048 limbs)
+    through the device-resident loop and through the lanes (the loop off),
+    back to back (the loop stays alive) and 5 ms apart (each request finds the
+    loop's kernel gone and starts a launch).  Every result checked exactly
+    against Python integers."""
+    from pi_amd import gnt
+    reps, warm = args.requests, 10
+    rows = []
+    for n in (64, 512, 1024):
+        a, b = operand(11, n), operand(12, n)
+        want = _to_int(a) * _to_int(b)
+        for dev in (True, False):
+            for gap in (0.0, 0.005):
+                ctx = make_ctx(args, serve_device_limbs=0 if dev else gnt.SERVE_DEVICE_OFF)
+                srv = gnt.Server(ctx)
+                lat, rt, ok = [], [], True
+                for i in range(reps + warm):
+                    t0 = time.perf_counter()
+                    buf, us = srv.wait(srv.submit(a, b))
+                    t1 = time.perf_counter()
+                    if i == 0:
+                        ok = _to_int(buf.download()) == want
+                    buf.release()
+                    if i >= warm:
+                        lat.append(us)
+                        rt.append((t1 - t0) * 1e6)
+                    if gap:
+                        time.sleep(gap)
+                st = srv.stats()
+                srv.stop()
+                ctx.close()
+                rows.append({"n": n, "path": "device loop" if dev else "lanes",
+                             "arrivals": "back to back" if not gap else "5 ms apart",
+                             "submit_to_done_p50_us": round(float(np.percentile(lat, 50)), 1),
+                             "submit_to_done_p99_us": round(float(np.percentile(lat, 99)), 1),
+                             "python_round_trip_p50_us": round(float(np.percentile(rt, 50)), 1),
+                             "device_loop_launches": st["device_loop_launches"], "exact": ok})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    best = min(r["submit_to_done_p50_us"] for r in rows if r["path"] == "device loop" and r["n"] == 1024)
+    line = {"metric": "small served product latency, device-resident loop vs lanes (us, submit to done)",
+            "value": best, "unit": "us", "n_gpus": 1, "steps": reps, "warmup": warm, "ms_per_step": best / 1e3,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+            "data": "synthetic (SplitMix64 limbs, seeds 11/12)",
+            "config": {"workload": "closed loop, one n x n multiply at a time, n = 64, 512, 1024",
+                       "value_is": "p50 for n = 1024 through the device loop", "rows": rows}}
+    print(json.dumps(line))
+
+
+def _to_int(v) -> int:
+    return int.from_bytes(np.ascontiguousarray(v, dtype="<u8").tobytes(), "little")
 
 
 def bench_chudnovsky(args):
