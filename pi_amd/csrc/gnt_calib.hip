@@ -23,6 +23,13 @@
 //                             against the same functions compiled for the
 //                             host, on edge values and random ones; exits 1
 //                             on any difference
+//   gnt_calib lds             one ds_read_b64 and one ds_write_b64 kernel per
+//                             LDS access pattern (lane strides of 1..32
+//                             words, and the down-column and last column-DFT
+//                             walks of the 1024 x 4 tiles in the XOR-swizzled
+//                             and the padded layout); run under rocprofv3
+//                             --pmc SQ_LDS_BANK_CONFLICT SQ_INSTS_LDS to read
+//                             the bank-conflict cycles of each pattern
 // Prints one JSON line per measurement.
 #include <hip/hip_runtime.h>
 
@@ -32,6 +39,7 @@
 #include <vector>
 
 #include "gl_field.hpp"
+#include "gnt_pass.hpp"
 
 using gl::u64;
 
@@ -273,6 +281,74 @@ static void launch_isa(void *p) {
     hipLaunchKernelGGL(a->k, dim3(ISA_BLOCKS), dim3(256), 0, 0, a->out, 12345u);
 }
 
+// ---- LDS access patterns ----
+// Word (8-byte) address of lane l's access in pattern P: strides of 2^P
+// words for P < 6; then the 1024 x 4 tile's down-column walk (64 rows of one
+// column) and last column-DFT sub-step walk (16 rows apart, 4 columns per
+// row), XOR-swizzled (the contiguous tiles' layout) and padded (the fused
+// multiply's half-width tiles).
+constexpr int LDS_PATTERNS = 10, LDS_ITERS = 4096, LDS_BLOCKS = 1024;
+using TileXor = gnt::Geom<10, 2, false, true, false>;
+using TilePad = gnt::Geom<10, 2, false, true, true>;
+static_assert(TilePad::padded && !TileXor::padded, "the two layouts");
+template <int P>
+__device__ __forceinline__ int lds_pattern(int l) {
+    if constexpr (P < 6) return l << P;
+    else if constexpr (P == 6) return gnt::tix(TileXor{}, l, 0);
+    else if constexpr (P == 7) return gnt::tix(TilePad{}, l, 0);
+    else if constexpr (P == 8) return gnt::tix(TileXor{}, 16 * (l >> 2) + 1, l & 3);
+    else return gnt::tix(TilePad{}, 16 * (l >> 2) + 1, l & 3);
+}
+static const char *lds_pattern_name[LDS_PATTERNS] = {
+    "stride_1", "stride_2", "stride_4", "stride_8", "stride_16", "stride_32",
+    "xor_down_column", "padded_down_column", "xor_last_substep", "padded_last_substep"};
+
+// Every lane reads (or writes) its pattern address LDS_ITERS times; the
+// address is masked into the 8192-word array and moved by it * zero, a zero
+// the compiler cannot see, so every iteration issues its own access.
+template <int P, bool WRITE>
+__global__ __launch_bounds__(256) void k_lds(u64 *out, int seed) {
+    __shared__ u64 lds[8192];
+    for (int i = threadIdx.x; i < 8192; i += 256) lds[i] = (u64)i * 0x9E3779B97F4A7C15ull + (u64)seed;
+    __syncthreads();
+    const int a0 = lds_pattern<P>(threadIdx.x & 63);
+    const int zero = seed >> 20;
+    u64 acc = 0;
+    for (int it = 0; it < LDS_ITERS; it++) {
+        const int a = (a0 + it * zero * 64) & 8191;
+        if (WRITE) lds[a] = acc + (u64)it;
+        else acc += lds[a];
+    }
+    __syncthreads();
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = acc + lds[threadIdx.x];
+}
+
+template <int P>
+static void lds_run(u64 *out) {
+    if constexpr (P < LDS_PATTERNS) {
+        for (int w = 0; w < 2; w++) {
+            hipEvent_t e0, e1;
+            CK(hipEventCreate(&e0));
+            CK(hipEventCreate(&e1));
+            CK(hipEventRecord(e0, 0));
+            if (w) hipLaunchKernelGGL((k_lds<P, true>), dim3(LDS_BLOCKS), dim3(256), 0, 0, out, 1);
+            else hipLaunchKernelGGL((k_lds<P, false>), dim3(LDS_BLOCKS), dim3(256), 0, 0, out, 1);
+            CK(hipGetLastError());
+            CK(hipEventRecord(e1, 0));
+            CK(hipEventSynchronize(e1));
+            float ms = 0;
+            CK(hipEventElapsedTime(&ms, e0, e1));
+            printf("{\"calib\": \"lds\", \"pattern\": \"%s\", \"access\": \"%s\", "
+                   "\"wave_accesses\": %llu, \"ms\": %.4f}\n",
+                   lds_pattern_name[P], w ? "ds_write_b64" : "ds_read_b64",
+                   (unsigned long long)LDS_BLOCKS * 4 * LDS_ITERS, ms);
+            CK(hipEventDestroy(e0));
+            CK(hipEventDestroy(e1));
+        }
+        lds_run<P + 1>(out);
+    }
+}
+
 int main(int argc, char **argv) {
     const char *mode = argc > 1 ? argv[1] : "all";
     if (!strcmp(mode, "field")) {
@@ -459,6 +535,12 @@ int main(int argc, char **argv) {
             printf("{\"calib\": \"%s\", \"Gops_per_s\": %.1f, \"ms\": %.3f}\n", names[k], ops / (ms * 1e-3) / 1e9, ms);
         }
         CK(hipFree(v.out));
+    }
+    if (!strcmp(mode, "lds")) {
+        u64 *out;
+        CK(hipMalloc(&out, (size_t)LDS_BLOCKS * 256 * sizeof(u64)));
+        lds_run<0>(out);
+        CK(hipFree(out));
     }
     if (!strcmp(mode, "isa") || !strcmp(mode, "all")) {
         IsaArg a;

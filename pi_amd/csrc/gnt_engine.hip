@@ -419,8 +419,9 @@ static int launch_tgroup(const Group &g, const TItem *dev, const Tw &tw, hipStre
         switch (g.shape) {
 #define TP_CASE(LR, LC, S)                                                                                        \
     case shape_key(LR, LC, S):                                                                                    \
-        hipLaunchKernelGGL((k_tpass<Geom<LR, LC, S>, KIND>), grid, dim3(TBLOCK), lds, st, items, n, g.tiles, base, \
-                           tw);                                                                                   \
+        hipLaunchKernelGGL((k_tpass<Geom<LR, LC, S>, KIND>), grid, dim3(TBLOCK),                                  \
+                           (KIND == K_FUSED_AB ? fused_ab_lds_bytes<Geom<LR, LC, S>>() : lds), st, items, n, g.tiles,   \
+                           base, tw);                                                                             \
         break;
             GNT_FOR_EACH_SHAPE(TP_CASE)
 #undef TP_CASE

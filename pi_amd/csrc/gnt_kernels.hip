@@ -252,7 +252,7 @@ int launch_last_fused_ab(const Plan &pl, u64 *xa, const u64 *xb, const Tw &tw, c
     case shape_key(LR, LC, S):                                                                                     \
         if (!HalfGeom<Geom<LR, LC, S>>::ok) return -1;                                                            \
         hipLaunchKernelGGL((k_last_fused_ab<Geom<LR, LC, S>>), dim3(2 * pass_grid(d, pl.log_n).x), dim3(TBLOCK),  \
-                           pass_lds(d), st, xa, xb, d, pl.log_n, tw, stw);                                              \
+                           (fused_ab_lds_bytes<Geom<LR, LC, S>>()), st, xa, xb, d, pl.log_n, tw, stw);                   \
         break;
         GNT_FOR_EACH_SHAPE(FAB_CASE)
 #undef FAB_CASE

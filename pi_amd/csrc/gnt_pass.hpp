@@ -90,14 +90,21 @@ constexpr int bitrev_c(int x, int bits) { return bits == 0 ? 0 : ((x & 1) << (bi
 
 // Tile shape, fixed at compile time: R = 2^LOGR rows, 2^LOGC columns;
 // STRIDED: columns are adjacent in memory (a pass with column stride > 1);
-// otherwise the pass is the contiguous last one (rows adjacent).
-template <int LOGR_, int LOGC_, bool STRIDED_, bool L16_ = false>
+// otherwise the pass is the contiguous last one (rows adjacent).  PAD: a
+// contiguous tile of 4 columns (the fused multiply's half-width tiles) whose
+// rows are padded in LDS (tix); the tile then takes tile_words words of LDS.
+template <int LOGR_, int LOGC_, bool STRIDED_, bool L16_ = false, bool PAD_ = false>
 struct Geom {
     using S = Sub<LOGR_, L16_>;  // sub-steps of the column DFT
     static constexpr int LOGR = LOGR_;
     static constexpr int log_c = LOGC_;
     static constexpr bool strided = STRIDED_;
     static constexpr int tile_n = 1 << (LOGR_ + LOGC_);
+    static constexpr bool padded = PAD_ && !STRIDED_ && LOGC_ == 2;
+    // Words of padding before row n (tix): 2 after every 8 rows and 32 more
+    // after every 128 rows.
+    static constexpr int row_pad(int n) { return padded ? 2 * (n >> 3) + 32 * (n >> 7) : 0; }
+    static constexpr int tile_words = tile_n + row_pad((1 << LOGR_) - 1);
     int log_m, log_s;  // pass block size, column stride (logs)
     size_t cg0;        // first column of the tile
 };
@@ -140,17 +147,40 @@ GL_HD G make_geom(const PassDesc &d, size_t tile_index) {
 // pass also walks down columns (its rows are adjacent in memory), so its
 // columns are XOR-swizzled by the row.  With rows of 32 columns or more both
 // walks then hit distinct banks.  With 8 columns or fewer a row covers only
-// part of the 64 banks and a column permutation cannot separate rows that
-// share them: the down-column walks and the last column-DFT sub-steps
-// conflict.  A layout that also permutes the low row bits by higher ones
-// removes every conflict, but its addresses are no longer one base plus
-// constants, so the compiler loses the paired LDS accesses and adds address
-// arithmetic.  On a 2^24-limb product that cost more than the conflicts did,
-// so the plain swizzle stays.
+// part of the banks and a column permutation cannot separate rows that share
+// them: the down-column walks and the last column-DFT sub-steps conflict.
+//
+// gfx950's LDS, measured with SQ_LDS_BANK_CONFLICT on single-pattern
+// kernels: a ds_read_b64 is served 32 lanes at a time from 32 banks of 8
+// bytes, a ds_write_b64 16 lanes at a time from 16 banks of 8 bytes (a wave's
+// lanes at the same bank and different addresses serialise).  Rows of 4 words
+// (the fused multiply's half-width tiles) are therefore padded instead
+// (Geom::row_pad): 2 unused words after every 8 rows put the next 8 rows 2
+// banks over, so a down-column walk has at most 2 lanes per bank and every
+// column-DFT walk of the 1024-row tile is conflict-free; 32 more words after
+// every 128 rows (a multiple of the bank count, so no bank changes) make rows
+// 128 apart 576 = 9 x 64 words apart, which the paired ds_read2st64_b64 /
+// ds_write2st64_b64 accesses of the first column sub-step need.  A layout
+// that also permutes the low row bits by higher ones removes every conflict
+// but its addresses are no longer one base plus constants: the compiler
+// drops the paired LDS accesses and adds address arithmetic, which on a
+// 2^24-limb product cost more than the conflicts did.
 template <class G>
 GL_HD int tix(const G &, int n, int c) {
     if constexpr (G::strided) return (n << G::log_c) + c;
+    else if constexpr (G::padded) return (n << G::log_c) + c + G::row_pad(n);
     else return (n << G::log_c) + (c ^ (n & ((1 << G::log_c) - 1)));
+}
+
+// Offset of row base + k from row base in a padded tile, for the rows of a
+// column-DFT task: base = (blk << LM) + jj with jj < 2^LT and k = u << LT (u <
+// 2^(LM - LT)), so base and k have no bits in common and, for every shift s,
+// (base + k) >> s = (base >> s) + (k >> s) (when LM <= s both sides are in
+// one aligned 2^s block).  The task then addresses its rows as one base plus
+// compile-time constants.
+template <class G>
+GL_HD constexpr int prow_off(int k) {
+    return (k << G::log_c) + G::row_pad(k);
 }
 
 // Global index of tile element (row n, tile column c).  Transforms hold at
@@ -247,7 +277,13 @@ GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
     task_coords<G, I>(L, c, jj, blk);
     const int base = (blk << LM) + jj;
     u64 v[R];
-    if constexpr (UPPER_ZERO) {
+    if constexpr (G::padded) {
+        static_assert(!UPPER_ZERO, "padded tiles belong to contiguous passes, which never pack");
+        const u64 *p = tile + tix(g, base, c);
+#pragma unroll
+        for (int u = 0; u < R; u++) v[u] = p[prow_off<G>(u << LT)];
+        gl::dft_dif<R>(v);
+    } else if constexpr (UPPER_ZERO) {
 #pragma unroll
         for (int u = 0; u < R / 2; u++) v[u] = tile[tix(g, base + (u << LT), c)];
         gl::dft_dif_upper_zero<R>(v);
@@ -263,8 +299,14 @@ GL_HD void fwd_task(u64 *tile, const G &g, int L, const Tw &tw) {
 #pragma unroll
         for (int q = 1; q < R; q++) v[q] = gl::mul(v[q], w[(u32)jj * gl::bitrev((u32)q, A)]);
     }
+    if constexpr (G::padded) {
+        u64 *p = tile + tix(g, base, c);
 #pragma unroll
-    for (int q = 0; q < R; q++) tile[tix(g, base + (q << LT), c)] = v[q];
+        for (int q = 0; q < R; q++) p[prow_off<G>(q << LT)] = v[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; q++) tile[tix(g, base + (q << LT), c)] = v[q];
+    }
 }
 
 // Inverse of fwd_task<LOGR, I>: undo the twiddle, then the inverse r-point
@@ -280,9 +322,10 @@ GL_HD void inv_task(u64 *tile, const G &g, int L, const Tw &tw) {
     int c, jj, blk;
     task_coords<G, I>(L, c, jj, blk);
     const int base = (blk << LM) + jj;
+    u64 *p = tile + tix(g, base, c);
     u64 v[R];
 #pragma unroll
-    for (int q = 0; q < R; q++) v[q] = tile[tix(g, base + (q << LT), c)];
+    for (int q = 0; q < R; q++) v[q] = G::padded ? p[prow_off<G>(q << LT)] : tile[tix(g, base + (q << LT), c)];
     if constexpr (ShiftTw<G>::on && I == ShiftTw<G>::I) {
         shift_twiddles_by<192 / (1 << LM), A, (1 << LT), true>(v, wave_uniform_int(jj));
     } else if constexpr (!LAST) {
@@ -295,7 +338,10 @@ GL_HD void inv_task(u64 *tile, const G &g, int L, const Tw &tw) {
     }
     gl::dft_dit_inv<R>(v);
 #pragma unroll
-    for (int u = 0; u < R; u++) tile[tix(g, base + (u << LT), c)] = v[u];
+    for (int u = 0; u < R; u++) {
+        if constexpr (G::padded) p[prow_off<G>(u << LT)] = v[u];
+        else tile[tix(g, base + (u << LT), c)] = v[u];
+    }
 }
 
 // ---- tile-level routines ----
@@ -579,15 +625,16 @@ GL_HD void body_fused(u64 *tile, const G &g, const u64 *xb, const u64 *limbs, si
 // pointwise product, 1/N and the first inverse pass, result to xa: one tile
 // round trip instead of a separate pass over A.  G is a half-width tile
 // (2^(TILE_LOG-1) elements) of the plan's contiguous last pass, so A's and
-// B's tiles share the LDS of one full tile: tile[0, n) holds A's, tile[n, 2n)
-// B's.  Plans of two or more passes (no pass twiddle on the last pass).
+// B's tiles share the LDS of about one full tile: tile[0, w) holds A's,
+// tile[w, 2w) B's, w = G::tile_words.  Plans of two or more passes (no pass
+// twiddle on the last pass).
 template <class X, class G, int I>
 GL_HD void tile_fwd_pair(u64 *tile, const G &g, const Tw &tw) {
     if constexpr (I < G::S::count) {
         const int ntask = G::tile_n >> G::S::log(I);
         GNT_FOR(X, L, 2 * ntask) {
             const bool b = L >= ntask;
-            fwd_task<G, I>(tile + (b ? G::tile_n : 0), g, b ? L - ntask : L, tw);
+            fwd_task<G, I>(tile + (b ? G::tile_words : 0), g, b ? L - ntask : L, tw);
         }
         X::sync();
         tile_fwd_pair<X, G, I + 1>(tile, g, tw);
@@ -597,7 +644,7 @@ GL_HD void tile_fwd_pair(u64 *tile, const G &g, const Tw &tw) {
 template <class X, class G>
 GL_HD void body_fused_ab(u64 *tile, const G &g, u64 *xa, const u64 *xb, int log_n, const Tw &tw,
                          const StoreTw &st) {
-    u64 *ta = tile, *tb = tile + G::tile_n;
+    u64 *ta = tile, *tb = tile + G::tile_words;
     GNT_FOR(X, idx, 2 * G::tile_n) {
         const bool b = idx >= G::tile_n;
         const int k = b ? idx - G::tile_n : idx;
@@ -627,8 +674,14 @@ struct HalfGeom {
     static constexpr bool ok = !FullG::strided && FullG::log_c >= 1 && FullG::LOGR + FullG::log_c == TILE_LOG;
     // the fused multiply kernel holds few registers besides a task's values:
     // a radix-16 last sub-step fits it without spilling
-    using type = Geom<FullG::LOGR, ok ? FullG::log_c - 1 : FullG::log_c, false, true>;
+    using type = Geom<FullG::LOGR, ok ? FullG::log_c - 1 : FullG::log_c, false, true, true>;
 };
+
+// LDS bytes of the fused A+B kernel for the last-pass shape FullG.
+template <class FullG>
+constexpr size_t fused_ab_lds_bytes() {
+    return 2 * (size_t)HalfGeom<FullG>::type::tile_words * sizeof(u64);
+}
 
 // Carry-lookahead pairs: bit 0 = generate, bit 1 = propagate.  `lo` covers
 // the lower limbs, `hi` the limbs just above; identity is 2 (g=0, p=1).

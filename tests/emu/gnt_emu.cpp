@@ -98,7 +98,7 @@ template <class G>
 void k_last_fused_ab(const PassDesc &d, int log_n, u64 *xa, const u64 *xb, const StoreTw &st) {
     using H = typename HalfGeom<G>::type;
     if constexpr (HalfGeom<G>::ok) {
-        std::vector<u64> tile(2 * H::tile_n);
+        std::vector<u64> tile(2 * H::tile_words);
         const size_t grid = ((size_t)1 << log_n) / H::tile_n;
         for (size_t t = 0; t < grid; t++)
             body_fused_ab<HX, H>(tile.data(), make_geom<H>(d, t), xa, xb, log_n, tw(), st);
