@@ -70,11 +70,36 @@ GL_HD u64 sub_borrow(u64 a, u64 b, u32 *borrow) {
 
 // (a - b) mod p for a, b in [0, p).  On borrow add p, i.e. subtract EPS
 // (cannot underflow again: a - b + 2^64 >= 2^64 - p + 1 > EPS).
+#ifdef GL_DEVICE_ASM
+// d - B * EPS for the borrow B of d = a - b, as (d_hi - B) 2^32 + (d_lo + B):
+// the low word takes B as its carry-in, and the high word loses B unless
+// that add carried out of the low word (then d_lo + 1 wrapped to 0 and the
+// 2^32 it carried cancels the -2^32), i.e. it loses B & ~carry, formed on
+// the scalar unit.  Four vector instructions instead of the five of a
+// selected EPS subtracted with borrow.
+__device__ __forceinline__ u64 sub(u64 a, u64 b) {
+    u32 lo, hi;
+    u64 b1, bw, c, m;
+    asm("v_sub_co_u32_e64 %0, %2, %6, %8\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %1, %3, %7, %9, %2\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %0, %4, %0, 0, %3\n\t"
+        "s_andn2_b64 %5, %3, %4\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %1, %2, %1, 0, %5"
+        : "=&v"(lo), "=v"(hi), "=&s"(b1), "=&s"(bw), "=&s"(c), "=&s"(m)
+        : "v"((u32)a), "v"((u32)(a >> 32)), "v"((u32)b), "v"((u32)(b >> 32))
+        : "scc");
+    return ((u64)hi << 32) | lo;
+}
+#else
 GL_HD u64 sub(u64 a, u64 b) {
     u32 bw, bw2;
     const u64 d = sub_borrow(a, b, &bw);
     return sub_borrow(d, bw ? EPS : 0u, &bw2);
 }
+#endif
 
 GL_HD u64 neg(u64 a) { return sub(0, a); }
 
@@ -103,20 +128,22 @@ __device__ __forceinline__ u64 add_hl_eps(u64 t, u32 hl) {
 }
 
 // lo - hh as a value in [0, 2^64) congruent mod p: on borrow subtract EPS
-// (lo - hh + 2^64 - EPS = lo - hh + p, in [p - hh, p)).
+// (lo - hh + 2^64 - EPS = lo - hh + p, in [p - hh, p)), the EPS taken off as
+// in sub: the borrow carried into the low word, B & ~carry out of the high.
 __device__ __forceinline__ u64 sub_hh(u64 lo, u32 hh) {
-    u32 r0, r1, e;
-    u64 m, m2;
-    asm("v_sub_co_u32_e64 %0, %3, %5, %7\n\t"
+    u32 r0, r1;
+    u64 m, c, m2;
+    asm("v_sub_co_u32_e64 %0, %2, %5, %7\n\t"
         "s_nop 1\n\t"
-        "v_subbrev_co_u32_e64 %1, %3, 0, %6, %3\n\t"
+        "v_subbrev_co_u32_e64 %1, %2, 0, %6, %2\n\t"
         "s_nop 1\n\t"
-        "v_cndmask_b32_e64 %2, 0, -1, %3\n\t"
-        "v_sub_co_u32_e64 %0, %4, %0, %2\n\t"
+        "v_addc_co_u32_e64 %0, %3, %0, 0, %2\n\t"
+        "s_andn2_b64 %4, %2, %3\n\t"
         "s_nop 1\n\t"
-        "v_subbrev_co_u32_e64 %1, %4, 0, %1, %4"
-        : "=&v"(r0), "=v"(r1), "=v"(e), "=&s"(m), "=&s"(m2)
-        : "v"((u32)lo), "v"((u32)(lo >> 32)), "v"(hh));
+        "v_subbrev_co_u32_e64 %1, %3, 0, %1, %4"
+        : "=&v"(r0), "=v"(r1), "=&s"(m), "=&s"(c), "=&s"(m2)
+        : "v"((u32)lo), "v"((u32)(lo >> 32)), "v"(hh)
+        : "scc");
     return ((u64)r1 << 32) | r0;
 }
 #endif
@@ -176,10 +203,16 @@ GL_HD u64 mul_2_72(u64 x) {
     const unsigned al = __builtin_addc(yl, yh, 0u, &ca);
     const u64 B = ((((u64)xl + xh) << 8)) + yh;
     const u64 V0 = ((u64)al << 32) | (ca ? (u32)EPS : 0u);
+#ifdef GL_DEVICE_ASM
+    const u64 v = sub(V0, B);               // V0 - B, or on borrow V0 - B + p (V0 - B > -p)
+    const u32 g = v >= P;
+    return v + (u64)g * EPS;                // v >= p wraps to v - p
+#else
     u32 bw, bw2;
     u64 v = sub_borrow(V0, B, &bw);         // on borrow add p: subtract EPS (V0 - B > -p)
     v = sub_borrow(v, bw ? EPS : 0u, &bw2);
     return v >= P ? v + EPS : v;            // wraps to v - p
+#endif
 }
 
 // 2^e mod p for 0 <= e < 192, as a plain constant.
