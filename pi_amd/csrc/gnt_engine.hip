@@ -367,7 +367,7 @@ void finalize_iteration(Iteration &it) {
             it.citems[q].tile0 = tiles;
             tiles += step == 1 ? 1 : it.citems[q].ntiles;
         }
-        it.groups.push_back(Group{10 + (int)step, 0, i, j - i, tiles, 0});
+        it.groups.push_back(Group{K_CARRY + (int)step, 0, i, j - i, tiles, 0});
         i = j;
     }
 }
@@ -450,8 +450,8 @@ int launch_iteration(const Iteration &it, const TItem *dev_t, const CItem *dev_c
             const CItem *items = dev_c + g.first;
             const int n = (int)g.count;
             dim3 grid((unsigned)std::min<uint64_t>(g.tiles, 65536));
-            if (g.kind == 10) hipLaunchKernelGGL(k_ctiles, grid, dim3(CARRY_BLOCK), 0, st, items, n, g.tiles);
-            else if (g.kind == 11) hipLaunchKernelGGL(k_cscan, dim3((unsigned)n), dim3(SCAN_BLOCK), 0, st, items);
+            if (g.kind == K_CARRY) hipLaunchKernelGGL(k_ctiles, grid, dim3(CARRY_BLOCK), 0, st, items, n, g.tiles);
+            else if (g.kind == K_CARRY + 1) hipLaunchKernelGGL(k_cscan, dim3((unsigned)n), dim3(SCAN_BLOCK), 0, st, items);
             else {
                 hipLaunchKernelGGL(k_cripple, grid, dim3(CARRY_RIPPLE_BLOCK), 0, st, items, n, g.tiles);
                 if (it.want_lens)

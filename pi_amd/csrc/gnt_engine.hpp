@@ -37,6 +37,9 @@ enum ItemKind : int {
     K_INV_TSTORE = 6,
     K_INV_SUMS_PLAIN = 7
 };
+// Group kinds of the carry steps: K_CARRY + step (0: tiles, 1: scan, 2: ripple).
+constexpr int K_CARRY = 16;
+static_assert(K_INV_SUMS_PLAIN < K_CARRY, "item kinds and carry-step kinds must not overlap");
 
 // One tile-pass over one array (device descriptor).
 struct TItem {
@@ -112,7 +115,7 @@ bool prod_init(Prod &p);
 
 // Launch groups of one iteration.
 struct Group {
-    int kind;   // an ItemKind, or 10+k for carry step k
+    int kind;   // an ItemKind, or K_CARRY + k for carry step k
     int shape;  // shape_key of the group's tiles
     size_t first, count;  // range in the iteration's item table
     uint64_t tiles;       // total tiles of the launch

@@ -246,11 +246,14 @@ def test_table_and_stepped_twiddles_give_the_same_products(emu, orc, max_log):
             a, b = orc.splitmix64(la, la), orc.splitmix64(lb + 7, lb)
             assert np.array_equal(emu_mul(E, a, b), orc.mul_mag_ntt(a, b)), (la, lb)
             assert np.array_equal(emu_mul(E, a), orc.mul_mag_ntt(a, a)), la
-        n = 1 << 17
-        x = np.random.default_rng(3).integers(0, 2**63, size=n, dtype=np.uint64)
-        y = x.copy()
-        E.emu_forward(y.ctypes.data_as(PU), 17)
-        E.emu_inverse(y.ctypes.data_as(PU), 17)
-        assert all(int(y[i]) == int(x[i]) for i in range(0, n, 991))
+        # 2^17: two passes; 2^21: three passes, where with max_log 14 pass 0
+        # steps its twiddles and pass 1 takes them from its table
+        for log_n in (17, 21) if max_log == 14 else (17,):
+            n = 1 << log_n
+            x = np.random.default_rng(3).integers(0, 2**63, size=n, dtype=np.uint64)
+            y = x.copy()
+            E.emu_forward(y.ctypes.data_as(PU), log_n)
+            E.emu_inverse(y.ctypes.data_as(PU), log_n)
+            assert all(int(y[i]) == int(x[i]) for i in range(0, n, 991)), log_n
     finally:
         E.emu_set_table_max_log(22)
