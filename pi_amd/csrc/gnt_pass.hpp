@@ -346,10 +346,29 @@ GL_HD void inv_task(u64 *tile, const G &g, int L, const Tw &tw) {
 
 // ---- tile-level routines ----
 
+// Two adjacent tile columns of one row, as one 16-byte access (a strided
+// tile's columns are adjacent in memory, and an even column's global index
+// is even, so the pair is 16-byte aligned).
+struct alignas(16) U64Pair {
+    u64 v[2];
+};
+
 // Load a tile from x, or (limbs != nullptr) pack it from the operand's limbs.
 template <class X, class G>
 GL_HD void tile_load(u64 *tile, const G &g, const u64 *__restrict__ x, const u64 *__restrict__ limbs,
                      size_t n_limbs) {
+    if constexpr (G::strided && G::log_c >= 1) {
+        if (!limbs) {
+            GNT_FOR(X, k, G::tile_n / 2) {
+                const int n = (2 * k) >> G::log_c, c = (2 * k) & ((1 << G::log_c) - 1);
+                const U64Pair t = *reinterpret_cast<const U64Pair *>(x + tile_addr(g.cg0, n, c, g.log_m, g.log_s));
+                tile[tix(g, n, c)] = t.v[0];
+                tile[tix(g, n, c + 1)] = t.v[1];
+            }
+            X::sync();
+            return;
+        }
+    }
     GNT_FOR(X, idx, G::tile_n) {
         int n, c;
         tile_rc<G>(idx, n, c);
@@ -361,6 +380,16 @@ GL_HD void tile_load(u64 *tile, const G &g, const u64 *__restrict__ x, const u64
 
 template <class X, class G>
 GL_HD void tile_store(const u64 *tile, const G &g, u64 *__restrict__ x) {
+    if constexpr (G::strided && G::log_c >= 1) {
+        GNT_FOR(X, k, G::tile_n / 2) {
+            const int n = (2 * k) >> G::log_c, c = (2 * k) & ((1 << G::log_c) - 1);
+            U64Pair t;
+            t.v[0] = tile[tix(g, n, c)];
+            t.v[1] = tile[tix(g, n, c + 1)];
+            *reinterpret_cast<U64Pair *>(x + tile_addr(g.cg0, n, c, g.log_m, g.log_s)) = t;
+        }
+        return;
+    }
     GNT_FOR(X, idx, G::tile_n) {
         int n, c;
         tile_rc<G>(idx, n, c);
@@ -375,6 +404,18 @@ GL_HD void tile_store(const u64 *tile, const G &g, u64 *__restrict__ x) {
 template <class X, class G>
 GL_HD void tile_store_tw(const u64 *tile, const G &g, u64 *__restrict__ x, const u64 *__restrict__ T, int log_t) {
     const u32 mt = (1u << log_t) - 1;
+    if constexpr (G::strided && G::log_c >= 1) {
+        GNT_FOR(X, k, G::tile_n / 2) {
+            const int n = (2 * k) >> G::log_c, c = (2 * k) & ((1 << G::log_c) - 1);
+            const u32 a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
+            const U64Pair w = *reinterpret_cast<const U64Pair *>(T + (a & mt));
+            U64Pair t;
+            t.v[0] = gl::mul(tile[tix(g, n, c)], w.v[0]);
+            t.v[1] = gl::mul(tile[tix(g, n, c + 1)], w.v[1]);
+            *reinterpret_cast<U64Pair *>(x + a) = t;
+        }
+        return;
+    }
     GNT_FOR(X, idx, G::tile_n) {
         int n, c;
         tile_rc<G>(idx, n, c);
