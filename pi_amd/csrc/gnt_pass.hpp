@@ -346,9 +346,11 @@ GL_HD void inv_task(u64 *tile, const G &g, int L, const Tw &tw) {
 
 // ---- tile-level routines ----
 
-// Two adjacent tile columns of one row, as one 16-byte access (a strided
-// tile's columns are adjacent in memory, and an even column's global index
-// is even, so the pair is 16-byte aligned).
+// Two adjacent tile columns of one row, as one 16-byte access in HBM and in
+// LDS (a strided tile is plain row-major, and an even column's global index
+// and tile index are even, so the pair is 16-byte aligned in both).  With
+// two 8-byte LDS accesses per lane instead, consecutive lanes are 16 bytes
+// apart and every access conflicts 2-way.
 struct alignas(16) U64Pair {
     u64 v[2];
 };
@@ -361,9 +363,8 @@ GL_HD void tile_load(u64 *tile, const G &g, const u64 *__restrict__ x, const u64
         if (!limbs) {
             GNT_FOR(X, k, G::tile_n / 2) {
                 const int n = (2 * k) >> G::log_c, c = (2 * k) & ((1 << G::log_c) - 1);
-                const U64Pair t = *reinterpret_cast<const U64Pair *>(x + tile_addr(g.cg0, n, c, g.log_m, g.log_s));
-                tile[tix(g, n, c)] = t.v[0];
-                tile[tix(g, n, c + 1)] = t.v[1];
+                *reinterpret_cast<U64Pair *>(tile + tix(g, n, c)) =
+                    *reinterpret_cast<const U64Pair *>(x + tile_addr(g.cg0, n, c, g.log_m, g.log_s));
             }
             X::sync();
             return;
@@ -383,10 +384,8 @@ GL_HD void tile_store(const u64 *tile, const G &g, u64 *__restrict__ x) {
     if constexpr (G::strided && G::log_c >= 1) {
         GNT_FOR(X, k, G::tile_n / 2) {
             const int n = (2 * k) >> G::log_c, c = (2 * k) & ((1 << G::log_c) - 1);
-            U64Pair t;
-            t.v[0] = tile[tix(g, n, c)];
-            t.v[1] = tile[tix(g, n, c + 1)];
-            *reinterpret_cast<U64Pair *>(x + tile_addr(g.cg0, n, c, g.log_m, g.log_s)) = t;
+            *reinterpret_cast<U64Pair *>(x + tile_addr(g.cg0, n, c, g.log_m, g.log_s)) =
+                *reinterpret_cast<const U64Pair *>(tile + tix(g, n, c));
         }
         return;
     }
@@ -409,9 +408,9 @@ GL_HD void tile_store_tw(const u64 *tile, const G &g, u64 *__restrict__ x, const
             const int n = (2 * k) >> G::log_c, c = (2 * k) & ((1 << G::log_c) - 1);
             const u32 a = tile_addr(g.cg0, n, c, g.log_m, g.log_s);
             const U64Pair w = *reinterpret_cast<const U64Pair *>(T + (a & mt));
-            U64Pair t;
-            t.v[0] = gl::mul(tile[tix(g, n, c)], w.v[0]);
-            t.v[1] = gl::mul(tile[tix(g, n, c + 1)], w.v[1]);
+            U64Pair t = *reinterpret_cast<const U64Pair *>(tile + tix(g, n, c));
+            t.v[0] = gl::mul(t.v[0], w.v[0]);
+            t.v[1] = gl::mul(t.v[1], w.v[1]);
             *reinterpret_cast<U64Pair *>(x + a) = t;
         }
         return;
