@@ -147,6 +147,46 @@ int ceil_log2(size_t x) {
 
 }  // namespace
 
+// Layout checks of the fused multiply kernel's half-width tiles (see
+// emu_fused_tile_layout_check).
+template <class G, int I = 0>
+long tile_tasks_check(const G &g) {
+    if constexpr (I >= G::S::count) {
+        return 0;
+    } else {
+        constexpr int A = G::S::log(I), LM = G::S::log_m(I), LT = LM - A;
+        long bad = 0;
+        for (int L = 0; L < (G::tile_n >> A); L++) {
+            int c, jj, blk;
+            task_coords<G, I>(L, c, jj, blk);
+            const int base = (blk << LM) + jj;
+            for (int u = 0; u < (1 << A); u++)
+                bad += tix(g, base + (u << LT), c) != tix(g, base, c) + prow_off<G>(u << LT);
+        }
+        return bad + tile_tasks_check<G, I + 1>(g);
+    }
+}
+
+template <class FullG>
+long half_tile_check() {
+    if constexpr (!HalfGeom<FullG>::ok) {
+        return 0;
+    } else {
+        using H = typename HalfGeom<FullG>::type;
+        const H g{};
+        std::vector<int> seen(H::tile_words, 0);
+        long bad = 0;
+        for (int n = 0; n < (1 << H::LOGR); n++)
+            for (int c = 0; c < (1 << H::log_c); c++) {
+                const int t = tix(g, n, c);
+                if (t < 0 || t >= H::tile_words || seen[t]++) bad++;
+            }
+        if (H::padded) bad += tile_tasks_check(g);
+        bad += 2 * 2 * H::tile_words * 8 > 160 * 1024;
+        return bad;
+    }
+}
+
 extern "C" {
 
 // Forward transform of x (length 2^log_n) in place, as the GPU computes it.
@@ -261,6 +301,19 @@ size_t emu_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
         if (out[j]) len = j + 1;
     }
     return len;
+}
+
+// Layout checks of every tile shape the fused multiply kernel halves (its
+// half-width tiles): tix maps the tile one-to-one into [0, tile_words); in a
+// padded tile every column-DFT task's rows sit at one base plus the
+// compile-time offsets prow_off; and the fused kernel's two tiles fit two per
+// CU in 160 KiB of LDS.  Returns the number of violations.
+long emu_fused_tile_layout_check() {
+    long bad = 0;
+#define F(LR, LC, S) bad += half_tile_check<Geom<LR, LC, S>>();
+    GNT_FOR_EACH_SHAPE(F)
+#undef F
+    return bad;
 }
 
 // Field helpers for direct checks of the device arithmetic.

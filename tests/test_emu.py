@@ -257,3 +257,13 @@ def test_table_and_stepped_twiddles_give_the_same_products(emu, orc, max_log):
             assert all(int(y[i]) == int(x[i]) for i in range(0, n, 991)), log_n
     finally:
         E.emu_set_table_max_log(22)
+
+
+def test_fused_tile_layout(emu):
+    """The fused multiply's half-width tiles: their LDS positions are one to
+    one within the tile's words (padded 4-column tiles leave 2 words after
+    every 8 rows and 32 after every 128), every column-DFT task of a padded
+    tile addresses its rows as one base plus compile-time offsets, and two
+    A+B tile pairs fit a CU's 160 KiB of LDS."""
+    emu.emu_fused_tile_layout_check.restype = ctypes.c_long
+    assert emu.emu_fused_tile_layout_check() == 0
