@@ -519,8 +519,16 @@ GL_HD void tile_pack(u64 *tile, const G &g, const u64 *__restrict__ limbs, size_
         const int n = k >> QL, q = (k & ((1 << QL) - 1)) << 2;
         const size_t l = tile_addr(g.cg0, n, q, g.log_m, g.log_s) >> 2;
         const u64 v = l < n_limbs ? limbs[l] : 0;
-#pragma unroll
-        for (int t = 0; t < 4; t++) tile[tix(g, n, q + t)] = (v >> (16 * t)) & 0xFFFFull;
+        // the limb's four coefficients are adjacent words of the row (q is a
+        // multiple of 4): two 16-byte LDS writes
+        U64Pair lo, hi;
+        lo.v[0] = v & 0xFFFFull;
+        lo.v[1] = (v >> 16) & 0xFFFFull;
+        hi.v[0] = (v >> 32) & 0xFFFFull;
+        hi.v[1] = v >> 48;
+        U64Pair *dst = reinterpret_cast<U64Pair *>(tile + tix(g, n, q));
+        dst[0] = lo;
+        dst[1] = hi;
     }
     X::sync();
 }
