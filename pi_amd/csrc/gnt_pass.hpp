@@ -585,10 +585,19 @@ GL_HD void tile_limb_sums(const u64 *tile, const G &g, size_t n_coeffs, size_t n
         u32 j = a0 >> 2;
         if (j < n_limbs) {
             u64 cc[4];
+            if constexpr (G::strided) {
+                // four adjacent words of the row: two 16-byte LDS reads
+                const U64Pair *src = reinterpret_cast<const U64Pair *>(tile + tix(g, n, c));
+                const U64Pair lo = src[0], hi = src[1];
+                cc[0] = lo.v[0];
+                cc[1] = lo.v[1];
+                cc[2] = hi.v[0];
+                cc[3] = hi.v[1];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                int t = G::strided ? tix(g, n, c + k) : tix(g, n + k, c);
-                cc[k] = (a0 + k < n_coeffs) ? tile[t] : 0;
+                for (int k = 0; k < 4; k++) cc[k] = (a0 + k < n_coeffs) ? cc[k] : 0;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) cc[k] = (a0 + k < n_coeffs) ? tile[tix(g, n + k, c)] : 0;
             }
             u64 lo = cc[0], hi = 0, t;
             t = cc[1] << 16; lo += t; hi += (cc[1] >> 48) + (lo < t);
