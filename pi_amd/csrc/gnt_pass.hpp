@@ -100,10 +100,15 @@ struct Geom {
     static constexpr int log_c = LOGC_;
     static constexpr bool strided = STRIDED_;
     static constexpr int tile_n = 1 << (LOGR_ + LOGC_);
-    static constexpr bool padded = PAD_ && !STRIDED_ && LOGC_ == 2;
-    // Words of padding before row n (tix): 2 after every 8 rows and 32 more
-    // after every 128 rows.
-    static constexpr int row_pad(int n) { return padded ? 2 * (n >> 3) + 32 * (n >> 7) : 0; }
+    static constexpr bool pad4 = PAD_ && !STRIDED_ && LOGC_ == 2;                 // 1024 x 4 (radix 2^10)
+    static constexpr bool pad8 = PAD_ && !STRIDED_ && LOGC_ == 3 && LOGR_ == 9;  // 512 x 8
+    static constexpr bool padded = pad4 || pad8;
+    // Words of padding before row n (tix): rows of 4 words, 2 after every 8
+    // rows and 32 more after every 128; rows of 8 words, 3 after every 4 rows
+    // and 16 more after every 64.
+    static constexpr int row_pad(int n) {
+        return pad4 ? 2 * (n >> 3) + 32 * (n >> 7) : pad8 ? 3 * (n >> 2) + 16 * (n >> 6) : 0;
+    }
     static constexpr int tile_words = tile_n + row_pad((1 << LOGR_) - 1);
     int log_m, log_s;  // pass block size, column stride (logs)
     size_t cg0;        // first column of the tile
@@ -153,14 +158,18 @@ GL_HD G make_geom(const PassDesc &d, size_t tile_index) {
 // gfx950's LDS, measured with SQ_LDS_BANK_CONFLICT on single-pattern
 // kernels: a ds_read_b64 is served 32 lanes at a time from 32 banks of 8
 // bytes, a ds_write_b64 16 lanes at a time from 16 banks of 8 bytes (a wave's
-// lanes at the same bank and different addresses serialise).  Rows of 4 words
-// (the fused multiply's half-width tiles) are therefore padded instead
-// (Geom::row_pad): 2 unused words after every 8 rows put the next 8 rows 2
-// banks over, so a down-column walk has at most 2 lanes per bank and every
-// column-DFT walk of the 1024-row tile is conflict-free; 32 more words after
-// every 128 rows (a multiple of the bank count, so no bank changes) make rows
-// 128 apart 576 = 9 x 64 words apart, which the paired ds_read2st64_b64 /
-// ds_write2st64_b64 accesses of the first column sub-step need.  A layout
+// lanes at the same bank and different addresses serialise).  The fused
+// multiply's half-width tiles of 4 and of 8 columns are therefore padded
+// instead (Geom::row_pad).  Rows of 4 words (1024 rows): 2 unused words after
+// every 8 rows put the next 8 rows 2 banks over, so a down-column walk has at
+// most 2 lanes per bank and every column-DFT walk is conflict-free; 32 more
+// words after every 128 rows (a multiple of the bank count, so no bank
+// changes) make rows 128 apart 576 = 9 x 64 words apart, which the paired
+// ds_read2st64_b64 / ds_write2st64_b64 accesses of the first column sub-step
+// need.  Rows of 8 words (512 rows): 3 words after every 4 rows make the
+// down-column reads conflict-free and halve the conflicts of the tile's walks
+// in a model of the banks above, and 16 more after every 64 rows keep rows 64
+// apart 576 words apart for the same pairing.  A layout
 // that also permutes the low row bits by higher ones removes every conflict
 // but its addresses are no longer one base plus constants: the compiler
 // drops the paired LDS accesses and adds address arithmetic, which on a
