@@ -106,6 +106,12 @@ __global__ GNT_TBOUNDS void k_last_fused_ab(u64 *xa, const u64 *__restrict__ xb,
 // receive one.
 // ---------------------------------------------------------------------------
 
+// Tile flags: bit 0 generate, bit 1 propagate (gp_combine form), bit 2 the
+// tile holds a nonzero limb.  The last tile stores its own trimmed length
+// (1 + its highest nonzero limb, 0 if none); k_carry_scan replaces it when a
+// lower tile is the highest flagged one, so no tile updates a shared word.
+constexpr unsigned char TILE_NONZERO = 4;
+
 __global__ __launch_bounds__(CARRY_BLOCK) void k_carry_tiles(const u64 *__restrict__ Ls, const u64 *__restrict__ Hs,
                                                              u64 *__restrict__ out, size_t n_limbs,
                                                              unsigned char *__restrict__ tile_gp,
@@ -114,22 +120,36 @@ __global__ __launch_bounds__(CARRY_BLOCK) void k_carry_tiles(const u64 *__restri
     u32 gp, hi;
     carry_tile(SumsSrc{Ls, Hs}, n_limbs, blockIdx.x, out, sh, gp, hi);
     if (threadIdx.x == 0) {
-        tile_gp[blockIdx.x] = (unsigned char)gp;
-        if (hi) atomicMax(out_len, hi);
+        tile_gp[blockIdx.x] = (unsigned char)(gp | (hi ? TILE_NONZERO : 0));
+        if (blockIdx.x == gridDim.x - 1) *out_len = hi;
     }
 }
 
-// One workgroup: exclusive carry-in of every tile.
+// One workgroup: exclusive carry-in of every tile, and the trimmed length
+// before the ripple when the highest flagged tile is not the last (the
+// product's top 2048 limbs are zero): 1 + the highest nonzero limb of that
+// tile, read back from `out`.  A product's carry never leaves its top limb,
+// so the ripple can only raise that length.
 constexpr int SCAN_BLOCK = 1024;
+static_assert(CARRY_TILE == 2 * SCAN_BLOCK, "the length search reads two limbs per thread");
 __global__ __launch_bounds__(SCAN_BLOCK) void k_carry_scan(const unsigned char *__restrict__ tile_gp,
-                                                           unsigned char *__restrict__ tile_cin, size_t ntiles) {
+                                                           unsigned char *__restrict__ tile_cin, size_t ntiles,
+                                                           const u64 *__restrict__ out, size_t n_limbs,
+                                                           u32 *__restrict__ out_len) {
     __shared__ u32 scan[SCAN_BLOCK];
+    __shared__ u32 top_tile, top_limb;
     const size_t per = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
     const size_t t0 = (size_t)threadIdx.x * per;
-    u32 agg = 2;
-    for (size_t t = t0; t < t0 + per && t < ntiles; t++) agg = gp_combine(agg, tile_gp[t]);
+    u32 agg = 2, top = 0;
+    for (size_t t = t0; t < t0 + per && t < ntiles; t++) {
+        const u32 f = tile_gp[t];
+        agg = gp_combine(agg, f);
+        if (f & TILE_NONZERO) top = (u32)t + 1;
+    }
     scan[threadIdx.x] = agg;
+    if (threadIdx.x == 0) top_tile = top_limb = 0;
     __syncthreads();
+    if (top) atomicMax(&top_tile, top);
     for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
         u32 v = scan[threadIdx.x];
         u32 w = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 2u;
@@ -141,7 +161,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_carry_scan(const unsigned char *
     for (size_t t = t0; t < t0 + per && t < ntiles; t++) {
         tile_cin[t] = (unsigned char)carry;
         u32 gp = tile_gp[t];
-        carry = (gp & 1) | ((gp >> 1) & carry);
+        carry = (gp & 1) | ((gp >> 1) & 1 & carry);
+    }
+    const u32 tt = top_tile;  // final: written before the scan loop's barriers
+    if (tt && tt < ntiles) {   // uniform
+        const size_t j = (size_t)(tt - 1) * CARRY_TILE + 2 * threadIdx.x;
+        const u64 v0 = j < n_limbs ? out[j] : 0, v1 = j + 1 < n_limbs ? out[j + 1] : 0;
+        const u32 h = v1 ? (u32)(j + 2) : v0 ? (u32)(j + 1) : 0;
+        if (h) atomicMax(&top_limb, h);
+        __syncthreads();
+        if (threadIdx.x == 0) *out_len = top_limb;
     }
 }
 
@@ -157,11 +186,6 @@ __global__ __launch_bounds__(CARRY_RIPPLE_BLOCK) void k_carry_ripple(u64 *__rest
 __global__ void k_pass_table(u64 *__restrict__ T, int log_m, int log_r, int inv, int scale_e, Tw tw) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ((size_t)1 << log_m)) T[i] = pass_table_entry(tw, (u32)i, log_m, log_r, inv != 0, scale_e);
-}
-
-__global__ void k_zero_u32(u32 *p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -326,10 +350,9 @@ size_t carry_tiles(size_t n_limbs) { return (n_limbs + CARRY_TILE - 1) / CARRY_T
 int launch_carry(const u64 *Ls, const u64 *Hs, u64 *out, size_t n_limbs, unsigned char *tile_gp,
                  unsigned char *tile_cin, u32 *out_len, hipStream_t st) {
     size_t nt = carry_tiles(n_limbs);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, out_len, 1);
     hipLaunchKernelGGL(k_carry_tiles, dim3((unsigned)nt), dim3(CARRY_BLOCK), 0, st, Ls, Hs, out, n_limbs, tile_gp,
                        out_len);
-    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, tile_gp, tile_cin, nt);
+    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, tile_gp, tile_cin, nt, out, n_limbs, out_len);
     hipLaunchKernelGGL(k_carry_ripple, dim3((unsigned)nt), dim3(CARRY_RIPPLE_BLOCK), 0, st, out, n_limbs, tile_cin,
                        out_len);
     return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -88,6 +88,21 @@ def test_unnormalised_inputs_trimmed(ctx, orc):
     assert np.array_equal(got, orc.mul_mag_ntt(a, b))
 
 
+@pytest.mark.parametrize("top", [0, 2046, 2047, 2048, 4095, 6000, 16383])
+def test_zero_top_limbs_trimmed(ctx, orc, top):
+    """Products whose highest nonzero limb `top` lies below the last 2048-limb
+    carry tile (every operand padded with thousands of zero limbs): the
+    trimmed length is top + 1, also at and around tile boundaries."""
+    log(f"zero top limbs, top={top}")
+    ia = top // 3
+    a = np.zeros(ia + 9000, dtype=np.uint64); a[ia] = 3; a[:ia] = orc.splitmix64(top + 1, ia)
+    b = np.zeros(top - ia + 7000, dtype=np.uint64); b[top - ia] = 5
+    got = ctx.mul_mag_ntt(a, b)
+    assert got.size == top + 1   # limb `top` of the product is a[ia] * 5 plus a carry below 5: nonzero, no overflow
+    assert np.array_equal(got, orc.mul_mag_ntt(a, b))
+    assert orc.to_int(got) == orc.to_int(a) * orc.to_int(b)
+
+
 @pytest.mark.parametrize("la,lb", [(1000, 3), (1, 100000), (3, 70000), (12345, 54321), (8191, 8193)])
 def test_unbalanced(ctx, orc, la, lb):
     log(f"unbalanced {la}x{lb}")
