@@ -125,11 +125,27 @@ __global__ __launch_bounds__(CARRY_BLOCK) void k_carry_tiles(const u64 *__restri
     }
 }
 
+// 16 consecutive tile flags from p[c..c+16) (bytes past n read as 0) in one
+// 16-byte load when they are all inside and p + c is 16-byte aligned.
+__device__ __forceinline__ uint4 flags16(const unsigned char *__restrict__ p, size_t c, size_t n) {
+    if (c + 16 <= n && ((uintptr_t)(p + c) & 15) == 0) return *(const uint4 *)(p + c);
+    u32 w[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 16 && c + k < n; k++) w[k >> 2] |= (u32)p[c + k] << (8 * (k & 3));
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ u32 flag_byte(const uint4 &v, int k) {
+    const u32 w = k < 4 ? v.x : k < 8 ? v.y : k < 12 ? v.z : v.w;
+    return (w >> (8 * (k & 3))) & 0xFF;
+}
+
 // One workgroup: exclusive carry-in of every tile, and the trimmed length
 // before the ripple when the highest flagged tile is not the last (the
 // product's top 2048 limbs are zero): 1 + the highest nonzero limb of that
 // tile, read back from `out`.  A product's carry never leaves its top limb,
-// so the ripple can only raise that length.
+// so the ripple can only raise that length.  Each thread owns a run of
+// consecutive tiles, a multiple of 16 long, and reads their flags 16 at a
+// time (one load round trip per 16 tiles instead of one per tile).
 constexpr int SCAN_BLOCK = 1024;
 static_assert(CARRY_TILE == 2 * SCAN_BLOCK, "the length search reads two limbs per thread");
 __global__ __launch_bounds__(SCAN_BLOCK) void k_carry_scan(const unsigned char *__restrict__ tile_gp,
@@ -138,18 +154,25 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_carry_scan(const unsigned char *
                                                            u32 *__restrict__ out_len) {
     __shared__ u32 scan[SCAN_BLOCK];
     __shared__ u32 top_tile, top_limb;
-    const size_t per = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    const size_t t0 = (size_t)threadIdx.x * per;
+    const size_t per = (ntiles + 16 * SCAN_BLOCK - 1) / (16 * SCAN_BLOCK) * 16;
+    const size_t t0 = (size_t)threadIdx.x * per, t1 = min(t0 + per, ntiles);
     u32 agg = 2, top = 0;
-    for (size_t t = t0; t < t0 + per && t < ntiles; t++) {
-        const u32 f = tile_gp[t];
-        agg = gp_combine(agg, f);
-        if (f & TILE_NONZERO) top = (u32)t + 1;
+    for (size_t c = t0; c < t1; c += 16) {
+        const uint4 v = flags16(tile_gp, c, t1);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (c + k >= t1) break;
+            const u32 f = flag_byte(v, k);
+            agg = gp_combine(agg, f);
+            if (f & TILE_NONZERO) top = (u32)(c + k) + 1;
+        }
     }
     scan[threadIdx.x] = agg;
     if (threadIdx.x == 0) top_tile = top_limb = 0;
     __syncthreads();
-    if (top) atomicMax(&top_tile, top);
+    // the highest thread with a flagged tile holds the top one
+    const uint64_t has = __ballot(top != 0);
+    if (has && (int)(threadIdx.x & 63) == 63 - __clzll((long long)has)) atomicMax(&top_tile, top);
     for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
         u32 v = scan[threadIdx.x];
         u32 w = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 2u;
@@ -158,10 +181,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_carry_scan(const unsigned char *
         __syncthreads();
     }
     u32 carry = threadIdx.x > 0 ? (scan[threadIdx.x - 1] & 1) : 0;
-    for (size_t t = t0; t < t0 + per && t < ntiles; t++) {
-        tile_cin[t] = (unsigned char)carry;
-        u32 gp = tile_gp[t];
-        carry = (gp & 1) | ((gp >> 1) & 1 & carry);
+    for (size_t c = t0; c < t1; c += 16) {
+        const uint4 v = flags16(tile_gp, c, t1);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (c + k >= t1) break;
+            tile_cin[c + k] = (unsigned char)carry;
+            const u32 gp = flag_byte(v, k);
+            carry = (gp & 1) | ((gp >> 1) & 1 & carry);
+        }
     }
     const u32 tt = top_tile;  // final: written before the scan loop's barriers
     if (tt && tt < ntiles) {   // uniform
